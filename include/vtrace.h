@@ -1,0 +1,123 @@
+/* vtrace.h — C ABI of librender_hip, the MI355X (gfx950) drop-in for vtrace's C library.
+ *
+ * The first seven entry points are the exports the reference's Rust host binds over FFI
+ * (src/render.rs:110-128, declared in lib/common.h:253-357) with the same signatures,
+ * argument meaning and return conventions.  The vt_* functions are additive headless
+ * extensions (resolution, render mode, index+palette volumes, read-back, statistics).
+ *
+ * Threading: like the reference, the library keeps global state and is meant to be
+ * called by one thread at a time; calls may come from different OS threads (the
+ * reference's host spawns a render thread per frame, src/main.rs:44-54).  Every export
+ * selects the library's device itself and serialises on an internal lock.
+ */
+#ifndef VTRACE_H
+#define VTRACE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* lib/common.h:89-92 — pointers to 16 floats each (column-major mat4), read during the call */
+typedef struct render_tick_info {
+    void* perspective;
+    void* camera;
+} render_tick_info;
+
+/* lib/common.h:145-151 (== src/render.rs:37-51) */
+typedef struct user_input {
+    uint8_t keys[6];
+    double mouse_x;
+    double mouse_y;
+    double last_mouse_x;
+    double last_mouse_y;
+} user_input;
+
+/* Replaces lib/entry.c:54-57 (entry -> init, lib/entry.c:59-98).  0 on success, else
+ * ((uint64_t)hipError << 32) | custom_code, the reference's (VkResult << 32) | custom. */
+uint64_t entry(void);
+
+/* Replaces lib/entry.c:150-248.  Renders one frame with info->perspective and
+ * info->camera (16 floats each, read during the call only).  Writes the frame size to
+ * *window_width / *window_height.  0 = OK, -1 = error (the reference also returns -1
+ * when its window closes; a headless library has no window). */
+int32_t render_tick(int32_t* window_width, int32_t* window_height, const render_tick_info* render_tick_info);
+
+/* Replaces lib/entry.c:146-148.  Library-owned, valid until cleanup(); headless, so the
+ * fields stay zero unless the caller writes them. */
+user_input* get_input_data_pointer(void);
+
+/* Replaces lib/entry.c:100-144.  Waits for the device and frees everything. */
+void cleanup(void);
+
+/* Replaces lib/memory.c:286-385.  Uploads a width x height x depth RGBA8 volume (x fastest,
+ * alpha > 0 = occupied) into 8^3 bricks.  The data is copied during the call.  Returns the
+ * texture id (ids are consecutive from 0) or -1 (MAX_TEXTURES = 65536 reached, zero size,
+ * allocation failure). */
+int32_t add_texture(const uint8_t* data, uint32_t width, uint32_t height, uint32_t depth);
+
+/* Replaces lib/memory.c:235-248.  Returns a host-writable buffer for instance_count records
+ * of 16 floats (column-major model matrix, texture id bit-cast into element 15), NULL on
+ * failure.  instance_count == 0 is treated as 1, like the reference. */
+float* start_update_instances(uint32_t instance_count);
+
+/* Replaces lib/memory.c:250-267.  Commits instance_count records (0 -> 1) and rebuilds the
+ * LBVH on the device.  0 or -1. */
+int32_t end_update_instances(uint32_t instance_count);
+
+/* ---------------- headless extensions (no reference counterpart) ---------------------- */
+
+typedef struct vt_render_params {
+    uint32_t width, height;     /* frame size (default 1000 x 1000, lib/entry.c:64-65) */
+    uint32_t viewport_square;   /* 1: the reference's W x W viewport (lib/command.c:77-84) */
+    uint32_t mode;              /* 0 primary colour, 1 primary + sun shadow ray, 2 path tracing */
+    uint32_t spp;
+    uint32_t max_bounces;
+    uint32_t jitter;            /* 0: pixel centres, 1: hash-RNG jitter inside the pixel */
+    uint32_t seed;
+    uint32_t frame;
+    float sun_dir[3];           /* unit vector towards the sun */
+    float sun_radiance[3];
+    float sky[3];               /* sky radiance; default = the reference clear colour */
+    uint32_t stats;             /* 1: count traversal work (nodes, bricks, ...) */
+} vt_render_params;
+
+int32_t vt_set_params(const vt_render_params* params);
+int32_t vt_get_params(vt_render_params* params);
+
+/* u8 palette-index volume + 256 x RGBA8 palette (alpha > 0 = occupied).  Same id space as
+ * add_texture. */
+int32_t vt_add_volume_indexed(const uint8_t* indices, uint32_t width, uint32_t height, uint32_t depth,
+                              const uint32_t* palette);
+
+/* Read-back of the last frame (synchronises).  bytes must be >= width*height*16 (hits and
+ * radiance) or width*height*4 (B8G8R8A8 sRGB, the reference's swapchain format). */
+int32_t vt_read_hits(void* dst, uint64_t bytes);
+int32_t vt_read_radiance(float* dst, uint64_t bytes);
+int32_t vt_read_bgra8(uint32_t* dst, uint64_t bytes);
+/* 7 counters of the last frame rendered with stats = 1: rays, LBVH nodes, instances,
+ * bricks, non-empty bricks, hits, DDA steps. */
+int32_t vt_read_stats(uint64_t* dst7);
+/* GPU time of the last render_tick's trace kernel (HIP events on the library stream). */
+float vt_last_frame_ms(void);
+/* Renders `iters` frames after `warmup` with the last camera; per-frame kernel time from
+ * HIP events on the library stream.  Writes the mean kernel ms and the wall ms per frame. */
+int32_t vt_bench_frames(uint32_t warmup, uint32_t iters, float* kernel_ms, float* wall_ms);
+
+/* Dense read-back of a bricked volume in voxel order (u8 indices or RGBA8), for round
+ * trips; info8 = {w, h, d, fmt, non-empty bricks, bytes lo, bytes hi, 0}. */
+int32_t vt_read_volume(int32_t id, void* dst, uint64_t bytes);
+int32_t vt_volume_info(int32_t id, uint32_t* info8);
+/* LBVH of the current instances: (2n-1) x 32 B nodes and n x 128 B instance records. */
+int32_t vt_read_lbvh(void* nodes, uint64_t node_bytes, void* instances, uint64_t inst_bytes);
+int32_t vt_instance_count(void);
+int32_t vt_synchronize(void);
+const char* vt_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VTRACE_H */

@@ -1,0 +1,616 @@
+// abi.hip — the C-ABI boundary of librender_hip (include/vtrace.h) and its host runtime.
+//
+// Mirrors the reference's exported interface (src/render.rs:110-128, lib/entry.c,
+// lib/memory.c) with a HIP runtime underneath: one device, one library stream (default
+// priority), device-resident bricked volumes, a device instance buffer and an LBVH
+// rebuilt on the stream whenever instances are committed.  The Vulkan objects of the
+// reference (swapchain, descriptor sets, render pass, fences) have no counterpart here;
+// stream order replaces the secondary command queue and its ordering passes
+// (lib/command.c:111-251).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vtrace.h"
+#include "vt_internal.h"
+
+void vt_trace_set_tables(const float* dec_dev, const float* thr_dev);
+
+namespace {
+
+constexpr uint32_t kMaxTextures = 65536;  // lib/common.h:35
+
+struct State {
+    bool inited = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    user_input input{};
+    // instances (lib/memory.c:221-267)
+    float* inst_host = nullptr;    // pinned staging
+    float* inst_dev = nullptr;
+    uint32_t inst_capacity = 0;
+    uint32_t inst_count = 0;
+    hipEvent_t inst_copied = nullptr;
+    LbvhWork bvh;
+    // volumes (lib/memory.c:286-385)
+    std::vector<VolumeStore> vols;
+    VolumeDesc* vol_table = nullptr;  // kMaxTextures descriptors
+    uint8_t* staging = nullptr;
+    size_t staging_size = 0;
+    uint32_t* palette_rgba_identity = nullptr;
+    uint16_t* remap = nullptr;        // 3 tables
+    size_t remap_cap = 0;
+    // frame
+    vt_render_params params{};
+    HitRecord* hits = nullptr;
+    float4* rad = nullptr;
+    uint32_t* bgra = nullptr;
+    size_t frame_cap = 0;
+    unsigned long long* stats = nullptr;
+    float* srgb_dec = nullptr;
+    float* srgb_thr = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool have_frame = false;
+    FrameGPU last_frame{};
+    float last_ms = 0.0f;
+    uint64_t frames = 0;
+    std::string err;
+};
+
+State g;
+std::mutex g_lock;
+
+void set_err(const char* what, hipError_t e = hipSuccess) {
+    g.err = what;
+    if (e != hipSuccess) {
+        g.err += ": ";
+        g.err += hipGetErrorString(e);
+    }
+}
+
+// ---- host math (frame constants) -------------------------------------------------
+// FP32 products in GLSL order (m[c*4 + r]); compiled with -ffp-contract=off.
+void mm4(const float* a, const float* b, float* o) {
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r)
+            o[c * 4 + r] = ((a[0 * 4 + r] * b[c * 4 + 0] + a[1 * 4 + r] * b[c * 4 + 1]) + a[2 * 4 + r] * b[c * 4 + 2]) +
+                           a[3 * 4 + r] * b[c * 4 + 3];
+}
+void mv4(const float* m, const float v[4], float o[4]) {
+    for (int r = 0; r < 4; ++r)
+        o[r] = ((m[0 * 4 + r] * v[0] + m[1 * 4 + r] * v[1]) + m[2 * 4 + r] * v[2]) + m[3 * 4 + r] * v[3];
+}
+
+double srgb_lin(double c) { return c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4); }
+
+void default_params(vt_render_params& p) {
+    std::memset(&p, 0, sizeof(p));
+    p.width = 1000;  // lib/entry.c:64-65
+    p.height = 1000;
+    p.viewport_square = 1;
+    p.mode = 0;
+    p.spp = 1;
+    p.max_bounces = 0;
+    p.jitter = 0;
+    p.sun_dir[0] = 0.0f; p.sun_dir[1] = -1.0f; p.sun_dir[2] = 0.0f;
+    p.sun_radiance[0] = p.sun_radiance[1] = p.sun_radiance[2] = 2.5f;
+    p.sky[0] = 0.53f; p.sky[1] = 0.81f; p.sky[2] = 0.92f;  // lib/command.c:57-60
+}
+
+int ensure_frame_buffers() {
+    size_t n = (size_t)g.params.width * g.params.height;
+    if (n <= g.frame_cap && g.hits) return 0;
+    hipFree(g.hits); hipFree(g.rad); hipFree(g.bgra);
+    g.hits = nullptr; g.rad = nullptr; g.bgra = nullptr;
+    hipError_t e;
+    if ((e = hipMalloc(&g.hits, n * sizeof(HitRecord))) != hipSuccess ||
+        (e = hipMalloc(&g.rad, n * sizeof(float4))) != hipSuccess || (e = hipMalloc(&g.bgra, n * 4)) != hipSuccess) {
+        set_err("frame buffer allocation", e);
+        g.frame_cap = 0;
+        return -1;
+    }
+    g.frame_cap = n;
+    return 0;
+}
+
+int grow_instances(uint32_t n) {
+    if (n <= g.inst_capacity && g.inst_host) return 0;
+    uint32_t cap = 1;
+    while (cap < n + 1) cap *= 2;  // round_up_p2(n + 1), lib/memory.c:191
+    if (g.inst_copied) hipEventSynchronize(g.inst_copied);
+    hipHostFree(g.inst_host);
+    hipFree(g.inst_dev);
+    g.inst_host = nullptr;
+    g.inst_dev = nullptr;
+    hipError_t e;
+    if ((e = hipHostMalloc(&g.inst_host, (size_t)cap * 64, hipHostMallocDefault)) != hipSuccess ||
+        (e = hipMalloc(&g.inst_dev, (size_t)cap * 64)) != hipSuccess) {
+        set_err("instance buffer allocation", e);
+        g.inst_capacity = 0;
+        return -1;
+    }
+    std::memset(g.inst_host, 0, (size_t)cap * 64);
+    g.inst_capacity = cap;
+    return 0;
+}
+
+// Nearest-texel remap per axis: the texel NEAREST + CLAMP_TO_EDGE sampling of voxel/size
+// selects (shaders/trace.frag:77, lib/descriptor.c:100-105), FP32.
+uint16_t nearest_texel(int v, uint32_t dim) {
+    float size = (float)dim;
+    float u = (float)v / size;
+    float x = u * size;
+    int t = (int)std::floor(x);
+    if (t < 0) t = 0;
+    if (t > (int)dim - 1) t = (int)dim - 1;
+    return (uint16_t)t;
+}
+
+int upload_volume(const void* data, size_t bytes, uint32_t fmt, const uint32_t dim[3], const uint32_t* palette) {
+    if (g.vols.size() >= kMaxTextures) {
+        std::fprintf(stderr, "ERROR: Tried allocating too many textures\n");  // lib/memory.c:288
+        set_err("MAX_TEXTURES reached");
+        return -1;
+    }
+    if (!data || dim[0] == 0 || dim[1] == 0 || dim[2] == 0 || dim[0] > 65535 || dim[1] > 65535 || dim[2] > 65535) {
+        set_err("invalid volume");
+        return -1;
+    }
+    hipError_t e;
+    if (bytes > g.staging_size) {  // grow to a power of two (lib/memory.c:297-302)
+        size_t sz = 1;
+        while (sz < bytes) sz *= 2;
+        hipFree(g.staging);
+        g.staging = nullptr;
+        g.staging_size = 0;
+        if ((e = hipMalloc(&g.staging, sz)) != hipSuccess) {
+            set_err("staging allocation", e);
+            return -1;
+        }
+        g.staging_size = sz;
+    }
+    if ((e = hipMemcpyAsync(g.staging, data, bytes, hipMemcpyHostToDevice, g.stream)) != hipSuccess) {
+        set_err("volume copy", e);
+        return -1;
+    }
+    size_t rc = (size_t)dim[0] + dim[1] + dim[2];
+    if (rc > g.remap_cap) {
+        hipFree(g.remap);
+        if ((e = hipMalloc(&g.remap, rc * 2)) != hipSuccess) {
+            set_err("remap allocation", e);
+            g.remap_cap = 0;
+            return -1;
+        }
+        g.remap_cap = rc;
+    }
+    std::vector<uint16_t> rm(rc);
+    size_t off = 0;
+    for (int a = 0; a < 3; ++a)
+        for (uint32_t i = 0; i < dim[a]; ++i) rm[off++] = nearest_texel((int)i, dim[a]);
+    hipMemcpyAsync(g.remap, rm.data(), rc * 2, hipMemcpyHostToDevice, g.stream);
+    VolumeStore vs;
+    if (fmt == 0) {
+        if ((e = hipMalloc(&vs.palette, 1024)) != hipSuccess) {
+            set_err("palette allocation", e);
+            return -1;
+        }
+        hipMemcpyAsync(vs.palette, palette, 1024, hipMemcpyHostToDevice, g.stream);
+    }
+    const uint16_t* remaps[3] = {g.remap, g.remap + dim[0], g.remap + dim[0] + dim[1]};
+    if (vt_brick_volume(g.staging, fmt, vs.palette, remaps, dim, &vs, g.stream) != 0) {
+        hipFree(vs.palette);
+        set_err("bricking failed");
+        return -1;
+    }
+    VolumeDesc d{};
+    for (int a = 0; a < 3; ++a) { d.dim[a] = vs.dim[a]; d.bdim[a] = vs.bdim[a]; }
+    d.fmt = vs.fmt;
+    d.nbricks = vs.nbricks;
+    d.brick_map = vs.brick_map;
+    d.masks = vs.masks;
+    d.payload = vs.payload;
+    d.palette = vs.palette;
+    int32_t id = (int32_t)g.vols.size();
+    hipMemcpyAsync(&g.vol_table[id], &d, sizeof(d), hipMemcpyHostToDevice, g.stream);
+    if ((e = hipStreamSynchronize(g.stream)) != hipSuccess) {
+        set_err("upload", e);
+        return -1;
+    }
+    g.vols.push_back(vs);
+    return id;
+}
+
+bool build_frame(const float* P, const float* C, FrameGPU& F) {
+    float invP[16], invC[16], invCc[16], Cc[16];
+    std::memcpy(Cc, C, 64);
+    Cc[12] = Cc[13] = Cc[14] = 0.0f;  // centered_camera (shaders/trace.frag:55-58)
+    if (!vt_inverse4(P, invP) || !vt_inverse4(C, invC) || !vt_inverse4(Cc, invCc)) return false;
+    mm4(P, C, F.PC);
+    mm4(invCc, invP, F.AB);
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) F.R[c * 3 + r] = invCc[c * 4 + r];
+    float z[4] = {0.0f, 0.0f, 0.0f, 1.0f}, o[4];
+    mv4(invC, z, o);  // cam_pos (shaders/trace.frag:51)
+    F.O[0] = o[0]; F.O[1] = o[1]; F.O[2] = o[2];
+    F.ip00 = invP[0];
+    F.ip11 = invP[5];
+    const vt_render_params& p = g.params;
+    F.W = (float)p.width;
+    F.Hv = p.viewport_square ? (float)p.width : (float)p.height;
+    F.width = p.width;
+    F.height = p.height;
+    F.mode = p.mode;
+    F.spp = p.spp ? p.spp : 1;
+    F.max_bounces = p.max_bounces;
+    F.jitter = p.jitter;
+    F.seed = p.seed;
+    F.frame = p.frame;
+    for (int k = 0; k < 3; ++k) {
+        F.sun[k] = p.sun_dir[k];
+        F.sun_rad[k] = p.sun_radiance[k];
+        F.sky[k] = p.sky[k];
+    }
+    F.stats = p.stats;
+    return true;
+}
+
+SceneGPU scene_gpu() {
+    SceneGPU S;
+    S.inst = g.bvh.inst;
+    S.nodes = g.bvh.nodes;
+    S.vols = g.vol_table;
+    S.ninst = g.inst_count;
+    S.nvol = (uint32_t)g.vols.size();
+    return S;
+}
+
+int render_frame(const FrameGPU& F) {
+    if (ensure_frame_buffers() != 0) return -1;
+    if (F.stats) hipMemsetAsync(g.stats, 0, VT_STAT_COUNT * 8, g.stream);
+    hipEventRecord(g.ev0, g.stream);
+    int rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.stats, g.stream);
+    hipEventRecord(g.ev1, g.stream);
+    if (rc != 0) {
+        set_err("trace launch failed", hipGetLastError());
+        return -1;
+    }
+    g.last_frame = F;
+    g.have_frame = true;
+    return 0;
+}
+
+struct Guard {
+    std::lock_guard<std::mutex> lk;
+    Guard() : lk(g_lock) {
+        if (g.inited) hipSetDevice(g.device);
+    }
+};
+
+}  // namespace
+
+// ---- host FP64 inverse (Laplace expansion over 2x2 minors, Eberly 2008) -------------
+bool vt_inverse4(const float* mf, float* out) {
+    auto A = [&](int r, int c) -> double { return (double)mf[c * 4 + r]; };
+    double s0 = A(0, 0) * A(1, 1) - A(1, 0) * A(0, 1);
+    double s1 = A(0, 0) * A(1, 2) - A(1, 0) * A(0, 2);
+    double s2 = A(0, 0) * A(1, 3) - A(1, 0) * A(0, 3);
+    double s3 = A(0, 1) * A(1, 2) - A(1, 1) * A(0, 2);
+    double s4 = A(0, 1) * A(1, 3) - A(1, 1) * A(0, 3);
+    double s5 = A(0, 2) * A(1, 3) - A(1, 2) * A(0, 3);
+    double c5 = A(2, 2) * A(3, 3) - A(3, 2) * A(2, 3);
+    double c4 = A(2, 1) * A(3, 3) - A(3, 1) * A(2, 3);
+    double c3 = A(2, 1) * A(3, 2) - A(3, 1) * A(2, 2);
+    double c2 = A(2, 0) * A(3, 3) - A(3, 0) * A(2, 3);
+    double c1 = A(2, 0) * A(3, 2) - A(3, 0) * A(2, 2);
+    double c0 = A(2, 0) * A(3, 1) - A(3, 0) * A(2, 1);
+    double det = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+    if (!(det != 0.0) || !std::isfinite(det)) return false;
+    double b[16];  // b[r*4 + c]
+    b[0] = (A(1, 1) * c5 - A(1, 2) * c4 + A(1, 3) * c3) / det;
+    b[1] = (-A(0, 1) * c5 + A(0, 2) * c4 - A(0, 3) * c3) / det;
+    b[2] = (A(3, 1) * s5 - A(3, 2) * s4 + A(3, 3) * s3) / det;
+    b[3] = (-A(2, 1) * s5 + A(2, 2) * s4 - A(2, 3) * s3) / det;
+    b[4] = (-A(1, 0) * c5 + A(1, 2) * c2 - A(1, 3) * c1) / det;
+    b[5] = (A(0, 0) * c5 - A(0, 2) * c2 + A(0, 3) * c1) / det;
+    b[6] = (-A(3, 0) * s5 + A(3, 2) * s2 - A(3, 3) * s1) / det;
+    b[7] = (A(2, 0) * s5 - A(2, 2) * s2 + A(2, 3) * s1) / det;
+    b[8] = (A(1, 0) * c4 - A(1, 1) * c2 + A(1, 3) * c0) / det;
+    b[9] = (-A(0, 0) * c4 + A(0, 1) * c2 - A(0, 3) * c0) / det;
+    b[10] = (A(3, 0) * s4 - A(3, 1) * s2 + A(3, 3) * s0) / det;
+    b[11] = (-A(2, 0) * s4 + A(2, 1) * s2 - A(2, 3) * s0) / det;
+    b[12] = (-A(1, 0) * c3 + A(1, 1) * c1 - A(1, 2) * c0) / det;
+    b[13] = (A(0, 0) * c3 - A(0, 1) * c1 + A(0, 2) * c0) / det;
+    b[14] = (-A(3, 0) * s3 + A(3, 1) * s1 - A(3, 2) * s0) / det;
+    b[15] = (A(2, 0) * s3 - A(2, 1) * s1 + A(2, 2) * s0) / det;
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            float v = (float)b[r * 4 + c];
+            if (!std::isfinite(v)) return false;
+            out[c * 4 + r] = v;
+        }
+    return true;
+}
+
+extern "C" {
+
+uint64_t entry(void) {
+    std::lock_guard<std::mutex> lk(g_lock);
+    if (g.inited) return 0;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        set_err("no HIP device", e);
+        return ((uint64_t)(uint32_t)(e == hipSuccess ? hipErrorNoDevice : e) << 32) | 1u;
+    }
+    g.device = 0;
+    if ((e = hipSetDevice(g.device)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&g.ev0)) != hipSuccess || (e = hipEventCreate(&g.ev1)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&g.inst_copied, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipMalloc(&g.vol_table, (size_t)kMaxTextures * sizeof(VolumeDesc))) != hipSuccess ||
+        (e = hipMalloc(&g.stats, VT_STAT_COUNT * 8)) != hipSuccess ||
+        (e = hipMalloc(&g.srgb_dec, 256 * 4)) != hipSuccess || (e = hipMalloc(&g.srgb_thr, 255 * 4)) != hipSuccess) {
+        set_err("init", e);
+        return ((uint64_t)(uint32_t)e << 32) | 1u;
+    }
+    float dec[256], thr[255];
+    for (int i = 0; i < 256; ++i) dec[i] = (float)srgb_lin((double)i / 255.0);
+    for (int k = 0; k < 255; ++k) thr[k] = (float)srgb_lin((k + 0.5) / 255.0);
+    hipMemcpy(g.srgb_dec, dec, sizeof(dec), hipMemcpyHostToDevice);
+    hipMemcpy(g.srgb_thr, thr, sizeof(thr), hipMemcpyHostToDevice);
+    hipMemset(g.stats, 0, VT_STAT_COUNT * 8);
+    vt_trace_set_tables(g.srgb_dec, g.srgb_thr);
+    default_params(g.params);
+    g.inited = true;
+    if (grow_instances(0) != 0) return ((uint64_t)hipErrorOutOfMemory << 32) | 1u;
+    return 0;
+}
+
+int32_t render_tick(int32_t* window_width, int32_t* window_height, const render_tick_info* info) {
+    Guard G;
+    if (!g.inited || !info || !info->perspective || !info->camera) {
+        set_err("render_tick: not initialised or null info");
+        return -1;
+    }
+    FrameGPU F;
+    if (!build_frame(static_cast<const float*>(info->perspective), static_cast<const float*>(info->camera), F)) {
+        set_err("render_tick: singular projection or camera");
+        return -1;
+    }
+    if (render_frame(F) != 0) return -1;
+    hipEventSynchronize(g.ev1);
+    hipEventElapsedTime(&g.last_ms, g.ev0, g.ev1);
+    ++g.frames;
+    if (window_width) *window_width = (int32_t)g.params.width;
+    if (window_height) *window_height = (int32_t)g.params.height;
+    return 0;
+}
+
+user_input* get_input_data_pointer(void) { return &g.input; }
+
+void cleanup(void) {
+    std::lock_guard<std::mutex> lk(g_lock);
+    if (!g.inited) return;
+    hipSetDevice(g.device);
+    hipStreamSynchronize(g.stream);
+    for (auto& v : g.vols) {
+        hipFree(v.brick_map); hipFree(v.masks); hipFree(v.payload); hipFree(v.palette);
+    }
+    g.vols.clear();
+    vt_lbvh_free(g.bvh);
+    hipHostFree(g.inst_host);
+    hipFree(g.inst_dev); hipFree(g.vol_table); hipFree(g.staging); hipFree(g.remap);
+    hipFree(g.hits); hipFree(g.rad); hipFree(g.bgra); hipFree(g.stats); hipFree(g.srgb_dec); hipFree(g.srgb_thr);
+    hipEventDestroy(g.ev0); hipEventDestroy(g.ev1); hipEventDestroy(g.inst_copied);
+    hipStreamDestroy(g.stream);
+    vt_trace_set_tables(nullptr, nullptr);
+    g = State{};
+}
+
+int32_t add_texture(const uint8_t* data, uint32_t width, uint32_t height, uint32_t depth) {
+    Guard G;
+    if (!g.inited) return -1;
+    const uint32_t dim[3] = {width, height, depth};
+    return upload_volume(data, (size_t)4 * width * height * depth, 1, dim, nullptr);
+}
+
+float* start_update_instances(uint32_t instance_count) {
+    Guard G;
+    if (!g.inited) return nullptr;
+    if (instance_count == 0) instance_count = 1;  // lib/memory.c:236
+    if (instance_count > g.inst_capacity && grow_instances(instance_count) != 0) return nullptr;
+    hipEventSynchronize(g.inst_copied);  // previous commit has left the staging buffer
+    return g.inst_host;
+}
+
+int32_t end_update_instances(uint32_t instance_count) {
+    Guard G;
+    if (!g.inited) return -1;
+    if (instance_count == 0) instance_count = 1;  // lib/memory.c:251
+    if (instance_count > g.inst_capacity) {
+        set_err("end_update_instances: more instances than started");
+        return -1;
+    }
+    hipError_t e = hipMemcpyAsync(g.inst_dev, g.inst_host, (size_t)instance_count * 64, hipMemcpyHostToDevice, g.stream);
+    if (e != hipSuccess) {
+        set_err("instance copy", e);
+        return -1;
+    }
+    hipEventRecord(g.inst_copied, g.stream);
+    if (vt_lbvh_build(g.bvh, g.inst_dev, instance_count, g.vol_table, (uint32_t)g.vols.size(), g.stream) != 0) {
+        set_err("lbvh build", hipGetLastError());
+        return -1;
+    }
+    g.inst_count = instance_count;
+    return 0;
+}
+
+int32_t vt_set_params(const vt_render_params* p) {
+    Guard G;
+    if (!g.inited || !p || p->width == 0 || p->height == 0 || p->width > 16384 || p->height > 16384 || p->mode > 2 ||
+        p->spp == 0 || p->spp > 65536) {
+        set_err("vt_set_params: invalid parameters");
+        return -1;
+    }
+    g.params = *p;
+    return ensure_frame_buffers();
+}
+
+int32_t vt_get_params(vt_render_params* p) {
+    Guard G;
+    if (!g.inited || !p) return -1;
+    *p = g.params;
+    return 0;
+}
+
+int32_t vt_add_volume_indexed(const uint8_t* indices, uint32_t width, uint32_t height, uint32_t depth,
+                              const uint32_t* palette) {
+    Guard G;
+    if (!g.inited || !palette) return -1;
+    const uint32_t dim[3] = {width, height, depth};
+    return upload_volume(indices, (size_t)width * height * depth, 0, dim, palette);
+}
+
+static int32_t read_frame(const void* src, void* dst, uint64_t bytes, size_t elem) {
+    Guard G;
+    if (!g.inited || !g.have_frame || !dst) return -1;
+    size_t need = (size_t)g.last_frame.width * g.last_frame.height * elem;
+    if (bytes < need) {
+        set_err("read-back buffer too small");
+        return -1;
+    }
+    hipError_t e = hipMemcpyAsync(dst, src, need, hipMemcpyDeviceToHost, g.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+    if (e != hipSuccess) {
+        set_err("read-back", e);
+        return -1;
+    }
+    return 0;
+}
+
+int32_t vt_read_hits(void* dst, uint64_t bytes) { return read_frame(g.hits, dst, bytes, sizeof(HitRecord)); }
+int32_t vt_read_radiance(float* dst, uint64_t bytes) { return read_frame(g.rad, dst, bytes, sizeof(float4)); }
+
+int32_t vt_read_bgra8(uint32_t* dst, uint64_t bytes) {
+    {
+        Guard G;
+        if (!g.inited || !g.have_frame) return -1;
+        uint32_t n = g.last_frame.width * g.last_frame.height;
+        if (vt_encode_srgb8(reinterpret_cast<const float4*>(g.rad), g.bgra, n, g.stream) != 0) return -1;
+    }
+    return read_frame(g.bgra, dst, bytes, 4);
+}
+
+int32_t vt_read_stats(uint64_t* dst7) {
+    Guard G;
+    if (!g.inited || !dst7) return -1;
+    hipMemcpyAsync(dst7, g.stats, VT_STAT_COUNT * 8, hipMemcpyDeviceToHost, g.stream);
+    return hipStreamSynchronize(g.stream) == hipSuccess ? 0 : -1;
+}
+
+float vt_last_frame_ms(void) { return g.last_ms; }
+
+int32_t vt_bench_frames(uint32_t warmup, uint32_t iters, float* kernel_ms, float* wall_ms) {
+    Guard G;
+    if (!g.inited || !g.have_frame || iters == 0) return -1;
+    FrameGPU F = g.last_frame;
+    F.stats = 0;
+    for (uint32_t i = 0; i < warmup; ++i)
+        if (render_frame(F) != 0) return -1;
+    hipStreamSynchronize(g.stream);
+    std::vector<hipEvent_t> ev(2 * (size_t)iters);
+    for (auto& e : ev) hipEventCreate(&e);
+    hipEvent_t start, stop;
+    hipEventCreate(&start);
+    hipEventCreate(&stop);
+    hipEventRecord(start, g.stream);
+    int rc = 0;
+    for (uint32_t i = 0; i < iters && rc == 0; ++i) {
+        hipEventRecord(ev[2 * i], g.stream);
+        rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.stats, g.stream);
+        hipEventRecord(ev[2 * i + 1], g.stream);
+    }
+    hipEventRecord(stop, g.stream);
+    hipEventSynchronize(stop);
+    float total = 0.0f, k = 0.0f;
+    hipEventElapsedTime(&total, start, stop);
+    for (uint32_t i = 0; i < iters; ++i) {
+        float t = 0.0f;
+        hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]);
+        k += t;
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    hipEventDestroy(start);
+    hipEventDestroy(stop);
+    if (kernel_ms) *kernel_ms = k / (float)iters;
+    if (wall_ms) *wall_ms = total / (float)iters;
+    return rc;
+}
+
+int32_t vt_volume_info(int32_t id, uint32_t* info8) {
+    Guard G;
+    if (!g.inited || id < 0 || (size_t)id >= g.vols.size() || !info8) return -1;
+    const VolumeStore& v = g.vols[id];
+    info8[0] = v.dim[0]; info8[1] = v.dim[1]; info8[2] = v.dim[2];
+    info8[3] = v.fmt;
+    info8[4] = v.nbricks;
+    info8[5] = (uint32_t)(v.bytes & 0xFFFFFFFFu);
+    info8[6] = (uint32_t)(v.bytes >> 32);
+    info8[7] = 0;
+    return 0;
+}
+
+int32_t vt_read_volume(int32_t id, void* dst, uint64_t bytes) {
+    Guard G;
+    if (!g.inited || id < 0 || (size_t)id >= g.vols.size() || !dst) return -1;
+    const VolumeStore& v = g.vols[id];
+    size_t need = (size_t)v.dim[0] * v.dim[1] * v.dim[2] * (v.fmt == 0 ? 1 : 4);
+    if (bytes < need) return -1;
+    uint8_t* tmp = nullptr;
+    if (hipMalloc(&tmp, need) != hipSuccess) return -1;
+    VolumeDesc d{};
+    for (int a = 0; a < 3; ++a) { d.dim[a] = v.dim[a]; d.bdim[a] = v.bdim[a]; }
+    d.fmt = v.fmt;
+    d.nbricks = v.nbricks;
+    d.brick_map = v.brick_map;
+    d.masks = v.masks;
+    d.payload = v.payload;
+    d.palette = v.palette;
+    int rc = vt_debrick(d, tmp, g.stream);
+    if (rc == 0) hipMemcpyAsync(dst, tmp, need, hipMemcpyDeviceToHost, g.stream);
+    if (hipStreamSynchronize(g.stream) != hipSuccess) rc = -1;
+    hipFree(tmp);
+    return rc;
+}
+
+int32_t vt_read_lbvh(void* nodes, uint64_t node_bytes, void* instances, uint64_t inst_bytes) {
+    Guard G;
+    if (!g.inited || g.inst_count == 0) return -1;
+    size_t nn = 2 * (size_t)g.inst_count - 1;
+    if (nodes) {
+        if (node_bytes < nn * sizeof(BvhNode)) return -1;
+        hipMemcpyAsync(nodes, g.bvh.nodes, nn * sizeof(BvhNode), hipMemcpyDeviceToHost, g.stream);
+    }
+    if (instances) {
+        if (inst_bytes < g.inst_count * sizeof(InstanceGPU)) return -1;
+        hipMemcpyAsync(instances, g.bvh.inst, g.inst_count * sizeof(InstanceGPU), hipMemcpyDeviceToHost, g.stream);
+    }
+    return hipStreamSynchronize(g.stream) == hipSuccess ? 0 : -1;
+}
+
+int32_t vt_instance_count(void) { return (int32_t)g.inst_count; }
+
+int32_t vt_synchronize(void) {
+    Guard G;
+    if (!g.inited) return -1;
+    return hipStreamSynchronize(g.stream) == hipSuccess ? 0 : -1;
+}
+
+const char* vt_last_error(void) { return g.err.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,238 @@
+// upload.hip — dense voxel volume -> 8^3 bricks (the GPU half of add_texture).
+//
+// Replaces the reference's staging copy into an R8G8B8A8_SRGB 3-D image
+// (lib/memory.c:286-385, lib/command.c:132-134).  The dense input is the reference's
+// texel order (x fastest).  Voxel v of the bricked volume holds the texel that
+// NEAREST/CLAMP sampling at v/size selects (shaders/trace.frag:77, lib/descriptor.c:100-105);
+// the per-axis remap tables carry that choice (identity for power-of-two sizes).
+//
+// Three kernels:
+//   k_brick_classify  one 512-thread block per brick: occupancy bit per voxel
+//                     (alpha > 0, trace.frag:78), one ballot per z-slice -> 8 x u64 mask;
+//   k_scan_*          exclusive scan of the non-empty flags -> brick ids in brick order;
+//   k_brick_scatter   masks and payload of non-empty bricks into the pools.
+#include <hip/hip_runtime.h>
+
+#include "vt_internal.h"
+
+namespace {
+
+__device__ inline uint32_t texel_rgba(const uint8_t* dense, uint32_t fmt, const uint32_t* palette, size_t ti,
+                                      uint32_t* raw) {
+    if (fmt == 0) {
+        uint32_t i = dense[ti];
+        *raw = i;
+        return palette[i];
+    }
+    uint32_t c = reinterpret_cast<const uint32_t*>(dense)[ti];
+    *raw = c;
+    return c;
+}
+
+struct BrickArgs {
+    const uint8_t* dense;
+    const uint32_t* palette;
+    const uint16_t* remap[3];
+    uint32_t dim[3];
+    uint32_t bdim[3];
+    uint32_t fmt;
+};
+
+__global__ __launch_bounds__(512) void k_brick_classify(BrickArgs a, uint64_t* masks_all, uint32_t* flags) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t bx = b % a.bdim[0];
+    const uint32_t by = (b / a.bdim[0]) % a.bdim[1];
+    const uint32_t bz = b / (a.bdim[0] * a.bdim[1]);
+    const uint32_t t = threadIdx.x;  // x + 8y + 64z inside the brick
+    const uint32_t gx = bx * 8 + (t & 7), gy = by * 8 + ((t >> 3) & 7), gz = bz * 8 + (t >> 6);
+    bool occ = false;
+    if (gx < a.dim[0] && gy < a.dim[1] && gz < a.dim[2]) {
+        size_t ti = (size_t)a.remap[0][gx] + (size_t)a.dim[0] * ((size_t)a.remap[1][gy] + (size_t)a.dim[1] * a.remap[2][gz]);
+        uint32_t raw;
+        occ = (texel_rgba(a.dense, a.fmt, a.palette, ti, &raw) >> 24) != 0u;
+    }
+    uint64_t m = __ballot(occ);  // wave t>>6 holds z-slice t>>6
+    __shared__ uint32_t any;
+    if (t == 0) any = 0;
+    __syncthreads();
+    if ((t & 63) == 0) {
+        masks_all[(size_t)b * 8 + (t >> 6)] = m;
+        if (m) atomicOr(&any, 1u);
+    }
+    __syncthreads();
+    if (t == 0) flags[b] = any;
+}
+
+__global__ __launch_bounds__(512) void k_brick_scatter(BrickArgs a, const uint64_t* masks_all, const uint32_t* ids,
+                                                       const uint32_t* flags, uint32_t* brick_map, uint64_t* masks,
+                                                       uint8_t* payload) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t t = threadIdx.x;
+    const bool nonempty = flags[b] != 0;
+    const uint32_t id = ids[b];
+    if (t == 0) brick_map[b] = nonempty ? id : VT_EMPTY;
+    if (!nonempty) return;
+    if (t < 8) masks[(size_t)id * 8 + t] = masks_all[(size_t)b * 8 + t];
+    const uint32_t bx = b % a.bdim[0];
+    const uint32_t by = (b / a.bdim[0]) % a.bdim[1];
+    const uint32_t bz = b / (a.bdim[0] * a.bdim[1]);
+    const uint32_t gx = bx * 8 + (t & 7), gy = by * 8 + ((t >> 3) & 7), gz = bz * 8 + (t >> 6);
+    uint32_t raw = 0;
+    if (gx < a.dim[0] && gy < a.dim[1] && gz < a.dim[2]) {
+        size_t ti = (size_t)a.remap[0][gx] + (size_t)a.dim[0] * ((size_t)a.remap[1][gy] + (size_t)a.dim[1] * a.remap[2][gz]);
+        (void)texel_rgba(a.dense, a.fmt, a.palette, ti, &raw);  // payload keeps the texel as stored
+    }
+    if (a.fmt == 0) payload[(size_t)id * 512 + t] = (uint8_t)raw;
+    else reinterpret_cast<uint32_t*>(payload)[(size_t)id * 512 + t] = raw;
+}
+
+// ---- exclusive scan of u32 flags (3 phases, 1024 elements per block) ----------------
+constexpr int kScanBlock = 1024;
+
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t wsum[kScanBlock / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        uint32_t s = lane < kScanBlock / 64 ? wsum[lane] : 0;
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(s, o, 64);
+            if (lane >= o) s += y;
+        }
+        if (lane < kScanBlock / 64) wsum[lane] = s;
+    }
+    __syncthreads();
+    uint32_t incl = x + (w > 0 ? wsum[w - 1] : 0u);
+    *total = wsum[kScanBlock / 64 - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t* in, uint32_t n, uint32_t* block_sums) {
+    uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+    uint32_t v = i < n ? (in[i] != 0u ? 1u : 0u) : 0u;
+    uint32_t total;
+    block_exclusive_scan(v, &total);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_blocks(uint32_t* block_sums, uint32_t nb, uint32_t* grand_total) {
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < nb; base += kScanBlock) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t v = i < nb ? block_sums[i] : 0u;
+        uint32_t total;
+        uint32_t ex = block_exclusive_scan(v, &total);
+        uint32_t c = carry;
+        if (i < nb) block_sums[i] = ex + c;
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *grand_total = carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const uint32_t* in, uint32_t n, const uint32_t* block_sums,
+                                                            uint32_t* out) {
+    uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+    uint32_t v = i < n ? (in[i] != 0u ? 1u : 0u) : 0u;
+    uint32_t total;
+    uint32_t ex = block_exclusive_scan(v, &total);
+    if (i < n) out[i] = ex + block_sums[blockIdx.x];
+}
+
+}  // namespace
+
+// Exclusive scan of (flag != 0) over n entries; *total_dev receives the count.
+hipError_t vt_scan_flags(const uint32_t* flags, uint32_t n, uint32_t* out, uint32_t* scratch_blocks, uint32_t* total_dev,
+                         hipStream_t s) {
+    uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
+    if (nb == 0) nb = 1;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, flags, n, scratch_blocks);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanBlock), 0, s, scratch_blocks, nb, total_dev);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBlock), 0, s, flags, n, scratch_blocks, out);
+    return hipGetLastError();
+}
+
+// Brick a dense device volume.  Allocates the pools (after one sync to learn the count).
+int vt_brick_volume(const uint8_t* dense_dev, uint32_t fmt, const uint32_t* palette_dev, const uint16_t* remap_dev[3],
+                    const uint32_t dim[3], VolumeStore* out, hipStream_t s) {
+    BrickArgs a{};
+    a.dense = dense_dev;
+    a.palette = palette_dev;
+    for (int k = 0; k < 3; ++k) {
+        a.remap[k] = remap_dev[k];
+        a.dim[k] = dim[k];
+        a.bdim[k] = (dim[k] + 7) / 8;
+    }
+    a.fmt = fmt;
+    const uint32_t nb = a.bdim[0] * a.bdim[1] * a.bdim[2];
+    uint64_t* masks_all = nullptr;
+    uint32_t *flags = nullptr, *ids = nullptr, *blocks = nullptr, *total = nullptr;
+    const uint32_t nblk = (nb + kScanBlock - 1) / kScanBlock + 1;
+    if (hipMalloc(&masks_all, (size_t)nb * 64) != hipSuccess) return -1;
+    if (hipMalloc(&flags, (size_t)nb * 4) != hipSuccess || hipMalloc(&ids, (size_t)nb * 4) != hipSuccess ||
+        hipMalloc(&blocks, (size_t)nblk * 4) != hipSuccess || hipMalloc(&total, 4) != hipSuccess) {
+        hipFree(masks_all); hipFree(flags); hipFree(ids); hipFree(blocks); hipFree(total);
+        return -1;
+    }
+    hipLaunchKernelGGL(k_brick_classify, dim3(nb), dim3(512), 0, s, a, masks_all, flags);
+    vt_scan_flags(flags, nb, ids, blocks, total, s);
+    uint32_t count = 0;
+    hipMemcpyAsync(&count, total, 4, hipMemcpyDeviceToHost, s);
+    hipError_t e = hipStreamSynchronize(s);
+    int rc = 0;
+    if (e != hipSuccess) rc = -1;
+    const size_t bpv = fmt == 0 ? 1 : 4;
+    if (rc == 0) {
+        out->nbricks = count;
+        out->fmt = fmt;
+        for (int k = 0; k < 3; ++k) { out->dim[k] = dim[k]; out->bdim[k] = a.bdim[k]; }
+        size_t cnt = count ? count : 1;
+        if (hipMalloc(&out->brick_map, (size_t)nb * 4) != hipSuccess ||
+            hipMalloc(&out->masks, cnt * 64) != hipSuccess ||
+            hipMalloc(&out->payload, cnt * 512 * bpv) != hipSuccess)
+            rc = -1;
+    }
+    if (rc == 0) {
+        hipLaunchKernelGGL(k_brick_scatter, dim3(nb), dim3(512), 0, s, a, masks_all, ids, flags, out->brick_map,
+                           reinterpret_cast<uint64_t*>(out->masks), out->payload);
+        if (hipGetLastError() != hipSuccess) rc = -1;
+        if (hipStreamSynchronize(s) != hipSuccess) rc = -1;
+        out->bytes = (size_t)nb * 4 + (size_t)(count ? count : 1) * (64 + 512 * bpv);
+    }
+    hipFree(masks_all); hipFree(flags); hipFree(ids); hipFree(blocks); hipFree(total);
+    return rc;
+}
+
+namespace {
+__global__ void k_debrick(VolumeDesc v, uint8_t* dense) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)v.dim[0] * v.dim[1] * v.dim[2];
+    if (i >= n) return;
+    uint32_t x = (uint32_t)(i % v.dim[0]);
+    uint32_t y = (uint32_t)((i / v.dim[0]) % v.dim[1]);
+    uint32_t z = (uint32_t)(i / ((size_t)v.dim[0] * v.dim[1]));
+    uint32_t b = (x >> 3) + v.bdim[0] * ((y >> 3) + v.bdim[1] * (z >> 3));
+    uint32_t e = v.brick_map[b];
+    uint32_t li = (x & 7) + 8 * (y & 7) + 64 * (z & 7);
+    if (v.fmt == 0) dense[i] = e == VT_EMPTY ? 0 : v.payload[(size_t)e * 512 + li];
+    else reinterpret_cast<uint32_t*>(dense)[i] = e == VT_EMPTY ? 0u : reinterpret_cast<const uint32_t*>(v.payload)[(size_t)e * 512 + li];
+}
+}  // namespace
+
+// Bricked -> dense (voxel order, after the texel remap) for round-trip checks.
+int vt_debrick(const VolumeDesc& v, uint8_t* dense_dev, hipStream_t s) {
+    size_t n = (size_t)v.dim[0] * v.dim[1] * v.dim[2];
+    uint32_t blocks = (uint32_t)((n + 255) / 256);
+    hipLaunchKernelGGL(k_debrick, dim3(blocks), dim3(256), 0, s, v, dense_dev);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}

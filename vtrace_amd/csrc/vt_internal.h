@@ -1,0 +1,53 @@
+// vt_internal.h — declarations shared by the translation units of librender_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "vt_types.h"
+
+// Device storage of one bricked volume.
+struct VolumeStore {
+    uint32_t dim[3] = {0, 0, 0};
+    uint32_t bdim[3] = {0, 0, 0};
+    uint32_t fmt = 0;
+    uint32_t nbricks = 0;
+    uint32_t* brick_map = nullptr;
+    uint32_t* masks = nullptr;
+    uint8_t* payload = nullptr;
+    uint32_t* palette = nullptr;
+    size_t bytes = 0;
+};
+
+// upload.hip
+hipError_t vt_scan_flags(const uint32_t* flags, uint32_t n, uint32_t* out, uint32_t* scratch_blocks, uint32_t* total_dev,
+                         hipStream_t s);
+int vt_brick_volume(const uint8_t* dense_dev, uint32_t fmt, const uint32_t* palette_dev, const uint16_t* remap_dev[3],
+                    const uint32_t dim[3], VolumeStore* out, hipStream_t s);
+int vt_debrick(const VolumeDesc& v, uint8_t* dense_dev, hipStream_t s);
+
+// lbvh.hip: builds InstanceGPU records and the LBVH from n ABI records on the device.
+struct LbvhWork {
+    uint32_t cap = 0;           // instances the buffers hold
+    InstanceGPU* inst = nullptr;
+    BvhNode* nodes = nullptr;   // 2*cap - 1
+    float* cen = nullptr;       // 3*cap centroids
+    uint32_t* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    int32_t* parent = nullptr;  // 2*cap - 1
+    uint32_t* counters = nullptr;  // cap
+    float* bounds = nullptr;    // 6
+};
+int vt_lbvh_reserve(LbvhWork& w, uint32_t n);
+void vt_lbvh_free(LbvhWork& w);
+int vt_lbvh_build(LbvhWork& w, const float* records_dev, uint32_t n, const VolumeDesc* vols_dev, uint32_t nvol,
+                  hipStream_t s);
+
+// trace.hip
+int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, unsigned long long* stats,
+                    hipStream_t s);
+int vt_encode_srgb8(const float4* rad, uint32_t* bgra, uint32_t n, hipStream_t s);
+
+// host math shared by the runtime (frame setup); FP64 inverse rounded to FP32.
+bool vt_inverse4(const float* m, float* out);

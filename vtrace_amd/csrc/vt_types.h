@@ -1,0 +1,95 @@
+// vt_types.h — data layout shared by the HIP kernels and the host runtime of librender_hip.
+//
+// HBM layout (one GPU):
+//   * volumes: per volume a brick map (u32 per 8^3 brick, x-fastest brick order,
+//     0xFFFFFFFF = empty), an occupancy-mask pool (64 B = 16 u32 per non-empty brick,
+//     bit x + 8y + 64z of the brick) and a payload pool (512 B u8 palette indices or
+//     2048 B RGBA8 per non-empty brick, voxel x + 8y + 64z), plus a 256-entry palette.
+//     Replaces the reference's R8G8B8A8_SRGB 3-D images (lib/memory.c:286-385).
+//   * a volume-descriptor table (64 B per volume) replacing the bindless sampler3D array
+//     (lib/descriptor.c:38-95, shaders/trace.frag:31).
+//   * instances: the 64 B ABI records (src/render.rs:31-35) expanded to 128 B GPU records
+//     holding M and inverse(M) (rows 0..2), the texture id and the volume size.
+//   * LBVH: 2n-1 nodes of 32 B (lo[3], hi[3], left, right); internal nodes 0..n-2, leaves
+//     n-1..2n-2 with left = instance index and right = 0xFFFFFFFF.
+//   * per pixel: a 16 B hit record and an RGBA32F radiance value.
+#pragma once
+
+#include <stdint.h>
+
+#define VT_EMPTY 0xFFFFFFFFu
+
+struct VolumeDesc {             // 64 B
+    uint32_t dim[3];
+    uint32_t bdim[3];
+    uint32_t fmt;               // 0 = u8 index + palette, 1 = RGBA8
+    uint32_t nbricks;           // non-empty bricks
+    const uint32_t* brick_map;
+    const uint32_t* masks;
+    const uint8_t* payload;
+    const uint32_t* palette;
+};
+
+struct InstanceGPU {            // 128 B
+    float M[12];                // model matrix rows 0..2, column-major: M[c*3 + r]
+    float inv[12];              // inverse(model) rows 0..2, column-major
+    uint32_t tex;
+    uint32_t valid;
+    float size[3];
+    uint32_t dim[3];
+    uint32_t pad[2];
+};
+
+struct BvhNode {                // 32 B
+    float lo[3];
+    float hi[3];
+    uint32_t left;
+    uint32_t right;
+};
+
+struct HitRecord {              // 16 B, one per pixel
+    uint32_t inst;              // instance index or 0xFFFFFFFF
+    uint32_t voxel;             // x | y << 10 | z << 20
+    uint16_t steps;
+    uint8_t face;               // 0..5 = -X,+X,-Y,+Y,-Z,+Z (outward normal), 0xFF miss
+    uint8_t pal;                // palette index (u8 volumes)
+    uint8_t mask;               // last DDA step mask
+    uint8_t flags;              // bit0 hit, bit1 primary vertex shadowed
+    uint16_t pad;
+};
+
+// Per-frame constants, passed by value to the render kernels.
+struct FrameGPU {
+    float PC[16];               // projection * camera
+    float AB[16];               // inverse(centered camera) * inverse(projection)
+    float R[9];                 // rotation part of inverse(centered camera), column-major 3x3
+    float O[3];                 // camera position
+    float ip00, ip11;           // inverse(projection)[0][0], [1][1]
+    float W, Hv;                // viewport width and height (Hv = W for the reference quirk)
+    uint32_t width, height;
+    uint32_t mode, spp, max_bounces, jitter, seed, frame;
+    float sun[3];
+    float sun_rad[3];
+    float sky[3];
+    uint32_t stats;             // nonzero: count traversal work into the stats buffer
+};
+
+struct SceneGPU {
+    const InstanceGPU* inst;
+    const BvhNode* nodes;
+    const VolumeDesc* vols;
+    uint32_t ninst;
+    uint32_t nvol;
+};
+
+// stats buffer layout (u64 each)
+enum VtStat {
+    VT_STAT_RAYS = 0,
+    VT_STAT_NODES,
+    VT_STAT_INSTANCES,
+    VT_STAT_BRICKS,
+    VT_STAT_BRICKS_NONEMPTY,
+    VT_STAT_HITS,
+    VT_STAT_STEPS,
+    VT_STAT_COUNT
+};

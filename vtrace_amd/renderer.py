@@ -1,0 +1,294 @@
+"""Host-side mirror of the reference's renderer client (src/render.rs, src/scene.rs).
+
+``Renderer`` drives librender_hip through the same call sequence ``src/render.rs`` uses on
+the Vulkan library: ``entry`` at construction, ``start_update_instances`` /
+``end_update_instances`` with one record per uploaded child (un-uploaded textures are
+skipped, render.rs:220-238), at most one ``add_texture`` per ``render_tick``
+(render.rs:246-268), then ``render_tick`` with the perspective and camera matrices.
+``SceneGraph`` flattens parents and children depth-first in child order; instance order
+is the draw order and therefore the instance index of the hit buffers (scene.rs:146-181).
+
+The headless extensions (render mode, index+palette volumes, read-back) are exposed as
+methods.  Nothing here computes pixels: every frame is rendered by the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from collections import deque
+
+import numpy as np
+
+from . import scenes
+from ._native import render_lib
+from .abi import RenderTickInfo, VtRenderParams
+
+HIT_DTYPE = np.dtype([("inst", "<u4"), ("voxel", "<u4"), ("steps", "<u2"), ("face", "u1"), ("pal", "u1"),
+                      ("mask", "u1"), ("flags", "u1"), ("pad", "<u2")])
+INSTANCE_DTYPE = np.dtype([("M", "<f4", 12), ("inv", "<f4", 12), ("tex", "<u4"), ("valid", "<u4"),
+                           ("size", "<f4", 3), ("dim", "<u4", 3), ("pad", "<u4", 2)])
+NODE_DTYPE = np.dtype([("lo", "<f4", 3), ("hi", "<f4", 3), ("left", "<u4"), ("right", "<u4")])
+STAT_NAMES = ("rays", "nodes", "instances", "bricks", "bricks_nonempty", "hits", "steps")
+
+
+class RendererError(RuntimeError):
+    pass
+
+
+class SceneGraph:
+    """scene.rs:19-29 Parent/Child tree; iteration = DFS pre-order with accumulated models."""
+
+    def __init__(self, model=None, texture_handle=None, children=None):
+        self.model = np.eye(4, dtype=np.float32).reshape(16) if model is None else np.asarray(model, np.float32)
+        self.texture_handle = texture_handle
+        self.children = children if children is not None else ([] if texture_handle is None else None)
+
+    @classmethod
+    def child(cls, model, texture_handle):
+        return cls(model, texture_handle, None)
+
+    def add_child(self, child: "SceneGraph"):
+        if self.children is None:  # a Child becomes a Parent keeping itself as first child (scene.rs:94-115)
+            me = SceneGraph.child(np.eye(4, dtype=np.float32).reshape(16), self.texture_handle)
+            self.children = [me]
+            self.texture_handle = None
+        self.children.append(child)
+
+    def num_total_children(self) -> int:
+        if self.children is None:
+            return 1
+        return sum(c.num_total_children() for c in self.children)
+
+    def __iter__(self):
+        # scene.rs:146-181: a parent pushes its accumulated model on a stack that is never
+        # popped, so siblings after a subtree see the last pushed model (reproduced).
+        stack_nodes = [self]
+        model_stack = []
+        while stack_nodes:
+            node = stack_nodes.pop()
+            if node.children is not None:
+                stack_nodes.extend(reversed(node.children))
+                m = node.model.reshape(4, 4)
+                model_stack.append(m if not model_stack else _matmul_cm(model_stack[-1], m))
+                continue
+            m = node.model.reshape(4, 4)
+            yield (m if not model_stack else _matmul_cm(model_stack[-1], m)).reshape(16), node.texture_handle
+
+
+def _matmul_cm(a, b):
+    """Column-major product a*b of (4,4) arrays stored as m[c][r]."""
+    return (b.astype(np.float32) @ a.astype(np.float32)).astype(np.float32)
+
+
+class TextureUploadQueue:
+    """render.rs:137-171: handles are issued at once, uploads happen one per tick."""
+
+    def __init__(self):
+        self.num_textures_created = 0
+        self.queue = deque()
+
+    def add_texture(self, volume: "scenes.Volume") -> int:
+        handle = self.num_textures_created
+        self.num_textures_created += 1
+        self.queue.append((volume, handle))
+        return handle
+
+    def pop(self):
+        return self.queue.popleft() if self.queue else None
+
+
+class Renderer:
+    """render.rs:173-320 over librender_hip."""
+
+    def __init__(self):
+        self.lib = render_lib()
+        code = self.lib.entry()
+        if code != 0:
+            raise RendererError(f"ERROR: initialization failed ({code:#x}): {self.error()}")
+        self.texture_handle_lookup: dict[int, int] = {}
+        self.window_width = 1
+        self.window_height = 1
+
+    def error(self) -> str:
+        e = self.lib.vt_last_error()
+        return e.decode() if e else ""
+
+    def close(self):
+        self.lib.cleanup()
+
+    # -- reference call sequence ------------------------------------------------------
+    def update_instances(self, scene: SceneGraph):
+        count = scene.num_total_children()
+        ptr = self.lib.start_update_instances(count)
+        if not ptr:
+            raise RendererError("ERROR: Updating instances failed")
+        buf = np.ctypeslib.as_array(ptr, shape=(max(count, 1) * 16,))
+        n = 0
+        for model, handle in scene:
+            tex = self.texture_handle_lookup.get(handle)
+            if tex is None:
+                continue
+            buf[16 * n:16 * (n + 1)] = scenes.instance_record(model, tex)
+            n += 1
+        if self.lib.end_update_instances(n) != 0:
+            raise RendererError("ERROR: Updating instances failed")
+
+    def render_tick(self, perspective: np.ndarray, camera: np.ndarray, queue: TextureUploadQueue | None = None) -> bool:
+        if queue is not None:
+            item = queue.pop()
+            if item is not None:
+                volume, handle = item
+                self.texture_handle_lookup[handle] = self.upload(volume)
+        P = np.ascontiguousarray(perspective, dtype=np.float32)
+        Cm = np.ascontiguousarray(camera, dtype=np.float32)
+        info = RenderTickInfo(P.ctypes.data, Cm.ctypes.data)
+        w, h = C.c_int32(0), C.c_int32(0)
+        code = self.lib.render_tick(C.byref(w), C.byref(h), C.byref(info))
+        self.window_width, self.window_height = w.value, h.value
+        return code == 0
+
+    # -- uploads ----------------------------------------------------------------------------
+    def add_texture_rgba(self, rgba: np.ndarray) -> int:
+        """add_texture (lib/memory.c:286): RGBA8 volume shaped (d, h, w) of uint32 or (d, h, w, 4) uint8."""
+        a = np.ascontiguousarray(rgba)
+        if a.dtype == np.uint32:
+            d, h, w = a.shape
+        else:
+            d, h, w, _ = a.shape
+        tid = self.lib.add_texture(a.ctypes.data, w, h, d)
+        if tid < 0:
+            raise RendererError(f"ERROR: Adding texture failed: {self.error()}")
+        return tid
+
+    def add_volume_indexed(self, idx: np.ndarray, palette: np.ndarray) -> int:
+        a = np.ascontiguousarray(idx, dtype=np.uint8)
+        pal = np.ascontiguousarray(palette, dtype=np.uint32)
+        d, h, w = a.shape
+        tid = self.lib.vt_add_volume_indexed(a.ctypes.data, w, h, d, pal.ctypes.data)
+        if tid < 0:
+            raise RendererError(f"ERROR: Adding volume failed: {self.error()}")
+        return tid
+
+    def upload(self, volume: "scenes.Volume") -> int:
+        if volume.fmt == 0:
+            return self.add_volume_indexed(volume.data, volume.palette)
+        return self.add_texture_rgba(volume.data)
+
+    def set_instances(self, records: np.ndarray):
+        recs = np.ascontiguousarray(records, dtype=np.float32).reshape(-1, 16)
+        n = recs.shape[0]
+        ptr = self.lib.start_update_instances(n)
+        if not ptr:
+            raise RendererError("start_update_instances failed")
+        buf = np.ctypeslib.as_array(ptr, shape=(max(n, 1) * 16,))
+        buf[: n * 16] = recs.reshape(-1)
+        if self.lib.end_update_instances(n) != 0:
+            raise RendererError(f"end_update_instances failed: {self.error()}")
+
+    # -- headless extensions ----------------------------------------------------------------
+    def set_params(self, p: "scenes.RenderParams", stats: bool = False):
+        s = VtRenderParams()
+        s.width, s.height, s.viewport_square = p.width, p.height, p.viewport_square
+        s.mode, s.spp, s.max_bounces, s.jitter = p.mode, p.spp, p.max_bounces, p.jitter
+        s.seed, s.frame = p.seed & 0xFFFFFFFF, p.frame & 0xFFFFFFFF
+        for k in range(3):
+            s.sun_dir[k], s.sun_radiance[k], s.sky[k] = p.sun_dir[k], p.sun_radiance[k], p.sky[k]
+        s.stats = 1 if stats else 0
+        if self.lib.vt_set_params(C.byref(s)) != 0:
+            raise RendererError(f"vt_set_params failed: {self.error()}")
+        self._wh = (p.width, p.height)
+
+    def render(self, P, Cm) -> None:
+        if not self.render_tick(P, Cm):
+            raise RendererError(f"render_tick failed: {self.error()}")
+
+    def hits(self) -> np.ndarray:
+        w, h = self.window_width, self.window_height
+        out = np.zeros(w * h, dtype=HIT_DTYPE)
+        if self.lib.vt_read_hits(out.ctypes.data, out.nbytes) != 0:
+            raise RendererError(self.error())
+        return out.reshape(h, w)
+
+    def radiance(self) -> np.ndarray:
+        w, h = self.window_width, self.window_height
+        out = np.zeros((h, w, 4), dtype=np.float32)
+        if self.lib.vt_read_radiance(out.ctypes.data, out.nbytes) != 0:
+            raise RendererError(self.error())
+        return out
+
+    def bgra8(self) -> np.ndarray:
+        w, h = self.window_width, self.window_height
+        out = np.zeros((h, w), dtype=np.uint32)
+        if self.lib.vt_read_bgra8(out.ctypes.data, out.nbytes) != 0:
+            raise RendererError(self.error())
+        return out
+
+    def stats(self) -> dict:
+        out = np.zeros(7, dtype=np.uint64)
+        if self.lib.vt_read_stats(out.ctypes.data) != 0:
+            raise RendererError(self.error())
+        return {k: int(v) for k, v in zip(STAT_NAMES, out)}
+
+    def last_frame_ms(self) -> float:
+        return float(self.lib.vt_last_frame_ms())
+
+    def bench(self, warmup: int, iters: int):
+        k, w = C.c_float(0), C.c_float(0)
+        if self.lib.vt_bench_frames(warmup, iters, C.byref(k), C.byref(w)) != 0:
+            raise RendererError(self.error())
+        return k.value, w.value
+
+    def volume_dense(self, tid: int) -> np.ndarray:
+        info = np.zeros(8, dtype=np.uint32)
+        if self.lib.vt_volume_info(tid, info.ctypes.data) != 0:
+            raise RendererError("bad volume id")
+        w, h, d, fmt = (int(x) for x in info[:4])
+        out = np.zeros((d, h, w), dtype=np.uint8 if fmt == 0 else np.uint32)
+        if self.lib.vt_read_volume(tid, out.ctypes.data, out.nbytes) != 0:
+            raise RendererError(self.error())
+        return out
+
+    def volume_info(self, tid: int) -> dict:
+        info = np.zeros(8, dtype=np.uint32)
+        if self.lib.vt_volume_info(tid, info.ctypes.data) != 0:
+            raise RendererError("bad volume id")
+        return {"dims": tuple(int(x) for x in info[:3]), "fmt": int(info[3]), "nbricks": int(info[4]),
+                "bytes": int(info[5]) | (int(info[6]) << 32)}
+
+    def lbvh(self):
+        n = self.lib.vt_instance_count()
+        nodes = np.zeros(max(1, 2 * n - 1), dtype=NODE_DTYPE)
+        inst = np.zeros(max(1, n), dtype=INSTANCE_DTYPE)
+        if self.lib.vt_read_lbvh(nodes.ctypes.data, nodes.nbytes, inst.ctypes.data, inst.nbytes) != 0:
+            raise RendererError(self.error())
+        return nodes, inst
+
+
+def load_scene(r: Renderer, scene: "scenes.Scene", stats: bool = False) -> list:
+    """Uploads a Scene's volumes and instances, sets its parameters; returns texture ids."""
+    ids = [r.upload(v) for v in scene.volumes]
+    recs = np.array(scene.records, dtype=np.float32).copy()
+    # scene records reference volumes by position; remap to the library's texture ids
+    tex = recs.view(np.uint32)[:, 15].copy()
+    recs.view(np.uint32)[:, 15] = np.array([ids[t] if t < len(ids) else t for t in tex], dtype=np.uint32)
+    r.set_instances(recs)
+    r.set_params(scene.params, stats=stats)
+    return ids
+
+
+_singleton: Renderer | None = None
+
+
+def get_renderer() -> Renderer:
+    """The library keeps global state (like the reference's `glbl`); one Renderer per process."""
+    global _singleton
+    if _singleton is None:
+        _singleton = Renderer()
+    return _singleton
+
+
+def reset_renderer() -> Renderer:
+    global _singleton
+    if _singleton is not None:
+        _singleton.close()
+        _singleton = None
+    return get_renderer()
