@@ -450,6 +450,29 @@ inline Slab slab(const float o[3], const float d[3], const float lo[3], const fl
     return s;
 }
 
+// LBVH node test: the ray's reciprocal direction is computed once per query and the
+// slab distances are (bound - o) * inv_d; zero direction components test containment.
+inline Slab slab_node(const float o[3], const float d[3], const float inv_d[3], const float lo[3], const float hi[3]) {
+    Slab s{-kInf, kInf, 0, true};
+    for (int a = 0; a < 3; ++a) {
+        float tn, tf;
+        if (d[a] != 0.0f) {
+            float t1 = (lo[a] - o[a]) * inv_d[a];
+            float t2 = (hi[a] - o[a]) * inv_d[a];
+            tn = gmin(t1, t2);
+            tf = gmax(t1, t2);
+        } else {
+            if (o[a] < lo[a] || o[a] > hi[a]) { s.ok = false; return s; }
+            tn = -kInf;
+            tf = kInf;
+        }
+        if (tn > s.tn) { s.tn = tn; s.axis = a; }
+        if (tf < s.tf) s.tf = tf;
+    }
+    s.ok = s.tn <= s.tf;
+    return s;
+}
+
 const float kUnitLo[3] = {-0.5f, -0.5f, -0.5f};
 const float kUnitHi[3] = {0.5f, 0.5f, 0.5f};
 
@@ -690,6 +713,7 @@ Hit query(const Scene& S, const float O[3], const float D[3], bool primary, uint
     }
     const Bvh& B = S.bvh;
     if (B.n == 0) return best;
+    const float inv_d[3] = {1.0f / D[0], 1.0f / D[1], 1.0f / D[2]};
     uint32_t stack[64];
     int sp = 0;
     stack[sp++] = 0;
@@ -697,7 +721,7 @@ Hit query(const Scene& S, const float O[3], const float D[3], bool primary, uint
         uint32_t ni = stack[--sp];
         const Node& N = B.nodes[ni];
         if (C.st) C.st->nodes++;
-        Slab s = slab(O, D, N.lo, N.hi);
+        Slab s = slab_node(O, D, inv_d, N.lo, N.hi);
         if (!s.ok || s.tf < 0.0f) continue;
         if (best.hit && s.tn > prune_bound(best.tn)) continue;
         if (N.right == kMiss) {

@@ -19,8 +19,6 @@
 #include "../../include/vtrace.h"
 #include "vt_internal.h"
 
-void vt_trace_set_tables(const float* dec_dev, const float* thr_dev);
-
 namespace {
 
 constexpr uint32_t kMaxTextures = 65536;  // lib/common.h:35
@@ -51,6 +49,9 @@ struct State {
     float4* rad = nullptr;
     uint32_t* bgra = nullptr;
     size_t frame_cap = 0;
+    void* wave_mem = nullptr;   // wavefront path state + queues
+    size_t wave_cap = 0;        // pixel samples it holds
+    WaveBuffers wave;
     unsigned long long* stats = nullptr;
     float* srgb_dec = nullptr;
     float* srgb_thr = nullptr;
@@ -102,7 +103,34 @@ void default_params(vt_render_params& p) {
     p.sky[0] = 0.53f; p.sky[1] = 0.81f; p.sky[2] = 0.92f;  // lib/command.c:57-60
 }
 
+int ensure_wave_buffers() {
+    const size_t paths = (size_t)g.params.width * g.params.height * (g.params.spp ? g.params.spp : 1u);
+    if (g.params.mode == 0) return 0;
+    if (paths <= g.wave_cap && g.wave_mem) return 0;
+    if (paths > 0xFFFFFFFFull / 2) {
+        set_err("frame too large (width * height * spp)");
+        return -1;
+    }
+    hipFree(g.wave_mem);
+    g.wave_mem = nullptr;
+    g.wave_cap = 0;
+    hipError_t e = hipMalloc(&g.wave_mem, vt_wave_bytes((uint32_t)paths));
+    if (e != hipSuccess) {
+        set_err("wavefront buffer allocation", e);
+        return -1;
+    }
+    vt_wave_bind(g.wave, g.wave_mem, (uint32_t)paths);
+    g.wave_cap = paths;
+    return 0;
+}
+
 int ensure_frame_buffers() {
+    if (ensure_wave_buffers() != 0) return -1;
+    if (!g.wave_mem) {  // mode 0 still needs the counters block
+        if (hipMalloc(&g.wave_mem, vt_wave_bytes(1)) != hipSuccess) return -1;
+        vt_wave_bind(g.wave, g.wave_mem, 1);
+        g.wave_cap = 1;
+    }
     size_t n = (size_t)g.params.width * g.params.height;
     if (n <= g.frame_cap && g.hits) return 0;
     hipFree(g.hits); hipFree(g.rad); hipFree(g.bgra);
@@ -273,7 +301,7 @@ int render_frame(const FrameGPU& F) {
     if (ensure_frame_buffers() != 0) return -1;
     if (F.stats) hipMemsetAsync(g.stats, 0, VT_STAT_COUNT * 8, g.stream);
     hipEventRecord(g.ev0, g.stream);
-    int rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.stats, g.stream);
+    int rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.wave, g.stats, g.stream);
     hipEventRecord(g.ev1, g.stream);
     if (rc != 0) {
         set_err("trace launch failed", hipGetLastError());
@@ -365,6 +393,11 @@ uint64_t entry(void) {
     hipMemcpy(g.srgb_thr, thr, sizeof(thr), hipMemcpyHostToDevice);
     hipMemset(g.stats, 0, VT_STAT_COUNT * 8);
     vt_trace_set_tables(g.srgb_dec, g.srgb_thr);
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, g.device) == hipSuccess && prop.multiProcessorCount > 0)
+            vt_trace_set_persistent_blocks((uint32_t)prop.multiProcessorCount * 4u);
+    }
     default_params(g.params);
     g.inited = true;
     if (grow_instances(0) != 0) return ((uint64_t)hipErrorOutOfMemory << 32) | 1u;
@@ -406,6 +439,7 @@ void cleanup(void) {
     hipHostFree(g.inst_host);
     hipFree(g.inst_dev); hipFree(g.vol_table); hipFree(g.staging); hipFree(g.remap);
     hipFree(g.hits); hipFree(g.rad); hipFree(g.bgra); hipFree(g.stats); hipFree(g.srgb_dec); hipFree(g.srgb_thr);
+    hipFree(g.wave_mem);
     hipEventDestroy(g.ev0); hipEventDestroy(g.ev1); hipEventDestroy(g.inst_copied);
     hipStreamDestroy(g.stream);
     vt_trace_set_tables(nullptr, nullptr);
@@ -532,7 +566,7 @@ int32_t vt_bench_frames(uint32_t warmup, uint32_t iters, float* kernel_ms, float
     int rc = 0;
     for (uint32_t i = 0; i < iters && rc == 0; ++i) {
         hipEventRecord(ev[2 * i], g.stream);
-        rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.stats, g.stream);
+        rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.wave, g.stats, g.stream);
         hipEventRecord(ev[2 * i + 1], g.stream);
     }
     hipEventRecord(stop, g.stream);

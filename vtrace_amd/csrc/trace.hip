@@ -1,4 +1,4 @@
-// trace.hip — per-pixel ray generation, LBVH traversal, brick DDA and shading.
+// trace.hip — ray generation, LBVH traversal, brick DDA and shading (wavefront form).
 //
 // Replaces the reference's raster pass (lib/command.c:50-109: one proxy cube per
 // instance, back-face cull, depth LESS) and its fragment shader (shaders/trace.frag:41-90):
@@ -15,6 +15,18 @@
 // The secondary-ray rules (origin voxel face, cosine sampling, sun/sky terms) have no
 // counterpart in the reference and follow this project's specification.
 //
+// Execution (modes 1 and 2) is a wavefront: k_primary traces every pixel sample and shades
+// its first hit; then per bounce depth k_shadow traces the sun rays and k_bounce the
+// diffuse rays of the surviving paths, whose hits are appended (wave-aggregated atomics)
+// to a compacted queue for the next depth; k_accumulate sums the samples of each pixel.
+// Every kernel is persistent: each wave fetches 64 consecutive work items at a time from
+// an atomic counter, so terminated paths never occupy lanes.  Each path's arithmetic is
+// independent of queue order, so results are deterministic.
+//
+// DDA: empty 8^3 bricks (brick map entry 0xFFFFFFFF) are crossed by a tight loop without
+// memory access; in non-empty bricks the 64 B occupancy mask is staged once into the
+// thread's LDS slot and tested per step from LDS.  The LBVH stack lives in LDS as well.
+//
 // All FP32 expressions are compiled with -ffp-contract=off and evaluated in the order
 // written (correctly rounded division and square root), which keeps the integer outputs
 // (instance, voxel, face, palette index, steps) bit-exact with the CPU oracle.
@@ -23,6 +35,9 @@
 #include "vt_internal.h"
 
 namespace {
+
+constexpr int kBlock = 256;
+constexpr int kStackLds = 20;   // LBVH stack entries per thread held in LDS
 
 // GLSL min/max as "y if y < x else x" (also fixes the NaN cases the DDA can produce)
 __device__ __forceinline__ float gmin(float x, float y) { return (y < x) ? y : x; }
@@ -75,6 +90,28 @@ struct Ctr<true> {
     }
 };
 
+// ---- per-thread LDS slots ------------------------------------------------------------
+// Interleaved [entry][thread] so that lanes reading their own slot never share a bank.
+struct Lds {
+    uint32_t* base;  // &smem[threadIdx.x]
+    __device__ __forceinline__ uint32_t& at(int k) const { return base[k * kBlock]; }
+};
+
+struct TravStack {
+    Lds lds;
+    uint32_t spill[48];
+    int sp;
+    __device__ __forceinline__ void push(uint32_t x) {
+        if (sp < kStackLds) lds.at(sp) = x;
+        else spill[sp - kStackLds] = x;
+        ++sp;
+    }
+    __device__ __forceinline__ uint32_t pop() {
+        --sp;
+        return sp < kStackLds ? lds.at(sp) : spill[sp - kStackLds];
+    }
+};
+
 // ---- small vector helpers (explicit evaluation order) -------------------------------
 struct V3 {
     float x, y, z;
@@ -114,9 +151,26 @@ struct Dda {
     bool hit;
 };
 
+// One DDA step (trace.frag:83-86): axes whose side distance is the minimum step together.
+#define VT_DDA_STEP()                                         \
+    {                                                         \
+        const bool mx = sx <= gmin(sy, sz);                   \
+        const bool my = sy <= gmin(sz, sx);                   \
+        const bool mz = sz <= gmin(sx, sy);                   \
+        t_last = mx ? sx : (my ? sy : sz);                    \
+        sx += (mx ? 1.0f : 0.0f) * dx;                        \
+        sy += (my ? 1.0f : 0.0f) * dy;                        \
+        sz += (mz ? 1.0f : 0.0f) * dz;                        \
+        vx += (mx ? 1 : 0) * stx;                             \
+        vy += (my ? 1 : 0) * sty;                             \
+        vz += (mz ? 1 : 0) * stz;                             \
+        mask = (mx ? 1u : 0u) | (my ? 2u : 0u) | (mz ? 4u : 0u); \
+        ++steps;                                              \
+    }
+
 template <bool ST>
 __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3], const int v0[3], V3 mdir, Dda& r,
-                                        Ctr<ST>& ctr) {
+                                        Ctr<ST>& ctr, const Lds& mslot) {
     int vx = v0[0], vy = v0[1], vz = v0[2];
     const float len = sqrtf((mdir.x * mdir.x + mdir.y * mdir.y) + mdir.z * mdir.z);
     const float sgx = gsign(mdir.x), sgy = gsign(mdir.y), sgz = gsign(mdir.z);
@@ -128,39 +182,48 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
     const int nx = (int)V.dim[0], ny = (int)V.dim[1], nz = (int)V.dim[2];
     const uint32_t max_steps = V.dim[0] + V.dim[1] + V.dim[2];
     const uint32_t bdx = V.bdim[0], bdy = V.bdim[1];
-    uint32_t steps = 0, mask = 0;
+    uint32_t steps = 0, mask = 0, entry = VT_EMPTY, li = 0;
     float t_last = 0.0f;
-    uint32_t cur_b = VT_EMPTY - 1u, entry = VT_EMPTY;
     bool hit = false;
-    uint32_t li = 0;
     while (steps < max_steps && (uint32_t)vx < (uint32_t)nx && (uint32_t)vy < (uint32_t)ny &&
            (uint32_t)vz < (uint32_t)nz) {
-        const uint32_t bl = (uint32_t)(vx >> 3) + bdx * ((uint32_t)(vy >> 3) + bdy * (uint32_t)(vz >> 3));
-        if (bl != cur_b) {
-            cur_b = bl;
-            entry = V.brick_map[bl];
-            ctr.brick(entry != VT_EMPTY);
+        // entering a brick: the current voxel is inside the volume
+        const int bx8 = vx & ~7, by8 = vy & ~7, bz8 = vz & ~7;
+        // extent of this brick inside the volume, per axis (the loop stays in brick and volume)
+        const uint32_t ex = (uint32_t)min(8, nx - bx8), ey = (uint32_t)min(8, ny - by8), ez = (uint32_t)min(8, nz - bz8);
+        entry = V.brick_map[(uint32_t)(bx8 >> 3) + bdx * ((uint32_t)(by8 >> 3) + bdy * (uint32_t)(bz8 >> 3))];
+        ctr.brick(entry != VT_EMPTY);
+        if (entry == VT_EMPTY) {
+            do {
+                VT_DDA_STEP();
+            } while (steps < max_steps && (uint32_t)(vx - bx8) < ex && (uint32_t)(vy - by8) < ey &&
+                     (uint32_t)(vz - bz8) < ez);
+            continue;
         }
-        if (entry != VT_EMPTY) {
-            li = (uint32_t)(vx & 7) + 8u * (uint32_t)(vy & 7) + 64u * (uint32_t)(vz & 7);
-            const uint32_t w = V.masks[(size_t)entry * 16 + (li >> 5)];
-            if ((w >> (li & 31u)) & 1u) {
+        // stage the brick's 64 B occupancy mask in this thread's LDS slot
+        {
+            const uint4* src = reinterpret_cast<const uint4*>(V.masks + (size_t)entry * 16);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint4 w4 = src[q];
+                mslot.at(4 * q + 0) = w4.x;
+                mslot.at(4 * q + 1) = w4.y;
+                mslot.at(4 * q + 2) = w4.z;
+                mslot.at(4 * q + 3) = w4.w;
+            }
+        }
+        for (;;) {
+            li = (uint32_t)(vx - bx8) + 8u * (uint32_t)(vy - by8) + 64u * (uint32_t)(vz - bz8);
+            if ((mslot.at((int)(li >> 5)) >> (li & 31u)) & 1u) {
                 hit = true;
                 break;
             }
+            VT_DDA_STEP();
+            if (!(steps < max_steps && (uint32_t)(vx - bx8) < ex && (uint32_t)(vy - by8) < ey &&
+                  (uint32_t)(vz - bz8) < ez))
+                break;
         }
-        const bool mx = sx <= gmin(sy, sz);
-        const bool my = sy <= gmin(sz, sx);
-        const bool mz = sz <= gmin(sx, sy);
-        t_last = mx ? sx : (my ? sy : sz);
-        sx += (mx ? 1.0f : 0.0f) * dx;
-        sy += (my ? 1.0f : 0.0f) * dy;
-        sz += (mz ? 1.0f : 0.0f) * dz;
-        vx += (mx ? 1 : 0) * stx;
-        vy += (my ? 1 : 0) * sty;
-        vz += (mz ? 1 : 0) * stz;
-        mask = (mx ? 1u : 0u) | (my ? 2u : 0u) | (mz ? 4u : 0u);
-        ++steps;
+        if (hit) break;
     }
     ctr.steps(steps);
     r.v[0] = vx; r.v[1] = vy; r.v[2] = vz;
@@ -190,19 +253,20 @@ struct SlabR {
     int axis;
     bool ok;
 };
-__device__ __forceinline__ SlabR slab(V3 o, V3 d, const float lo[3], const float hi[3]) {
+// instance slab in model space: exact IEEE divisions (defines the entry point)
+__device__ __forceinline__ SlabR slab_unit(V3 o, V3 d) {
     SlabR s{-INFINITY, INFINITY, 0, true};
     const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         float tn, tf;
         if (dd[a] != 0.0f) {
-            float t1 = (lo[a] - oo[a]) / dd[a];
-            float t2 = (hi[a] - oo[a]) / dd[a];
+            float t1 = (-0.5f - oo[a]) / dd[a];
+            float t2 = (0.5f - oo[a]) / dd[a];
             tn = gmin(t1, t2);
             tf = gmax(t1, t2);
         } else {
-            if (oo[a] < lo[a] || oo[a] > hi[a]) s.ok = false;
+            if (oo[a] < -0.5f || oo[a] > 0.5f) s.ok = false;
             tn = -INFINITY;
             tf = INFINITY;
         }
@@ -212,13 +276,36 @@ __device__ __forceinline__ SlabR slab(V3 o, V3 d, const float lo[3], const float
     s.ok = s.ok && s.tn <= s.tf;
     return s;
 }
-
-__device__ const float kLo[3] = {-0.5f, -0.5f, -0.5f};
-__device__ const float kHi[3] = {0.5f, 0.5f, 0.5f};
+// LBVH node slab: reciprocal direction, (bound - o) * inv_d
+__device__ __forceinline__ bool slab_node(V3 o, V3 d, V3 id, const BvhNode& N, float* tn_out, float* tf_out) {
+    const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z}, ii[3] = {id.x, id.y, id.z};
+    float tn = -INFINITY, tf = INFINITY;
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float n_, f_;
+        if (dd[a] != 0.0f) {
+            float t1 = (N.lo[a] - oo[a]) * ii[a];
+            float t2 = (N.hi[a] - oo[a]) * ii[a];
+            n_ = gmin(t1, t2);
+            f_ = gmax(t1, t2);
+        } else {
+            if (oo[a] < N.lo[a] || oo[a] > N.hi[a]) ok = false;
+            n_ = -INFINITY;
+            f_ = INFINITY;
+        }
+        if (n_ > tn) tn = n_;
+        if (f_ < tf) tf = f_;
+    }
+    *tn_out = tn;
+    *tf_out = tf;
+    return ok && tn <= tf;
+}
 
 // Enter instance i from outside along (O, D).  PRIMARY rebuilds the fragment inputs.
 template <bool PRIMARY, bool ST>
-__device__ bool enter_instance(const SceneGPU& S, const FrameGPU& F, uint32_t i, V3 O, V3 D, HitG& h, Ctr<ST>& ctr) {
+__device__ bool enter_instance(const SceneGPU& S, const FrameGPU& F, uint32_t i, V3 O, V3 D, HitG& h, Ctr<ST>& ctr,
+                               const Lds& mslot) {
     const InstanceGPU* I = &S.inst[i];
     if (!I->valid) return false;
     float inv[12];
@@ -226,8 +313,7 @@ __device__ bool enter_instance(const SceneGPU& S, const FrameGPU& F, uint32_t i,
     for (int k = 0; k < 12; ++k) inv[k] = I->inv[k];
     const V3 o = xf34(inv, O, 1.0f);
     const V3 d = xf34(inv, D, 0.0f);
-    const float lo[3] = {-0.5f, -0.5f, -0.5f}, hi[3] = {0.5f, 0.5f, 0.5f};
-    SlabR s = slab(o, d, lo, hi);
+    SlabR s = slab_unit(o, d);
     if (!s.ok || !(s.tn > 0.0f)) return false;
     float mp[3];
     const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
@@ -265,7 +351,7 @@ __device__ bool enter_instance(const SceneGPU& S, const FrameGPU& F, uint32_t i,
         v0[a] = (int)floorf(gmin(mpos[a], size[a] - 1.0f));
     }
     Dda r;
-    dda_run<ST>(V, mpos, v0, mdir, r, ctr);
+    dda_run<ST>(V, mpos, v0, mdir, r, ctr, mslot);
     if (!r.hit) return false;
     h.hit = true;
     h.inst = i;
@@ -289,46 +375,55 @@ __device__ __forceinline__ float prune_bound(float t) { return t + fabsf(t) * 0.
 
 // Closest (or any) hit over all instances except `skip`, through the LBVH.
 template <bool PRIMARY, bool ANY, bool ST>
-__device__ HitG query(const SceneGPU& S, const FrameGPU& F, V3 O, V3 D, uint32_t skip, Ctr<ST>& ctr) {
+__device__ HitG query(const SceneGPU& S, const FrameGPU& F, V3 O, V3 D, uint32_t skip, Ctr<ST>& ctr,
+                      const Lds& stk_lds, const Lds& mslot) {
     HitG best;
     best.hit = false;
     best.inst = VT_EMPTY;
     best.tn = INFINITY;
     if (S.ninst == 0) return best;
-    uint32_t stack[64];
-    int sp = 0;
-    stack[sp++] = 0;
-    while (sp > 0) {
-        const uint32_t ni = stack[--sp];
+    const V3 inv_d{1.0f / D.x, 1.0f / D.y, 1.0f / D.z};
+    TravStack stk;
+    stk.lds = stk_lds;
+    stk.sp = 0;
+    stk.push(0u);
+    while (stk.sp > 0) {
+        const uint32_t ni = stk.pop();
         const BvhNode N = S.nodes[ni];
         ctr.node();
-        SlabR s = slab(O, D, N.lo, N.hi);
-        if (!s.ok || s.tf < 0.0f) continue;
-        if (best.hit && s.tn > prune_bound(best.tn)) continue;
+        float tn, tf;
+        if (!slab_node(O, D, inv_d, N, &tn, &tf) || tf < 0.0f) continue;
+        if (best.hit && tn > prune_bound(best.tn)) continue;
         if (N.right == VT_EMPTY) {
             const uint32_t i = N.left;
             if (i == skip) continue;
             HitG h;
-            if (enter_instance<PRIMARY, ST>(S, F, i, O, D, h, ctr) && better(h, best)) {
+            if (enter_instance<PRIMARY, ST>(S, F, i, O, D, h, ctr, mslot) && better(h, best)) {
                 best = h;
                 if (ANY) return best;
             }
         } else {
-            stack[sp++] = N.right;
-            stack[sp++] = N.left;
+            stk.push(N.right);
+            stk.push(N.left);
         }
     }
     return best;
 }
 
-struct Surface {
-    float p[3];
-    int vstart[3];
-    V3 X;
-    V3 n;
+// Surface of a hit: what shading and the next query need (64 B, queue element).
+struct SurfaceRec {
+    float p[3];        // hit point, voxel units of the hit instance
+    uint32_t inst;
+    float X[3];        // world position
+    uint32_t path;
+    float n[3];        // world unit normal
+    uint32_t rgba;     // texel colour (RGBA8)
+    int vstart[3];     // empty neighbour voxel on the face side
+    uint32_t face;
 };
+static_assert(sizeof(SurfaceRec) == 64, "surface record is 64 bytes");
 
-__device__ void surface_of(const SceneGPU& S, const HitG& h, Surface& sf) {
+__device__ void surface_of(const SceneGPU& S, const HitG& h, SurfaceRec& sf) {
     const InstanceGPU* I = &S.inst[h.inst];
     const float len = sqrtf((h.mdir.x * h.mdir.x + h.mdir.y * h.mdir.y) + h.mdir.z * h.mdir.z);
     const int fa = h.face >> 1;
@@ -348,19 +443,23 @@ __device__ void surface_of(const SceneGPU& S, const HitG& h, Surface& sf) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) { M[k] = I->M[k]; inv[k] = I->inv[k]; }
     const V3 mp{sf.p[0] / I->size[0] - 0.5f, sf.p[1] / I->size[1] - 0.5f, sf.p[2] / I->size[2] - 0.5f};
-    sf.X = xf34(M, mp, 1.0f);
+    const V3 X = xf34(M, mp, 1.0f);
     const float nm = neg ? -1.0f : 1.0f;
-    // row fa of inverse(M): inv[c*3 + fa]
-    const V3 nw{inv[0 * 3 + fa] * nm, inv[1 * 3 + fa] * nm, inv[2 * 3 + fa] * nm};
-    sf.n = normalize3(nw);
+    // row fa of inverse(M): inv[c*3 + fa] (inverse-transpose applied to the model normal)
+    const V3 n = normalize3(V3{inv[0 * 3 + fa] * nm, inv[1 * 3 + fa] * nm, inv[2 * 3 + fa] * nm});
+    sf.X[0] = X.x; sf.X[1] = X.y; sf.X[2] = X.z;
+    sf.n[0] = n.x; sf.n[1] = n.y; sf.n[2] = n.z;
+    sf.inst = h.inst;
+    sf.face = h.face;
 }
 
 template <bool ANY, bool ST>
-__device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, uint32_t origin, const Surface& sf, uint8_t origin_face,
-                          V3 w, Ctr<ST>& ctr) {
+__device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const SurfaceRec& sf, V3 w, Ctr<ST>& ctr,
+                          const Lds& stk_lds, const Lds& mslot) {
     ctr.ray();
     HitG h;
     h.hit = false;
+    const uint32_t origin = sf.inst;
     const InstanceGPU* I = &S.inst[origin];
     float inv[12];
 #pragma unroll
@@ -369,7 +468,7 @@ __device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, uint32_t origin,
     ctr.instance();
     const VolumeDesc V = S.vols[I->tex];
     Dda r;
-    dda_run<ST>(V, sf.p, sf.vstart, mdir, r, ctr);
+    dda_run<ST>(V, sf.p, sf.vstart, mdir, r, ctr, mslot);
     if (r.hit) {
         h.hit = true;
         h.inst = origin;
@@ -381,12 +480,12 @@ __device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, uint32_t origin,
             int a = lowest_axis(r.mask);
             h.face = face_from(a, comp(mdir, a));
         } else {
-            h.face = origin_face;
+            h.face = (uint8_t)sf.face;
         }
         ctr.hit();
         return h;
     }
-    h = query<false, ANY, ST>(S, F, sf.X, w, origin, ctr);
+    h = query<false, ANY, ST>(S, F, V3{sf.X[0], sf.X[1], sf.X[2]}, w, origin, ctr, stk_lds, mslot);
     if (h.hit) ctr.hit();
     return h;
 }
@@ -425,63 +524,131 @@ __device__ __forceinline__ uint32_t pixel_seed(uint32_t seed, uint32_t frame, ui
     return pcg_hash(pixel ^ pcg_hash(sample ^ pcg_hash(frame ^ pcg_hash(seed))));
 }
 
-// XCD-aware tile order: consecutive workgroups land on different XCDs (round robin), so
-// workgroup b is given tile (b % 8) * (tiles / 8) + b / 8 to keep each XCD's L2 on one
-// contiguous band of the image.
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t ntiles) {
-    const uint32_t per = ntiles / 8u;
-    if (per == 0 || b >= per * 8u) return b;
-    return (b % 8u) * per + b / 8u;
+__device__ __forceinline__ V3 camera_ray(const FrameGPU& F, float sx, float sy) {
+    const float nxd = (2.0f * sx) / F.W - 1.0f;
+    const float nyd = (2.0f * sy) / F.Hv - 1.0f;
+    const V3 vv{nxd * F.ip00, nyd * F.ip11, -1.0f};
+    return V3{(F.R[0] * vv.x + F.R[3] * vv.y) + F.R[6] * vv.z, (F.R[1] * vv.x + F.R[4] * vv.y) + F.R[7] * vv.z,
+              (F.R[2] * vv.x + F.R[5] * vv.y) + F.R[8] * vv.z};
 }
 
-constexpr int kTile = 16;  // 16 x 16 pixels per 256-thread workgroup (4 waves of 8 x 8)
+// ---- persistent work distribution -----------------------------------------------------
+// Each wave takes 64 consecutive items from *fetch until `count` is exhausted.
+template <typename Body>
+__device__ __forceinline__ void for_each_item(uint32_t* fetch, uint32_t count, Body&& body) {
+    const uint32_t lane = threadIdx.x & 63u;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(fetch, 64u);
+        base = __shfl(base, 0, 64);
+        if (base >= count) break;
+        const uint32_t i = base + lane;
+        if (i < count) body(i);
+    }
+}
 
-template <int MODE, bool ST>
-__global__ __launch_bounds__(256) void k_render(const FrameGPU F, const SceneGPU S, const float* __restrict__ srgb_dec,
-                                                HitRecord* __restrict__ hits, float4* __restrict__ rad,
-                                                unsigned long long* stats) {
-    const uint32_t tiles_x = (F.width + kTile - 1) / kTile;
-    const uint32_t tiles_y = (F.height + kTile - 1) / kTile;
-    const uint32_t tile = xcd_tile(blockIdx.x, tiles_x * tiles_y);
-    const uint32_t t = threadIdx.x, wv = t >> 6, ln = t & 63;
-    const uint32_t px = (tile % tiles_x) * kTile + (wv & 1) * 8 + (ln & 7);
-    const uint32_t py = (tile / tiles_x) * kTile + (wv >> 1) * 8 + (ln >> 3);
+// Wave-aggregated append: one atomic per wave, slots in lane order.
+__device__ __forceinline__ uint32_t append_slot(uint32_t* counter) {
+    const uint64_t m = __ballot(1);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+    const uint32_t n = (uint32_t)__popcll(m);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, n);
+    base = __shfl(base, (int)leader, 64);
+    const uint64_t lower = lane == 0 ? 0ull : (m & (~0ull >> (64u - lane)));
+    return base + (uint32_t)__popcll(lower);
+}
+
+// Wavefront buffers (device pointers), see WaveBuffers in vt_internal.h.
+struct WF {
+    float4* L;            // per path: accumulated radiance
+    float4* beta;         // per path: throughput
+    uint32_t* rng;        // per path: RNG state
+    uint8_t* shadowed;    // per path: primary vertex shadowed
+    SurfaceRec* surf[2];  // per queue slot
+    float4* contrib[2];   // per slot: direct-light contribution if the sun ray is clear (w = 1: trace it)
+    float4* dir[2];       // per slot: bounce direction (w = 1: trace it)
+    uint32_t* counters;   // [0,1] queue sizes, [2..] fetch counters
+};
+
+// Shade a surface at `depth`: direct-light contribution for the sun ray and, if the path
+// continues, the cosine-sampled bounce direction (throughput times albedo).
+__device__ __forceinline__ void shade(const FrameGPU& F, const float* srgb_dec, const SurfaceRec& sf, uint32_t depth,
+                                      uint32_t max_b, float beta[3], Rng& rng, float4& contrib, float4& dirw,
+                                      bool& no_sun) {
+    const float rho[3] = {srgb_dec[sf.rgba & 255u], srgb_dec[(sf.rgba >> 8) & 255u], srgb_dec[(sf.rgba >> 16) & 255u]};
+    const float cosl = (sf.n[0] * F.sun[0] + sf.n[1] * F.sun[1]) + sf.n[2] * F.sun[2];
+    contrib = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    no_sun = !(cosl > 0.0f);
+    if (!no_sun) {
+        const float c = cosl * 0.3183098861837907f;
+        contrib = make_float4(beta[0] * (rho[0] * (F.sun_rad[0] * c)), beta[1] * (rho[1] * (F.sun_rad[1] * c)),
+                              beta[2] * (rho[2] * (F.sun_rad[2] * c)), 1.0f);
+    }
+    dirw = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (depth < max_b) {
+        V3 w;
+        cosine_dir(rng, V3{sf.n[0], sf.n[1], sf.n[2]}, w);
+        dirw = make_float4(w.x, w.y, w.z, 1.0f);
+        for (int k = 0; k < 3; ++k) beta[k] *= rho[k];
+    }
+}
+
+// Work index -> (pixel, sample): waves cover an 8 x 8 pixel tile at one sample index.
+__device__ __forceinline__ bool work_pixel(const FrameGPU& F, uint32_t w, uint32_t spp, uint32_t* px, uint32_t* py,
+                                           uint32_t* s) {
+    const uint32_t tiles_x = (F.width + 7u) / 8u;
+    const uint32_t per_tile = 64u * spp;
+    const uint32_t tile = w / per_tile, rem = w % per_tile;
+    *s = rem / 64u;
+    const uint32_t pt = rem % 64u;
+    *px = (tile % tiles_x) * 8u + (pt & 7u);
+    *py = (tile / tiles_x) * 8u + (pt >> 3);
+    return *px < F.width && *py < F.height;
+}
+
+template <bool ST>
+__global__ __launch_bounds__(kBlock) void k_primary(const FrameGPU F, const SceneGPU S, const float* __restrict__ srgb_dec,
+                                                    HitRecord* __restrict__ hits, WF wf, uint32_t nwork,
+                                                    unsigned long long* stats) {
+    __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
+    const Lds stk{&smem[threadIdx.x]}, mslot{&smem[kStackLds * kBlock + threadIdx.x]};
     Ctr<ST> ctr;
-    if (px < F.width && py < F.height) {
+    const uint32_t spp = F.spp ? F.spp : 1u;
+    const uint32_t max_b = F.mode == 2 ? F.max_bounces : 0u;
+    for_each_item(&wf.counters[2], nwork, [&](uint32_t w) {
+        uint32_t px, py, smp;
+        if (!work_pixel(F, w, spp, &px, &py, &smp)) return;
         const uint32_t pix = py * F.width + px;
-        const uint32_t spp = F.spp ? F.spp : 1u;
-        float acc[3] = {0.0f, 0.0f, 0.0f};
-        float acca = 0.0f;
-        HitRecord rec;
-        rec.inst = VT_EMPTY;
-        rec.voxel = VT_EMPTY;
-        rec.steps = 0;
-        rec.face = 0xFF;
-        rec.pal = 0;
-        rec.mask = 0;
-        rec.flags = 0;
-        rec.pad = 0;
-        const V3 O{F.O[0], F.O[1], F.O[2]};
-        const V3 sun{F.sun[0], F.sun[1], F.sun[2]};
-        for (uint32_t smp = 0; smp < spp; ++smp) {
-            Rng rng{pixel_seed(F.seed, F.frame, pix, smp)};
-            float jx = 0.5f, jy = 0.5f;
-            if (F.jitter) {
-                jx = rng.next();
-                jy = rng.next();
+        const uint32_t path = pix * spp + smp;
+        Rng rng{pixel_seed(F.seed, F.frame, pix, smp)};
+        float jx = 0.5f, jy = 0.5f;
+        if (F.jitter) {
+            jx = rng.next();
+            jy = rng.next();
+        }
+        const V3 D = camera_ray(F, (float)px + jx, (float)py + jy);
+        ctr.ray();
+        HitG h = query<true, false, ST>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, VT_EMPTY, ctr, stk, mslot);
+        float4 L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float beta[3] = {1.0f, 1.0f, 1.0f};
+        uint8_t shadowed = 0;
+        if (!h.hit) {
+            L = make_float4(1.0f * F.sky[0], 1.0f * F.sky[1], 1.0f * F.sky[2], 0.0f);
+            if (smp == 0) {
+                HitRecord rec;
+                rec.inst = VT_EMPTY; rec.voxel = VT_EMPTY; rec.steps = 0; rec.face = 0xFF;
+                rec.pal = 0; rec.mask = 0; rec.flags = 0; rec.pad = 0;
+                hits[pix] = rec;
             }
-            const float sx = (float)px + jx, sy = (float)py + jy;
-            const float nxd = (2.0f * sx) / F.W - 1.0f;
-            const float nyd = (2.0f * sy) / F.Hv - 1.0f;
-            const V3 vv{nxd * F.ip00, nyd * F.ip11, -1.0f};
-            const V3 D{(F.R[0] * vv.x + F.R[3] * vv.y) + F.R[6] * vv.z, (F.R[1] * vv.x + F.R[4] * vv.y) + F.R[7] * vv.z,
-                       (F.R[2] * vv.x + F.R[5] * vv.y) + F.R[8] * vv.z};
-            ctr.ray();
-            HitG h = query<true, false, ST>(S, F, O, D, VT_EMPTY, ctr);
-            if (h.hit) ctr.hit();
+        } else {
+            ctr.hit();
             uint8_t pal = 0;
-            uint32_t rgba = h.hit ? hit_rgba(S, h, &pal) : 0u;
-            if (smp == 0 && h.hit) {
+            SurfaceRec sf;
+            sf.rgba = hit_rgba(S, h, &pal);
+            if (smp == 0) {
+                HitRecord rec;
                 rec.inst = h.inst;
                 rec.voxel = (uint32_t)h.d.v[0] | ((uint32_t)h.d.v[1] << 10) | ((uint32_t)h.d.v[2] << 20);
                 rec.steps = (uint16_t)h.d.steps;
@@ -489,62 +656,174 @@ __global__ __launch_bounds__(256) void k_render(const FrameGPU F, const SceneGPU
                 rec.pal = pal;
                 rec.mask = (uint8_t)h.d.mask;
                 rec.flags = 1;
+                rec.pad = 0;
+                hits[pix] = rec;
             }
-            if (MODE == 0) {
-                if (h.hit) {
-                    const float a = (float)(rgba >> 24) / 255.0f;
-                    const float rgb[3] = {srgb_dec[rgba & 255u], srgb_dec[(rgba >> 8) & 255u], srgb_dec[(rgba >> 16) & 255u]};
-                    for (int k = 0; k < 3; ++k) acc[k] += rgb[k] * a + F.sky[k] * (1.0f - a);
-                    acca += a;
-                } else {
-                    for (int k = 0; k < 3; ++k) acc[k] += F.sky[k];
-                    acca += 1.0f;
-                }
-                continue;
+            surface_of(S, h, sf);
+            sf.path = path;
+            float4 contrib, dirw;
+            bool no_sun;
+            shade(F, srgb_dec, sf, 0u, max_b, beta, rng, contrib, dirw, no_sun);
+            shadowed = no_sun ? 1 : 0;
+            const uint32_t slot = append_slot(&wf.counters[0]);
+            wf.surf[0][slot] = sf;
+            wf.contrib[0][slot] = contrib;
+            wf.dir[0][slot] = dirw;
+        }
+        wf.L[path] = L;
+        wf.beta[path] = make_float4(beta[0], beta[1], beta[2], 0.0f);
+        wf.rng[path] = rng.s;
+        wf.shadowed[path] = shadowed;
+    });
+    if (ST) ctr.flush(stats);
+}
+
+template <bool ST>
+__global__ __launch_bounds__(kBlock) void k_shadow(const FrameGPU F, const SceneGPU S, WF wf, uint32_t q, uint32_t depth,
+                                                   uint32_t fetch_idx, unsigned long long* stats) {
+    __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
+    const Lds stk{&smem[threadIdx.x]}, mslot{&smem[kStackLds * kBlock + threadIdx.x]};
+    Ctr<ST> ctr;
+    const uint32_t count = wf.counters[q];
+    if (blockIdx.x == 0 && threadIdx.x == 0) wf.counters[1u - q] = 0u;  // next queue starts empty
+    for_each_item(&wf.counters[fetch_idx], count, [&](uint32_t slot) {
+        const float4 c = wf.contrib[q][slot];
+        if (c.w == 0.0f) return;
+        const SurfaceRec sf = wf.surf[q][slot];
+        HitG h = secondary<true, ST>(S, F, sf, V3{F.sun[0], F.sun[1], F.sun[2]}, ctr, stk, mslot);
+        if (!h.hit) {
+            float4 L = wf.L[sf.path];
+            L.x = L.x + c.x;
+            L.y = L.y + c.y;
+            L.z = L.z + c.z;
+            wf.L[sf.path] = L;
+        } else if (depth == 0) {
+            wf.shadowed[sf.path] = 1;
+        }
+    });
+    if (ST) ctr.flush(stats);
+}
+
+template <bool ST>
+__global__ __launch_bounds__(kBlock) void k_bounce(const FrameGPU F, const SceneGPU S, const float* __restrict__ srgb_dec,
+                                                   WF wf, uint32_t q, uint32_t depth, uint32_t fetch_idx,
+                                                   unsigned long long* stats) {
+    __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
+    const Lds stk{&smem[threadIdx.x]}, mslot{&smem[kStackLds * kBlock + threadIdx.x]};
+    Ctr<ST> ctr;
+    const uint32_t count = wf.counters[q];
+    const uint32_t nq = 1u - q;
+    const uint32_t max_b = F.max_bounces;
+    for_each_item(&wf.counters[fetch_idx], count, [&](uint32_t slot) {
+        const float4 dw = wf.dir[q][slot];
+        if (dw.w == 0.0f) return;
+        const SurfaceRec sf = wf.surf[q][slot];
+        const uint32_t path = sf.path;
+        HitG h = secondary<false, ST>(S, F, sf, V3{dw.x, dw.y, dw.z}, ctr, stk, mslot);
+        const float4 b4 = wf.beta[path];
+        if (!h.hit) {
+            float4 L = wf.L[path];
+            L.x = L.x + b4.x * F.sky[0];
+            L.y = L.y + b4.y * F.sky[1];
+            L.z = L.z + b4.z * F.sky[2];
+            wf.L[path] = L;
+            return;
+        }
+        SurfaceRec ns;
+        uint8_t pal;
+        ns.rgba = hit_rgba(S, h, &pal);
+        surface_of(S, h, ns);
+        ns.path = path;
+        float beta[3] = {b4.x, b4.y, b4.z};
+        Rng rng{wf.rng[path]};
+        float4 contrib, dirw;
+        bool no_sun;
+        shade(F, srgb_dec, ns, depth + 1u, max_b, beta, rng, contrib, dirw, no_sun);
+        const uint32_t ns_slot = append_slot(&wf.counters[nq]);
+        wf.surf[nq][ns_slot] = ns;
+        wf.contrib[nq][ns_slot] = contrib;
+        wf.dir[nq][ns_slot] = dirw;
+        wf.beta[path] = make_float4(beta[0], beta[1], beta[2], 0.0f);
+        wf.rng[path] = rng.s;
+    });
+    if (ST) ctr.flush(stats);
+}
+
+__global__ void k_accumulate(const FrameGPU F, WF wf, HitRecord* hits, float4* rad) {
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= F.width * F.height) return;
+    const uint32_t spp = F.spp ? F.spp : 1u;
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    for (uint32_t s = 0; s < spp; ++s) {
+        const float4 L = wf.L[pix * spp + s];
+        acc[0] += L.x;
+        acc[1] += L.y;
+        acc[2] += L.z;
+    }
+    const float fs = (float)spp;
+    rad[pix] = make_float4(acc[0] / fs, acc[1] / fs, acc[2] / fs, 1.0f);
+    if (wf.shadowed[pix * spp] && (hits[pix].flags & 1u)) hits[pix].flags |= 2u;
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// Mode 0: one thread per pixel, samples looped in-thread (primary colour only).
+template <bool ST>
+__global__ __launch_bounds__(kBlock) void k_render0(const FrameGPU F, const SceneGPU S, const float* __restrict__ srgb_dec,
+                                                    HitRecord* __restrict__ hits, float4* __restrict__ rad,
+                                                    uint32_t* fetch, uint32_t nwork, unsigned long long* stats) {
+    __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
+    const Lds stk{&smem[threadIdx.x]}, mslot{&smem[kStackLds * kBlock + threadIdx.x]};
+    Ctr<ST> ctr;
+    const uint32_t spp = F.spp ? F.spp : 1u;
+    for_each_item(fetch, nwork, [&](uint32_t w) {
+        uint32_t px, py, s0;
+        if (!work_pixel(F, w, 1u, &px, &py, &s0)) return;
+        const uint32_t pix = py * F.width + px;
+        float acc[3] = {0.0f, 0.0f, 0.0f};
+        float acca = 0.0f;
+        HitRecord rec;
+        rec.inst = VT_EMPTY; rec.voxel = VT_EMPTY; rec.steps = 0; rec.face = 0xFF;
+        rec.pal = 0; rec.mask = 0; rec.flags = 0; rec.pad = 0;
+        for (uint32_t smp = 0; smp < spp; ++smp) {
+            Rng rng{pixel_seed(F.seed, F.frame, pix, smp)};
+            float jx = 0.5f, jy = 0.5f;
+            if (F.jitter) {
+                jx = rng.next();
+                jy = rng.next();
             }
-            float beta[3] = {1.0f, 1.0f, 1.0f};
-            float L[3] = {0.0f, 0.0f, 0.0f};
-            const uint32_t max_b = MODE == 2 ? F.max_bounces : 0u;
-            HitG cur = h;
-            uint32_t cur_rgba = rgba;
-            for (uint32_t depth = 0;; ++depth) {
-                if (!cur.hit) {
-                    for (int k = 0; k < 3; ++k) L[k] += beta[k] * F.sky[k];
-                    break;
+            const V3 D = camera_ray(F, (float)px + jx, (float)py + jy);
+            ctr.ray();
+            HitG h = query<true, false, ST>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, VT_EMPTY, ctr, stk, mslot);
+            if (h.hit) {
+                ctr.hit();
+                uint8_t pal = 0;
+                const uint32_t rgba = hit_rgba(S, h, &pal);
+                if (smp == 0) {
+                    rec.inst = h.inst;
+                    rec.voxel = (uint32_t)h.d.v[0] | ((uint32_t)h.d.v[1] << 10) | ((uint32_t)h.d.v[2] << 20);
+                    rec.steps = (uint16_t)h.d.steps;
+                    rec.face = h.face;
+                    rec.pal = pal;
+                    rec.mask = (uint8_t)h.d.mask;
+                    rec.flags = 1;
                 }
-                Surface sf;
-                surface_of(S, cur, sf);
-                const float rho[3] = {srgb_dec[cur_rgba & 255u], srgb_dec[(cur_rgba >> 8) & 255u],
-                                      srgb_dec[(cur_rgba >> 16) & 255u]};
-                const float cosl = (sf.n.x * sun.x + sf.n.y * sun.y) + sf.n.z * sun.z;
-                if (cosl > 0.0f) {
-                    HitG sh = secondary<true, ST>(S, F, cur.inst, sf, cur.face, sun, ctr);
-                    if (!sh.hit) {
-                        const float c = cosl * 0.3183098861837907f;
-                        for (int k = 0; k < 3; ++k) L[k] += beta[k] * (rho[k] * (F.sun_rad[k] * c));
-                    } else if (smp == 0 && depth == 0) {
-                        rec.flags |= 2;
-                    }
-                } else if (smp == 0 && depth == 0) {
-                    rec.flags |= 2;
-                }
-                if (depth >= max_b) break;
-                V3 w;
-                cosine_dir(rng, sf.n, w);
-                for (int k = 0; k < 3; ++k) beta[k] *= rho[k];
-                cur = secondary<false, ST>(S, F, cur.inst, sf, cur.face, w, ctr);
-                if (cur.hit) {
-                    uint8_t p2;
-                    cur_rgba = hit_rgba(S, cur, &p2);
-                }
+                const float a = (float)(rgba >> 24) / 255.0f;
+                const float rgb[3] = {srgb_dec[rgba & 255u], srgb_dec[(rgba >> 8) & 255u], srgb_dec[(rgba >> 16) & 255u]};
+                for (int k = 0; k < 3; ++k) acc[k] += rgb[k] * a + F.sky[k] * (1.0f - a);
+                acca += a;
+            } else {
+                for (int k = 0; k < 3; ++k) acc[k] += F.sky[k];
+                acca += 1.0f;
             }
-            for (int k = 0; k < 3; ++k) acc[k] += L[k];
         }
         hits[pix] = rec;
         const float fs = (float)spp;
-        if (MODE == 0) rad[pix] = make_float4(acc[0] / fs, acc[1] / fs, acc[2] / fs, acca / fs);
-        else rad[pix] = make_float4(acc[0] / fs, acc[1] / fs, acc[2] / fs, 1.0f);
-    }
+        rad[pix] = make_float4(acc[0] / fs, acc[1] / fs, acc[2] / fs, acca / fs);
+    });
     if (ST) ctr.flush(stats);
 }
 
@@ -570,30 +849,89 @@ __global__ void k_encode_srgb8(const float4* rad, uint32_t* out, uint32_t n, con
 
 static const float* g_srgb_dec = nullptr;
 static const float* g_srgb_thr = nullptr;
+static uint32_t g_persistent_blocks = 2048;
 
 void vt_trace_set_tables(const float* dec_dev, const float* thr_dev) {
     g_srgb_dec = dec_dev;
     g_srgb_thr = thr_dev;
 }
 
-template <int MODE, bool ST>
-static void launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, unsigned long long* stats,
-                   hipStream_t s) {
-    const uint32_t tiles = ((F.width + kTile - 1) / kTile) * ((F.height + kTile - 1) / kTile);
-    hipLaunchKernelGGL((k_render<MODE, ST>), dim3(tiles), dim3(256), 0, s, F, S, g_srgb_dec, hits, rad, stats);
+void vt_trace_set_persistent_blocks(uint32_t blocks) { g_persistent_blocks = blocks ? blocks : 2048; }
+
+size_t vt_wave_bytes(uint32_t npaths) {
+    return (size_t)npaths * (16 + 16 + 4 + 1) + 2 * (size_t)npaths * (64 + 16 + 16) + 64 * 4 + 16 * 256;
 }
 
-int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, unsigned long long* stats,
-                    hipStream_t s) {
-    if (!g_srgb_dec) return -1;
-    const bool st = F.stats != 0 && stats != nullptr;
-    switch (F.mode) {
-        case 0: st ? launch<0, true>(F, S, hits, rad, stats, s) : launch<0, false>(F, S, hits, rad, stats, s); break;
-        case 1: st ? launch<1, true>(F, S, hits, rad, stats, s) : launch<1, false>(F, S, hits, rad, stats, s); break;
-        case 2: st ? launch<2, true>(F, S, hits, rad, stats, s) : launch<2, false>(F, S, hits, rad, stats, s); break;
-        default: return -1;
+int vt_wave_bind(WaveBuffers& wb, void* mem, uint32_t npaths) {
+    char* p = static_cast<char*>(mem);
+    auto take = [&](size_t bytes) {
+        char* r = p;
+        p += (bytes + 255) & ~(size_t)255;
+        return r;
+    };
+    wb.npaths = npaths;
+    wb.L = take((size_t)npaths * 16);
+    wb.beta = take((size_t)npaths * 16);
+    wb.rng = take((size_t)npaths * 4);
+    wb.shadowed = take((size_t)npaths);
+    for (int q = 0; q < 2; ++q) {
+        wb.surf[q] = take((size_t)npaths * 64);
+        wb.contrib[q] = take((size_t)npaths * 16);
+        wb.dir[q] = take((size_t)npaths * 16);
     }
+    wb.counters = reinterpret_cast<uint32_t*>(take(64 * 4));
+    return 0;
+}
+
+static WF to_wf(const WaveBuffers& wb) {
+    WF w;
+    w.L = reinterpret_cast<float4*>(wb.L);
+    w.beta = reinterpret_cast<float4*>(wb.beta);
+    w.rng = reinterpret_cast<uint32_t*>(wb.rng);
+    w.shadowed = reinterpret_cast<uint8_t*>(wb.shadowed);
+    for (int q = 0; q < 2; ++q) {
+        w.surf[q] = reinterpret_cast<SurfaceRec*>(wb.surf[q]);
+        w.contrib[q] = reinterpret_cast<float4*>(wb.contrib[q]);
+        w.dir[q] = reinterpret_cast<float4*>(wb.dir[q]);
+    }
+    w.counters = wb.counters;
+    return w;
+}
+
+template <bool ST>
+static int launch_all(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, const WaveBuffers& wb,
+                      unsigned long long* stats, hipStream_t s) {
+    const uint32_t spp = F.spp ? F.spp : 1u;
+    const uint32_t tiles = ((F.width + 7u) / 8u) * ((F.height + 7u) / 8u);
+    const dim3 grid(g_persistent_blocks), block(kBlock);
+    WF wf = to_wf(wb);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, wb.counters, 64u);
+    if (F.mode == 0) {
+        hipLaunchKernelGGL((k_render0<ST>), grid, block, 0, s, F, S, g_srgb_dec, hits, rad, &wb.counters[2],
+                           tiles * 64u, stats);
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    }
+    const uint32_t nwork = tiles * 64u * spp;
+    if ((size_t)F.width * F.height * spp > wb.npaths) return -1;
+    hipLaunchKernelGGL((k_primary<ST>), grid, block, 0, s, F, S, g_srgb_dec, hits, wf, nwork, stats);
+    const uint32_t max_b = F.mode == 2 ? F.max_bounces : 0u;
+    uint32_t fetch = 3;
+    for (uint32_t d = 0; d <= max_b && fetch + 2 < 64; ++d) {
+        const uint32_t q = d & 1u;
+        hipLaunchKernelGGL((k_shadow<ST>), grid, block, 0, s, F, S, wf, q, d, fetch++, stats);
+        if (d < max_b) hipLaunchKernelGGL((k_bounce<ST>), grid, block, 0, s, F, S, g_srgb_dec, wf, q, d, fetch++, stats);
+    }
+    const uint32_t npix = F.width * F.height;
+    hipLaunchKernelGGL(k_accumulate, dim3((npix + 255) / 256), dim3(256), 0, s, F, wf, hits, rad);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, const WaveBuffers& wb,
+                    unsigned long long* stats, hipStream_t s) {
+    if (!g_srgb_dec || F.mode > 2) return -1;
+    if (F.max_bounces > 29) return -1;  // fetch counters available per frame
+    const bool st = F.stats != 0 && stats != nullptr;
+    return st ? launch_all<true>(F, S, hits, rad, wb, stats, s) : launch_all<false>(F, S, hits, rad, wb, stats, s);
 }
 
 int vt_encode_srgb8(const float4* rad, uint32_t* bgra, uint32_t n, hipStream_t s) {

@@ -44,9 +44,24 @@ void vt_lbvh_free(LbvhWork& w);
 int vt_lbvh_build(LbvhWork& w, const float* records_dev, uint32_t n, const VolumeDesc* vols_dev, uint32_t nvol,
                   hipStream_t s);
 
-// trace.hip
-int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, unsigned long long* stats,
-                    hipStream_t s);
+// trace.hip: wavefront path state and queues (one block of device memory, see vt_wave_bind)
+struct WaveBuffers {
+    uint32_t npaths = 0;        // capacity in pixel samples
+    void* L = nullptr;          // float4 per path
+    void* beta = nullptr;       // float4 per path
+    void* rng = nullptr;        // u32 per path
+    void* shadowed = nullptr;   // u8 per path
+    void* surf[2] = {nullptr, nullptr};     // 64 B surface records per queue slot
+    void* contrib[2] = {nullptr, nullptr};  // float4 per slot
+    void* dir[2] = {nullptr, nullptr};      // float4 per slot
+    uint32_t* counters = nullptr;           // 64 u32: queue sizes and fetch counters
+};
+size_t vt_wave_bytes(uint32_t npaths);
+int vt_wave_bind(WaveBuffers& wb, void* mem, uint32_t npaths);
+void vt_trace_set_tables(const float* dec_dev, const float* thr_dev);
+void vt_trace_set_persistent_blocks(uint32_t blocks);
+int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, const WaveBuffers& wb,
+                    unsigned long long* stats, hipStream_t s);
 int vt_encode_srgb8(const float4* rad, uint32_t* bgra, uint32_t n, hipStream_t s);
 
 // host math shared by the runtime (frame setup); FP64 inverse rounded to FP32.
