@@ -482,7 +482,8 @@ struct DdaResult {
     int v[3];
     uint32_t steps;
     uint32_t mask;     // last step mask
-    float t_last;      // side value of the last step (distance along the unit model dir)
+    float t_last;      // side value of the last step (distance along the unit model dir from q)
+    float q[3];        // start of the voxel walk that ended at v (hit point = q + t_last * unit dir)
     uint32_t rgba;
     uint8_t pal;
 };
@@ -549,6 +550,129 @@ DdaResult dda(const Vol& V, const float mpos[3], const int v0[3], const float md
     r.mask = r.hit ? mask : 0;
     if (r.hit && steps == 0) r.mask = 0;
     r.t_last = t_last;
+    r.q[0] = mpos[0]; r.q[1] = mpos[1]; r.q[2] = mpos[2];
+    return r;
+}
+
+// Two-level DDA for secondary rays (no reference counterpart; this project's rule).
+// Empty 8^3 bricks are crossed in one step: the ray moves from its current position q to
+// the nearest exit plane of the brick (ties: every axis at that distance), the exit axes
+// step into the neighbouring brick and the other axes take floor(q) clamped to the brick.
+// Inside a non-empty brick a voxel walk from q (trace.frag's side/delta formulas, steps
+// with selects so zero direction components never produce NaN) tests occupancy until it
+// hits or leaves the brick; the position where it leaves is q + t * unit dir with the
+// exit axes snapped to the voxel boundary.  Steps count voxel steps and brick steps.
+DdaResult dda2(const Vol& V, const float p[3], const int v0[3], const float m[3], Counters& C) {
+    DdaResult r{};
+    const float len = std::sqrt((m[0] * m[0] + m[1] * m[1]) + m[2] * m[2]);
+    float sg[3], u[3], dl[3], rinv[3];
+    int st[3];
+    for (int a = 0; a < 3; ++a) {
+        sg[a] = gsign(m[a]);
+        st[a] = (int)sg[a];
+        u[a] = m[a] / len;
+        dl[a] = std::fabs(len / m[a]);
+        rinv[a] = len / m[a];
+    }
+    int v[3] = {v0[0], v0[1], v0[2]};
+    float q[3] = {p[0], p[1], p[2]};
+    const int n[3] = {(int)V.dim[0], (int)V.dim[1], (int)V.dim[2]};
+    uint32_t steps = 0, mask = 0;
+    const uint32_t guard = 2 * (V.bdim[0] + V.bdim[1] + V.bdim[2]) + 8;
+    for (uint32_t it = 0; it < guard; ++it) {
+        if (!(v[0] >= 0 && v[1] >= 0 && v[2] >= 0 && v[0] < n[0] && v[1] < n[1] && v[2] < n[2])) break;
+        const int b8[3] = {v[0] & ~7, v[1] & ~7, v[2] & ~7};
+        const bool nonempty =
+            V.brick_nonempty[(uint32_t)(b8[0] >> 3) + V.bdim[0] * ((uint32_t)(b8[1] >> 3) + V.bdim[1] * (uint32_t)(b8[2] >> 3))] != 0;
+        C.brick(V, v[0], v[1], v[2]);
+        if (!nonempty) {
+            float t[3];
+            float tmin = kInf;
+            for (int a = 0; a < 3; ++a) {
+                if (st[a] == 0) {
+                    t[a] = kInf;
+                } else {
+                    const float plane = (float)(b8[a] + (st[a] > 0 ? 8 : 0));
+                    t[a] = (plane - q[a]) * rinv[a];
+                }
+                tmin = gmin(tmin, t[a]);
+            }
+            uint32_t mk = 0;
+            float nq[3];
+            int nv[3];
+            for (int a = 0; a < 3; ++a) {
+                if (st[a] != 0 && t[a] <= tmin) {
+                    mk |= 1u << a;
+                    nq[a] = (float)(b8[a] + (st[a] > 0 ? 8 : 0));
+                    nv[a] = st[a] > 0 ? b8[a] + 8 : b8[a] - 1;
+                } else {
+                    const float x = q[a] + tmin * u[a];
+                    nq[a] = x;
+                    int f = (int)std::floor(x);
+                    f = f < b8[a] ? b8[a] : (f > b8[a] + 7 ? b8[a] + 7 : f);
+                    nv[a] = f;
+                }
+            }
+            for (int a = 0; a < 3; ++a) { q[a] = nq[a]; v[a] = nv[a]; }
+            mask = mk;
+            ++steps;
+            continue;
+        }
+        // voxel walk inside the non-empty brick, from q
+        float side[3];
+        for (int a = 0; a < 3; ++a) side[a] = ((sg[a] * ((float)v[a] - q[a]) + sg[a] * 0.5f) + 0.5f) * dl[a];
+        float t_last = 0.0f;
+        bool left = false;
+        for (;;) {
+            uint32_t ti = V.texel_index(v[0], v[1], v[2]);
+            uint8_t pal;
+            uint32_t c = V.rgba(ti, &pal);
+            if ((c >> 24) != 0u) {
+                r.hit = true;
+                r.rgba = c;
+                r.pal = pal;
+                r.t_last = t_last;
+                for (int a = 0; a < 3; ++a) r.q[a] = q[a];
+                break;
+            }
+            const bool mx = side[0] <= gmin(side[1], side[2]);
+            const bool my = side[1] <= gmin(side[2], side[0]);
+            const bool mz = side[2] <= gmin(side[0], side[1]);
+            t_last = mx ? side[0] : (my ? side[1] : side[2]);
+            side[0] = mx ? side[0] + dl[0] : side[0];
+            side[1] = my ? side[1] + dl[1] : side[1];
+            side[2] = mz ? side[2] + dl[2] : side[2];
+            v[0] += mx ? st[0] : 0;
+            v[1] += my ? st[1] : 0;
+            v[2] += mz ? st[2] : 0;
+            mask = (mx ? 1u : 0u) | (my ? 2u : 0u) | (mz ? 4u : 0u);
+            ++steps;
+            if ((v[0] & ~7) != b8[0] || (v[1] & ~7) != b8[1] || (v[2] & ~7) != b8[2] || v[0] < 0 || v[1] < 0 ||
+                v[2] < 0 || v[0] >= n[0] || v[1] >= n[1] || v[2] >= n[2]) {
+                left = true;
+                break;
+            }
+        }
+        if (r.hit) break;
+        if (left) {
+            float nq[3];
+            for (int a = 0; a < 3; ++a) {
+                if (mask & (1u << a)) {
+                    nq[a] = (float)(st[a] > 0 ? v[a] : v[a] + 1);
+                } else {
+                    const float x = q[a] + t_last * u[a];
+                    const float lo = (float)v[a], hi = (float)(v[a] + 1);
+                    nq[a] = x < lo ? lo : (x > hi ? hi : x);
+                }
+            }
+            for (int a = 0; a < 3; ++a) q[a] = nq[a];
+        }
+    }
+    if (C.st) C.st->steps += steps;
+    r.v[0] = v[0]; r.v[1] = v[1]; r.v[2] = v[2];
+    r.steps = steps;
+    r.mask = (r.hit && steps > 0) ? mask : 0;
+    if (!r.hit) r.t_last = 0.0f;
     return r;
 }
 
@@ -671,7 +795,7 @@ bool enter_instance(const Scene& S, uint32_t i, const float O[3], const float D[
         mpos[a] = (mp[a] + 0.5f) * V.size[a];                       // trace.frag:67
         v0[a] = (int)std::floor(gmin(mpos[a], V.size[a] - 1.0f));   // trace.frag:69
     }
-    DdaResult r = dda(V, mpos, v0, mdir, C);
+    DdaResult r = primary ? dda(V, mpos, v0, mdir, C) : dda2(V, mpos, v0, mdir, C);
     if (!r.hit) return false;
     h.hit = true;
     h.inst = i;
@@ -748,7 +872,7 @@ bool continue_in_instance(const Scene& S, uint32_t i, const float p[3], const in
     float mdir[3];
     xform_dir(I.inv, wdir, mdir);
     if (C.st) C.st->instances++;
-    DdaResult r = dda(V, p, vstart, mdir, C);
+    DdaResult r = dda2(V, p, vstart, mdir, C);
     if (!r.hit) return false;
     h.hit = true;
     h.inst = i;
@@ -780,7 +904,7 @@ void surface_of(const Scene& S, const Hit& h, Surface& sf) {
     bool neg = (h.face & 1) == 0;  // outward normal along -axis
     for (int a = 0; a < 3; ++a) {
         float u = h.mdir[a] / len;
-        float x = h.mpos[a] + h.d.t_last * u;
+        float x = h.d.q[a] + h.d.t_last * u;
         float lo = (float)h.d.v[a], hi = (float)(h.d.v[a] + 1);
         if (a == fa) x = neg ? lo : hi;
         else x = x < lo ? lo : (x > hi ? hi : x);

@@ -146,6 +146,7 @@ struct Dda {
     uint32_t steps;
     uint32_t mask;
     float t_last;
+    float q[3];       // start of the voxel walk that ended at v (hit point = q + t_last * unit dir)
     uint32_t entry;   // brick id of the hit voxel
     uint32_t li;      // voxel index inside the brick
     bool hit;
@@ -230,6 +231,136 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
     r.steps = steps;
     r.mask = hit && steps > 0 ? mask : 0u;
     r.t_last = t_last;
+    r.q[0] = mpos[0]; r.q[1] = mpos[1]; r.q[2] = mpos[2];
+    r.entry = entry;
+    r.li = li;
+    r.hit = hit;
+}
+
+// Two-level DDA for secondary rays (no reference counterpart; this project's rule).
+// Empty 8^3 bricks are crossed in one step: the ray moves from its current position q to
+// the nearest exit plane of the brick (ties: every axis at that distance), the exit axes
+// step into the neighbouring brick and the other axes take floor(q) clamped to the brick.
+// Inside a non-empty brick a voxel walk from q (trace.frag's side/delta formulas, steps
+// with selects so zero direction components never produce NaN) tests occupancy from the
+// LDS-staged mask until it hits or leaves the brick; the position where it leaves is
+// q + t * unit dir with the exit axes snapped to the voxel boundary.
+template <bool ST>
+__device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], const int v0[3], V3 m, Dda& r,
+                                         Ctr<ST>& ctr, const Lds& mslot) {
+    const float len = sqrtf((m.x * m.x + m.y * m.y) + m.z * m.z);
+    const float mm[3] = {m.x, m.y, m.z};
+    float sg[3], u[3], dl[3], rinv[3];
+    int st[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        sg[a] = gsign(mm[a]);
+        st[a] = (int)sg[a];
+        u[a] = mm[a] / len;
+        dl[a] = fabsf(len / mm[a]);
+        rinv[a] = len / mm[a];
+    }
+    int v[3] = {v0[0], v0[1], v0[2]};
+    float q[3] = {p[0], p[1], p[2]};
+    const int n[3] = {(int)V.dim[0], (int)V.dim[1], (int)V.dim[2]};
+    const uint32_t bdx = V.bdim[0], bdy = V.bdim[1];
+    uint32_t steps = 0, mask = 0, entry = VT_EMPTY, li = 0;
+    float t_last = 0.0f;
+    bool hit = false;
+    const uint32_t guard = 2u * (V.bdim[0] + V.bdim[1] + V.bdim[2]) + 8u;
+    for (uint32_t it = 0; it < guard; ++it) {
+        if (!((uint32_t)v[0] < (uint32_t)n[0] && (uint32_t)v[1] < (uint32_t)n[1] && (uint32_t)v[2] < (uint32_t)n[2])) break;
+        const int b8[3] = {v[0] & ~7, v[1] & ~7, v[2] & ~7};
+        entry = V.brick_map[(uint32_t)(b8[0] >> 3) + bdx * ((uint32_t)(b8[1] >> 3) + bdy * (uint32_t)(b8[2] >> 3))];
+        ctr.brick(entry != VT_EMPTY);
+        if (entry == VT_EMPTY) {
+            float t[3];
+            float tmin = INFINITY;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if (st[a] == 0) {
+                    t[a] = INFINITY;
+                } else {
+                    const float plane = (float)(b8[a] + (st[a] > 0 ? 8 : 0));
+                    t[a] = (plane - q[a]) * rinv[a];
+                }
+                tmin = gmin(tmin, t[a]);
+            }
+            uint32_t mk = 0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if (st[a] != 0 && t[a] <= tmin) {
+                    mk |= 1u << a;
+                    q[a] = (float)(b8[a] + (st[a] > 0 ? 8 : 0));
+                    v[a] = st[a] > 0 ? b8[a] + 8 : b8[a] - 1;
+                } else {
+                    const float x = q[a] + tmin * u[a];
+                    q[a] = x;
+                    int f = (int)floorf(x);
+                    f = f < b8[a] ? b8[a] : (f > b8[a] + 7 ? b8[a] + 7 : f);
+                    v[a] = f;
+                }
+            }
+            mask = mk;
+            ++steps;
+            continue;
+        }
+        {
+            const uint4* src = reinterpret_cast<const uint4*>(V.masks + (size_t)entry * 16);
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const uint4 w4 = src[w];
+                mslot.at(4 * w + 0) = w4.x;
+                mslot.at(4 * w + 1) = w4.y;
+                mslot.at(4 * w + 2) = w4.z;
+                mslot.at(4 * w + 3) = w4.w;
+            }
+        }
+        float sx = ((sg[0] * ((float)v[0] - q[0]) + sg[0] * 0.5f) + 0.5f) * dl[0];
+        float sy = ((sg[1] * ((float)v[1] - q[1]) + sg[1] * 0.5f) + 0.5f) * dl[1];
+        float sz = ((sg[2] * ((float)v[2] - q[2]) + sg[2] * 0.5f) + 0.5f) * dl[2];
+        float tl = 0.0f;
+        for (;;) {
+            li = (uint32_t)(v[0] - b8[0]) + 8u * (uint32_t)(v[1] - b8[1]) + 64u * (uint32_t)(v[2] - b8[2]);
+            if ((mslot.at((int)(li >> 5)) >> (li & 31u)) & 1u) {
+                hit = true;
+                t_last = tl;
+                break;
+            }
+            const bool mx = sx <= gmin(sy, sz);
+            const bool my = sy <= gmin(sz, sx);
+            const bool mz = sz <= gmin(sx, sy);
+            tl = mx ? sx : (my ? sy : sz);
+            sx = mx ? sx + dl[0] : sx;
+            sy = my ? sy + dl[1] : sy;
+            sz = mz ? sz + dl[2] : sz;
+            v[0] += mx ? st[0] : 0;
+            v[1] += my ? st[1] : 0;
+            v[2] += mz ? st[2] : 0;
+            mask = (mx ? 1u : 0u) | (my ? 2u : 0u) | (mz ? 4u : 0u);
+            ++steps;
+            if ((uint32_t)(v[0] - b8[0]) >= 8u || (uint32_t)(v[1] - b8[1]) >= 8u || (uint32_t)(v[2] - b8[2]) >= 8u ||
+                v[0] >= n[0] || v[1] >= n[1] || v[2] >= n[2])
+                break;
+        }
+        if (hit) break;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (mask & (1u << a)) {
+                q[a] = (float)(st[a] > 0 ? v[a] : v[a] + 1);
+            } else {
+                const float x = q[a] + tl * u[a];
+                const float lo = (float)v[a], hi = (float)(v[a] + 1);
+                q[a] = x < lo ? lo : (x > hi ? hi : x);
+            }
+        }
+    }
+    ctr.steps(steps);
+    r.v[0] = v[0]; r.v[1] = v[1]; r.v[2] = v[2];
+    r.steps = steps;
+    r.mask = hit && steps > 0 ? mask : 0u;
+    r.t_last = hit ? t_last : 0.0f;
+    r.q[0] = q[0]; r.q[1] = q[1]; r.q[2] = q[2];
     r.entry = entry;
     r.li = li;
     r.hit = hit;
@@ -351,7 +482,8 @@ __device__ bool enter_instance(const SceneGPU& S, const FrameGPU& F, uint32_t i,
         v0[a] = (int)floorf(gmin(mpos[a], size[a] - 1.0f));
     }
     Dda r;
-    dda_run<ST>(V, mpos, v0, mdir, r, ctr, mslot);
+    if (PRIMARY) dda_run<ST>(V, mpos, v0, mdir, r, ctr, mslot);
+    else dda2_run<ST>(V, mpos, v0, mdir, r, ctr, mslot);
     if (!r.hit) return false;
     h.hit = true;
     h.inst = i;
@@ -432,7 +564,7 @@ __device__ void surface_of(const SceneGPU& S, const HitG& h, SurfaceRec& sf) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const float u = md[a] / len;
-        float x = h.mpos[a] + h.d.t_last * u;
+        float x = h.d.q[a] + h.d.t_last * u;
         const float lo = (float)h.d.v[a], hi = (float)(h.d.v[a] + 1);
         if (a == fa) x = neg ? lo : hi;
         else x = x < lo ? lo : (x > hi ? hi : x);
@@ -468,7 +600,7 @@ __device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const SurfaceRec
     ctr.instance();
     const VolumeDesc V = S.vols[I->tex];
     Dda r;
-    dda_run<ST>(V, sf.p, sf.vstart, mdir, r, ctr, mslot);
+    dda2_run<ST>(V, sf.p, sf.vstart, mdir, r, ctr, mslot);
     if (r.hit) {
         h.hit = true;
         h.inst = origin;
