@@ -7,14 +7,20 @@ sun shadow ray per vertex, hash-RNG seeded per pixel/sample, FP32 radiance.
 A step is one frame: LBVH traversal + brick DDA + shading for every pixel sample, with the
 volumes, LBVH and instances already resident in HBM.  Rays counted = traversal queries
 (primary + shadow + bounce) from the library's own counters on an untimed counting frame.
+`value` = rays per frame / mean frame time (HIP events on the library stream around the K
+timed frames).
 
-Roofline: algorithmic bytes per frame = 32 B per LBVH node visited + 64 B per instance
-entered + (4 B + 64 B if non-empty) per brick entered + 5 B per hit + 32 B per pixel,
-divided by the trace kernel's mean duration (HIP events on the library stream), against
-the MI355X HBM peak of 8 TB/s.
+Roofline (HBM-bound gather work, no MFMA): algorithmic bytes = 32 B per LBVH node visited
++ 64 B per instance entered + (4 B + 64 B if non-empty) per 8^3 brick entered + 5 B per hit,
+plus 16 B per pixel for the hit record (primary stage) and 16 B per pixel of radiance
+(accumulate stage).  They are counted per launch stage; the dominant kernel is the stage
+kind with the largest share of frame time, and its achieved GB/s = its algorithmic bytes
+per launch / its mean launch duration (HIP events around each launch).  `traffic` is the
+HBM bytes per launch of that kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE, read from
+profiles/traffic.json when present (null otherwise).
 
-cpu_baseline: the CPU oracle (a scalar FP32/FP64 restatement of the same path, one core)
-on a band of rows of the same frame, scaled to Mrays/s.
+cpu_baseline: the CPU oracle (C++ restatement of the same path, host threads over rows) on a
+band of rows of the same frame, scaled to Mrays/s.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -30,33 +36,59 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+KERNEL_NAMES = {"render": "k_render0", "primary": "k_primary", "shadow": "k_shadow", "bounce": "k_bounce",
+                "accumulate": "k_accumulate"}
 
 
-def algorithmic_bytes(st: dict, pixels: int) -> int:
-    return (32 * st["nodes"] + 64 * st["instances"] + 4 * st["bricks"] + 64 * st["bricks_nonempty"] + 5 * st["hits"]
-            + 32 * pixels)
+def algorithmic_bytes(st: dict) -> int:
+    return 32 * st["nodes"] + 64 * st["instances"] + 4 * st["bricks"] + 64 * st["bricks_nonempty"] + 5 * st["hits"]
 
 
 def cpu_baseline(scene, target_s: float = 15.0) -> dict:
+    """Times the oracle on every k-th row of the frame (k sized for about target_s)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi as O  # the oracle is the CPU baseline's timed code, never the product
 
+    threads = max(1, min(16, os.cpu_count() or 1))
+    osc = O.OracleScene(scene, use_bvh=True)  # volumes / LBVH prepared outside the timed sample
     H = scene.params.height
-    rows, elapsed, rays = 1, 0.0, 0
-    y0 = H // 2
-    # grow the band until the sample takes a measurable time, then run the final sample
-    while True:
-        t0 = time.perf_counter()
-        _, _, st = O.render(scene, use_bvh=True, rows=(y0, min(H, y0 + rows)))
-        dt = time.perf_counter() - t0
-        if dt > target_s / 8 or y0 + rows >= H:
-            elapsed, rays = dt, st["rays"]
-            break
-        rows = min(H - y0, max(rows * 2, int(rows * (target_s / 8) / max(dt, 1e-3))))
-    rows_f = min(H, y0 + rows) - y0
-    return {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (scalar C++, -O2) on rows {y0}..{y0 + rows_f} of the same frame "
-                      f"({rays} rays, {elapsed:.1f} s)"}
+    stride = 64
+    t0 = time.perf_counter()
+    osc.render_rows(list(range(stride // 2, H, stride)), threads=threads)
+    dt = time.perf_counter() - t0
+    nrows = max(threads, min(H, int(len(range(stride // 2, H, stride)) * target_s / max(dt, 1e-3))))
+    stride = max(1, H // nrows)
+    rows = list(range(stride // 2, H, stride))
+    t0 = time.perf_counter()
+    st = osc.render_rows(rows, threads=threads)
+    elapsed = time.perf_counter() - t0
+    osc.close()
+    return {"value": round(st["rays"] / elapsed / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C++ restatement, -O2, {threads} threads) on every {stride}th row ({len(rows)} rows) of "
+                      f"the same 1920x1080 4-spp frame: {st['rays']} rays in {elapsed:.1f} s"}
+
+
+def calibrate_copy(nbytes: int = 256 << 20):
+    """A plain device copy of known size, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_bytes": nbytes}), flush=True)
+
+
+def load_traffic(kernel: str):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        t = json.load(open(path))
+        k = t["kernels"][kernel]
+        return int(k["fetch_bytes_per_launch"] + k["write_bytes_per_launch"])
+    except (KeyError, ValueError, TypeError):
+        return None
 
 
 def main():
@@ -65,6 +97,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true", help="run only a plain copy kernel (PMC scaling)")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     args = ap.parse_args()
@@ -72,6 +105,9 @@ def main():
         print(f"bench.py runs on one GPU only (the renderer has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     from vtrace_amd import renderer as R
     from vtrace_amd import scenes
@@ -85,31 +121,51 @@ def main():
     t_up = time.perf_counter() - t0
     print(f"# scene {sc.name}: generated in {t_gen:.1f} s, uploaded+bricked in {t_up:.1f} s", file=sys.stderr,
           flush=True)
-    # counting frame (untimed): rays and algorithmic bytes
+    # counting frame (untimed): rays and algorithmic bytes per launch stage
     r.render(sc.P, sc.C)
     st = r.stats()
+    stages = r.stage_stats()
     pixels = sc.params.width * sc.params.height
-    bytes_frame = algorithmic_bytes(st, pixels)
     hits = r.hits()
     hit_frac = float((hits["inst"] != 0xFFFFFFFF).mean())
     r.set_params(sc.params, stats=False)
     r.render(sc.P, sc.C)
-    # timed: K frames, kernel time from HIP events on the library stream, wall clock around
+    # timed: K frames, every launch bracketed by HIP events on the library stream
     r.lib.vt_synchronize()
     t0 = time.perf_counter()
-    kernel_ms, wall_ms = r.bench(args.warmup, args.steps)
+    kernel_ms, wall_ms, stage_ms = r.bench_stages(args.warmup, args.steps)
     r.lib.vt_synchronize()
     t_all = time.perf_counter() - t0
-    # per-frame latency distribution (render_tick round trips)
+    # per-frame latency distribution (render_tick round trips, GPU time)
     lat = []
-    for _ in range(min(args.steps, 20)):
+    for _ in range(max(5, min(args.steps, 20))):
         r.render(sc.P, sc.C)
         lat.append(r.last_frame_ms())
     lat.sort()
     p50 = lat[len(lat) // 2]
     p99 = lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))]
+
+    per_kind = {}
+    for (kind, s), (kind2, ms) in zip(stages, stage_ms):
+        assert kind == kind2
+        b = algorithmic_bytes(s)
+        if kind == "primary":
+            b += 16 * pixels
+        if kind == "accumulate":
+            b += 16 * pixels
+        e = per_kind.setdefault(kind, {"launches": 0, "bytes": 0, "ms": 0.0, "rays": 0})
+        e["launches"] += 1
+        e["bytes"] += b
+        e["ms"] += ms
+        e["rays"] += s["rays"]
+    dom = max(per_kind, key=lambda k: per_kind[k]["ms"])
+    d = per_kind[dom]
+    launch_ms = d["ms"] / d["launches"]
+    bytes_launch = d["bytes"] / d["launches"]
+    achieved = bytes_launch / (launch_ms * 1e-3) / 1e9
+    frame_bytes = sum(e["bytes"] for e in per_kind.values())
     mrays = st["rays"] / (wall_ms * 1e-3) / 1e6
-    achieved = bytes_frame / (kernel_ms * 1e-3) / 1e9
+    kname = KERNEL_NAMES[dom]
     out = {
         "metric": "Mrays/s @1080p 4spp; p50/p99 render latency; time to relMSE 1e-3 vs plain",
         "value": round(mrays, 2),
@@ -128,10 +184,16 @@ def main():
                    "max_bounces": sc.params.max_bounces, "rays_per_frame": st["rays"],
                    "primary_hit_fraction": round(hit_frac, 4), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_ray": round(bytes_frame / max(1, st["rays"]), 1),
-                     "kernel": "k_render<2,false>", "kernel_ms": round(kernel_ms, 4)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kname),
+                     "kernel": kname, "launches_per_frame": d["launches"], "launch_ms": round(launch_ms, 4),
+                     "algorithmic_bytes_per_launch": int(bytes_launch),
+                     "frame_algorithmic_bytes": int(frame_bytes),
+                     "frame_achieved_gbs": round(frame_bytes / (kernel_ms * 1e-3) / 1e9, 1),
+                     "bytes_per_ray": round(frame_bytes / max(1, st["rays"]), 1)},
+        "stages": {k: {"launches": v["launches"], "ms": round(v["ms"], 4), "rays": v["rays"],
+                       "GBs": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in per_kind.items()},
         "latency_ms": {"p50": round(p50, 4), "p99": round(p99, 4)},
+        "kernel_ms_per_frame": round(kernel_ms, 4),
         "counters": st,
         "timed_region_s": round(t_all, 3),
     }

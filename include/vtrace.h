@@ -100,6 +100,14 @@ int32_t vt_read_bgra8(uint32_t* dst, uint64_t bytes);
 /* 7 counters of the last frame rendered with stats = 1: rays, LBVH nodes, instances,
  * bricks, non-empty bricks, hits, DDA steps. */
 int32_t vt_read_stats(uint64_t* dst7);
+/* Per-launch-stage counters of the last counted frame: dst holds max_stages x 7 u64, kinds
+ * receives the stage kinds (0 render, 1 primary, 2 shadow, 3 bounce, 4 accumulate).
+ * Returns the number of stages. */
+int32_t vt_read_stage_stats(uint64_t* dst, uint32_t max_stages, uint8_t* kinds);
+/* Like vt_bench_frames, plus the mean GPU time of every launch stage (HIP events around
+ * each launch on the library stream).  Returns the number of stages or -1. */
+int32_t vt_bench_stages(uint32_t warmup, uint32_t iters, float* kernel_ms, float* wall_ms, float* stage_ms,
+                        uint32_t max_stages, uint8_t* kinds);
 /* GPU time of the last render_tick's trace kernel (HIP events on the library stream). */
 float vt_last_frame_ms(void);
 /* Renders `iters` frames after `warmup` with the last camera; per-frame kernel time from

@@ -40,6 +40,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <thread>
 #include <vector>
 
 extern "C" {
@@ -921,9 +922,30 @@ void surface_of(const Scene& S, const Hit& h, Surface& sf) {
     sf.rgba = h.d.rgba;
 }
 
+// Optional per-query work log (diagnostics for the cost distribution of secondary rays).
+struct WorkLog {
+    bool on = false;
+    std::vector<uint64_t> rec;  // per query: kind, steps, bricks, nodes, instances, hit
+} g_log;
+
 // secondary query from a surface: continue in the origin instance, then the rest of the scene
+Hit secondary_impl(const Scene& S, uint32_t origin, const Surface& sf, uint8_t origin_face, const float w[3],
+                   bool any_hit, bool use_bvh, Counters& C);
 Hit secondary(const Scene& S, uint32_t origin, const Surface& sf, uint8_t origin_face, const float w[3], bool any_hit,
               bool use_bvh, Counters& C) {
+    if (!g_log.on || !C.st) return secondary_impl(S, origin, sf, origin_face, w, any_hit, use_bvh, C);
+    orc_stats before = *C.st;
+    Hit h = secondary_impl(S, origin, sf, origin_face, w, any_hit, use_bvh, C);
+    g_log.rec.push_back(any_hit ? 1 : 2);
+    g_log.rec.push_back(C.st->steps - before.steps);
+    g_log.rec.push_back(C.st->bricks - before.bricks);
+    g_log.rec.push_back(C.st->nodes - before.nodes);
+    g_log.rec.push_back(C.st->instances - before.instances);
+    g_log.rec.push_back(h.hit ? 1 : 0);
+    return h;
+}
+Hit secondary_impl(const Scene& S, uint32_t origin, const Surface& sf, uint8_t origin_face, const float w[3],
+                   bool any_hit, bool use_bvh, Counters& C) {
     if (C.st) C.st->rays++;
     Hit h;
     if (continue_in_instance(S, origin, sf.p, sf.vstart, w, h, C)) {
@@ -985,6 +1007,9 @@ void write_hit(const Hit& h, orc_hit& out) {
 extern "C" {
 
 uint32_t orc_pcg_hash(uint32_t v) { return pcg_hash(v); }
+void orc_worklog_enable(int on) { g_log.on = on != 0; g_log.rec.clear(); }
+uint64_t orc_worklog_size(void) { return g_log.rec.size() / 6; }
+void orc_worklog_fetch(uint64_t* out) { std::memcpy(out, g_log.rec.data(), g_log.rec.size() * 8); }
 uint32_t orc_rng_seed(uint32_t seed, uint32_t frame, uint32_t pixel, uint32_t sample) {
     return rng_seed(seed, frame, pixel, sample);
 }
@@ -1069,27 +1094,23 @@ uint32_t orc_brick(const orc_volume* v, uint32_t* map_out, uint64_t* masks_out, 
     return next;
 }
 
-// Full frame.  P, C: 16 floats each (column-major, render.rs:190,208-214 conventions).
-// hits_out: W*H orc_hit.  rad_f32: W*H*4 floats (mode 0 primary colour, FP32 as the
-// GPU); rad_f64: W*H*4 doubles (modes 1/2: FP64 radiance over the same FP32 paths).
-// use_bvh selects LBVH traversal (needed for byte counters) vs brute force.
-// y0,y1 restrict to a band of rows (both inclusive-exclusive) for sampled baselines.
-int orc_render(uint32_t nvol, const orc_volume* vols, uint32_t ninst, const float* records, const float* P,
-               const float* C, const orc_params* prm, int use_bvh, uint32_t y0, uint32_t y1, orc_hit* hits_out,
-               float* rad_f32, double* rad_f64, orc_stats* stats) {
+// Persistent oracle scene: volumes prepared once (remaps, brick occupancy), instances, LBVH.
+struct OrcScene {
     Frame F;
-    if (!setup_frame(P, C, prm, F)) return -1;
     Scene S;
-    S.F = &F;
-    S.vols.resize(nvol);
-    for (uint32_t i = 0; i < nvol; ++i) init_vol(S.vols[i], &vols[i]);
-    S.inst.resize(ninst);
-    for (uint32_t i = 0; i < ninst; ++i) prep_instance(records + 16 * (size_t)i, nvol, S.inst[i]);
-    if (use_bvh) build_bvh(S.inst, S.bvh);
+    orc_params prm;
+    bool use_bvh;
+};
+
+static void render_rows(OrcScene& OS, uint32_t y0, uint32_t y1, orc_hit* hits_out, float* rad_f32, double* rad_f64,
+                        orc_stats* stats) {
+    const Scene& S = OS.S;
+    const Frame& F = OS.F;
+    const orc_params* prm = &OS.prm;
+    const int use_bvh = OS.use_bvh ? 1 : 0;
     Counters Cn{stats};
     const SrgbTables& T = srgb();
-    const uint32_t W = prm->width, H = prm->height;
-    if (y1 > H) y1 = H;
+    const uint32_t W = prm->width;
     const uint32_t spp = prm->spp ? prm->spp : 1;
     const double inv_pi = 1.0 / 3.14159265358979323846;
     const float sun[3] = {prm->sun_dir[0], prm->sun_dir[1], prm->sun_dir[2]};
@@ -1170,7 +1191,90 @@ int orc_render(uint32_t nvol, const orc_volume* vols, uint32_t ninst, const floa
                 rad_f64[4 * pix + 3] = 1.0;
             }
         }
+}
+
+void* orc_scene_create(uint32_t nvol, const orc_volume* vols, uint32_t ninst, const float* records, const float* P,
+                       const float* C, const orc_params* prm, int use_bvh) {
+    OrcScene* OS = new OrcScene();
+    OS->prm = *prm;
+    OS->use_bvh = use_bvh != 0;
+    if (!setup_frame(P, C, prm, OS->F)) {
+        delete OS;
+        return nullptr;
+    }
+    OS->S.F = &OS->F;
+    OS->S.vols.resize(nvol);
+    for (uint32_t i = 0; i < nvol; ++i) init_vol(OS->S.vols[i], &vols[i]);
+    OS->S.inst.resize(ninst);
+    for (uint32_t i = 0; i < ninst; ++i) prep_instance(records + 16 * (size_t)i, nvol, OS->S.inst[i]);
+    if (use_bvh) build_bvh(OS->S.inst, OS->S.bvh);
+    return OS;
+}
+
+void orc_scene_destroy(void* h) { delete static_cast<OrcScene*>(h); }
+
+// Renders rows [y0, y1) with `threads` host threads (interleaved rows; counters summed).
+int orc_scene_render(void* h, uint32_t y0, uint32_t y1, uint32_t threads, orc_hit* hits_out, float* rad_f32,
+                     double* rad_f64, orc_stats* stats) {
+    OrcScene& OS = *static_cast<OrcScene*>(h);
+    if (y1 > OS.prm.height) y1 = OS.prm.height;
+    if (threads <= 1 || y1 - y0 < 2) {
+        render_rows(OS, y0, y1, hits_out, rad_f32, rad_f64, stats);
+        return 0;
+    }
+    std::vector<orc_stats> part(threads);
+    std::vector<std::thread> th;
+    for (uint32_t t = 0; t < threads; ++t) {
+        part[t] = orc_stats{};
+        th.emplace_back([&, t]() {
+            for (uint32_t y = y0 + t; y < y1; y += threads)
+                render_rows(OS, y, y + 1, hits_out, rad_f32, rad_f64, stats ? &part[t] : nullptr);
+        });
+    }
+    for (auto& x : th) x.join();
+    if (stats)
+        for (auto& p : part) {
+            stats->rays += p.rays; stats->nodes += p.nodes; stats->instances += p.instances;
+            stats->bricks += p.bricks; stats->bricks_nonempty += p.bricks_nonempty; stats->hits += p.hits;
+            stats->steps += p.steps;
+        }
     return 0;
+}
+
+// Renders an explicit list of rows with `threads` host threads (rows dealt round robin).
+int orc_scene_render_rows(void* h, const uint32_t* rows, uint32_t nrows, uint32_t threads, orc_hit* hits_out,
+                          float* rad_f32, double* rad_f64, orc_stats* stats) {
+    OrcScene& OS = *static_cast<OrcScene*>(h);
+    if (threads == 0) threads = 1;
+    std::vector<orc_stats> part(threads);
+    std::vector<std::thread> th;
+    for (uint32_t t = 0; t < threads; ++t) {
+        part[t] = orc_stats{};
+        th.emplace_back([&, t]() {
+            for (uint32_t i = t; i < nrows; i += threads)
+                if (rows[i] < OS.prm.height)
+                    render_rows(OS, rows[i], rows[i] + 1, hits_out, rad_f32, rad_f64, stats ? &part[t] : nullptr);
+        });
+    }
+    for (auto& x : th) x.join();
+    if (stats)
+        for (auto& p : part) {
+            stats->rays += p.rays; stats->nodes += p.nodes; stats->instances += p.instances;
+            stats->bricks += p.bricks; stats->bricks_nonempty += p.bricks_nonempty; stats->hits += p.hits;
+            stats->steps += p.steps;
+        }
+    return 0;
+}
+
+// Full frame in one call (create, render rows [y0, y1) single-threaded, destroy).
+int orc_render(uint32_t nvol, const orc_volume* vols, uint32_t ninst, const float* records, const float* P,
+               const float* C, const orc_params* prm, int use_bvh, uint32_t y0, uint32_t y1, orc_hit* hits_out,
+               float* rad_f32, double* rad_f64, orc_stats* stats) {
+    void* h = orc_scene_create(nvol, vols, ninst, records, P, C, prm, use_bvh);
+    if (!h) return -1;
+    int rc = orc_scene_render(h, y0, y1, 1, hits_out, rad_f32, rad_f64, stats);
+    orc_scene_destroy(h);
+    return rc;
 }
 
 }  // extern "C"

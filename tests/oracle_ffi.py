@@ -71,6 +71,17 @@ def lib():
         L.orc_render.argtypes = [C.c_uint32, C.POINTER(OrcVolume), C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.POINTER(OrcParams), C.c_int, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.POINTER(OrcStats)]
+        L.orc_scene_create.restype = C.c_void_p
+        L.orc_scene_create.argtypes = [C.c_uint32, C.POINTER(OrcVolume), C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.POINTER(OrcParams), C.c_int]
+        L.orc_scene_destroy.restype = None
+        L.orc_scene_destroy.argtypes = [C.c_void_p]
+        L.orc_scene_render.restype = C.c_int
+        L.orc_scene_render.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.POINTER(OrcStats)]
+        L.orc_scene_render_rows.restype = C.c_int
+        L.orc_scene_render_rows.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.POINTER(OrcStats)]
         _lib = L
     return _lib
 
@@ -163,3 +174,55 @@ def instance_prep(records, nvol):
     lib().orc_instance_prep(n, recs.ctypes.data, nvol, inv.ctypes.data, aabb.ctypes.data, tex.ctypes.data,
                             valid.ctypes.data)
     return inv, aabb, tex, valid
+
+
+class OracleScene:
+    """A scene prepared once in the oracle (volumes, instances, LBVH); rows rendered on demand."""
+
+    def __init__(self, scene, use_bvh: bool = True, params=None):
+        self.p = params or scene.params
+        self._vols, self._keep = _vol_structs(scene.volumes)
+        self._recs = np.ascontiguousarray(scene.records, dtype=np.float32)
+        self._P = np.ascontiguousarray(scene.P, dtype=np.float32)
+        self._C = np.ascontiguousarray(scene.C, dtype=np.float32)
+        self._ps = params_struct(self.p)
+        self.h = lib().orc_scene_create(len(scene.volumes), self._vols, self._recs.shape[0], self._recs.ctypes.data,
+                                        self._P.ctypes.data, self._C.ctypes.data, C.byref(self._ps),
+                                        1 if use_bvh else 0)
+        if not self.h:
+            raise RuntimeError("orc_scene_create failed")
+
+    def render(self, rows=None, threads: int = 1):
+        W, H = self.p.width, self.p.height
+        y0, y1 = (0, H) if rows is None else rows
+        hits = np.zeros(W * H, dtype=HIT_DTYPE)
+        rad32 = np.zeros((H, W, 4), dtype=np.float32)
+        rad64 = np.zeros((H, W, 4), dtype=np.float64)
+        st = OrcStats()
+        rc = lib().orc_scene_render(self.h, y0, y1, threads, hits.ctypes.data, rad32.ctypes.data, rad64.ctypes.data,
+                                    C.byref(st))
+        if rc != 0:
+            raise RuntimeError("orc_scene_render failed")
+        return hits.reshape(H, W), (rad32 if self.p.mode == 0 else rad64), st.as_dict()
+
+    def render_rows(self, rows, threads: int = 1):
+        """Renders the listed rows (counters only; outputs of those rows are discarded)."""
+        W, H = self.p.width, self.p.height
+        r = np.ascontiguousarray(rows, dtype=np.uint32)
+        hits = np.zeros(W * H, dtype=HIT_DTYPE)
+        rad32 = np.zeros((H, W, 4), dtype=np.float32)
+        rad64 = np.zeros((H, W, 4), dtype=np.float64)
+        st = OrcStats()
+        rc = lib().orc_scene_render_rows(self.h, r.ctypes.data, len(r), threads, hits.ctypes.data, rad32.ctypes.data,
+                                         rad64.ctypes.data, C.byref(st))
+        if rc != 0:
+            raise RuntimeError("orc_scene_render_rows failed")
+        return st.as_dict()
+
+    def close(self):
+        if self.h:
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

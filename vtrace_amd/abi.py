@@ -11,7 +11,8 @@ import ctypes as C
 REFERENCE_EXPORTS = ("entry", "render_tick", "get_input_data_pointer", "add_texture", "start_update_instances",
                      "end_update_instances", "cleanup")
 EXTENSION_EXPORTS = ("vt_set_params", "vt_get_params", "vt_add_volume_indexed", "vt_read_hits", "vt_read_radiance",
-                     "vt_read_bgra8", "vt_read_stats", "vt_last_frame_ms", "vt_bench_frames", "vt_read_volume",
+                     "vt_read_bgra8", "vt_read_stats", "vt_read_stage_stats", "vt_last_frame_ms", "vt_bench_frames",
+                     "vt_bench_stages", "vt_read_volume",
                      "vt_volume_info", "vt_read_lbvh", "vt_instance_count", "vt_synchronize", "vt_last_error")
 
 
@@ -62,6 +63,11 @@ def declare(lib: C.CDLL) -> None:
         f.argtypes = [C.c_void_p, C.c_uint64]
     lib.vt_read_stats.restype = C.c_int32
     lib.vt_read_stats.argtypes = [C.c_void_p]
+    lib.vt_read_stage_stats.restype = C.c_int32
+    lib.vt_read_stage_stats.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+    lib.vt_bench_stages.restype = C.c_int32
+    lib.vt_bench_stages.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_void_p,
+                                    C.c_uint32, C.c_void_p]
     lib.vt_last_frame_ms.restype = C.c_float
     lib.vt_last_frame_ms.argtypes = []
     lib.vt_bench_frames.restype = C.c_int32

@@ -33,6 +33,7 @@ struct State {
     float* inst_dev = nullptr;
     uint32_t inst_capacity = 0;
     uint32_t inst_count = 0;
+    uint32_t bvh_depth = 0;
     hipEvent_t inst_copied = nullptr;
     LbvhWork bvh;
     // volumes (lib/memory.c:286-385)
@@ -52,7 +53,9 @@ struct State {
     void* wave_mem = nullptr;   // wavefront path state + queues
     size_t wave_cap = 0;        // pixel samples it holds
     WaveBuffers wave;
-    unsigned long long* stats = nullptr;
+    unsigned long long* stats = nullptr;  // VT_MAX_STAGES x VT_STAT_COUNT
+    uint8_t stage_kinds[VT_MAX_STAGES] = {};
+    uint32_t nstages = 0;
     float* srgb_dec = nullptr;
     float* srgb_thr = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -294,14 +297,16 @@ SceneGPU scene_gpu() {
     S.vols = g.vol_table;
     S.ninst = g.inst_count;
     S.nvol = (uint32_t)g.vols.size();
+    S.bvh_depth = g.bvh_depth;
     return S;
 }
 
 int render_frame(const FrameGPU& F) {
     if (ensure_frame_buffers() != 0) return -1;
-    if (F.stats) hipMemsetAsync(g.stats, 0, VT_STAT_COUNT * 8, g.stream);
+    if (F.stats) hipMemsetAsync(g.stats, 0, (size_t)VT_MAX_STAGES * VT_STAT_COUNT * 8, g.stream);
     hipEventRecord(g.ev0, g.stream);
-    int rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.wave, g.stats, g.stream);
+    int rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.wave, g.stats, nullptr,
+                             g.stage_kinds, &g.nstages, g.stream);
     hipEventRecord(g.ev1, g.stream);
     if (rc != 0) {
         set_err("trace launch failed", hipGetLastError());
@@ -381,7 +386,7 @@ uint64_t entry(void) {
         (e = hipEventCreate(&g.ev0)) != hipSuccess || (e = hipEventCreate(&g.ev1)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&g.inst_copied, hipEventDisableTiming)) != hipSuccess ||
         (e = hipMalloc(&g.vol_table, (size_t)kMaxTextures * sizeof(VolumeDesc))) != hipSuccess ||
-        (e = hipMalloc(&g.stats, VT_STAT_COUNT * 8)) != hipSuccess ||
+        (e = hipMalloc(&g.stats, (size_t)VT_MAX_STAGES * VT_STAT_COUNT * 8)) != hipSuccess ||
         (e = hipMalloc(&g.srgb_dec, 256 * 4)) != hipSuccess || (e = hipMalloc(&g.srgb_thr, 255 * 4)) != hipSuccess) {
         set_err("init", e);
         return ((uint64_t)(uint32_t)e << 32) | 1u;
@@ -391,7 +396,7 @@ uint64_t entry(void) {
     for (int k = 0; k < 255; ++k) thr[k] = (float)srgb_lin((k + 0.5) / 255.0);
     hipMemcpy(g.srgb_dec, dec, sizeof(dec), hipMemcpyHostToDevice);
     hipMemcpy(g.srgb_thr, thr, sizeof(thr), hipMemcpyHostToDevice);
-    hipMemset(g.stats, 0, VT_STAT_COUNT * 8);
+    hipMemset(g.stats, 0, (size_t)VT_MAX_STAGES * VT_STAT_COUNT * 8);
     vt_trace_set_tables(g.srgb_dec, g.srgb_thr);
     {
         hipDeviceProp_t prop;
@@ -480,6 +485,15 @@ int32_t end_update_instances(uint32_t instance_count) {
         set_err("lbvh build", hipGetLastError());
         return -1;
     }
+    // the tree depth selects the traversal-stack variant (LDS stack or a deep private stack)
+    uint32_t depth = 0;
+    e = hipMemcpyAsync(&depth, g.bvh.depth, 4, hipMemcpyDeviceToHost, g.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+    if (e != hipSuccess) {
+        set_err("lbvh depth", e);
+        return -1;
+    }
+    g.bvh_depth = depth;
     g.inst_count = instance_count;
     return 0;
 }
@@ -543,13 +557,35 @@ int32_t vt_read_bgra8(uint32_t* dst, uint64_t bytes) {
 int32_t vt_read_stats(uint64_t* dst7) {
     Guard G;
     if (!g.inited || !dst7) return -1;
-    hipMemcpyAsync(dst7, g.stats, VT_STAT_COUNT * 8, hipMemcpyDeviceToHost, g.stream);
-    return hipStreamSynchronize(g.stream) == hipSuccess ? 0 : -1;
+    std::vector<uint64_t> all((size_t)VT_MAX_STAGES * VT_STAT_COUNT);
+    hipMemcpyAsync(all.data(), g.stats, all.size() * 8, hipMemcpyDeviceToHost, g.stream);
+    if (hipStreamSynchronize(g.stream) != hipSuccess) return -1;
+    for (int k = 0; k < VT_STAT_COUNT; ++k) {
+        uint64_t t = 0;
+        for (uint32_t st = 0; st < VT_MAX_STAGES; ++st) t += all[(size_t)st * VT_STAT_COUNT + k];
+        dst7[k] = t;
+    }
+    return 0;
+}
+
+int32_t vt_read_stage_stats(uint64_t* dst, uint32_t max_stages, uint8_t* kinds) {
+    Guard G;
+    if (!g.inited || !dst) return -1;
+    const uint32_t n = g.nstages < max_stages ? g.nstages : max_stages;
+    hipMemcpyAsync(dst, g.stats, (size_t)n * VT_STAT_COUNT * 8, hipMemcpyDeviceToHost, g.stream);
+    if (hipStreamSynchronize(g.stream) != hipSuccess) return -1;
+    if (kinds) std::memcpy(kinds, g.stage_kinds, n);
+    return (int32_t)n;
 }
 
 float vt_last_frame_ms(void) { return g.last_ms; }
 
 int32_t vt_bench_frames(uint32_t warmup, uint32_t iters, float* kernel_ms, float* wall_ms) {
+    return vt_bench_stages(warmup, iters, kernel_ms, wall_ms, nullptr, 0, nullptr);
+}
+
+int32_t vt_bench_stages(uint32_t warmup, uint32_t iters, float* kernel_ms, float* wall_ms, float* stage_ms,
+                        uint32_t max_stages, uint8_t* kinds) {
     Guard G;
     if (!g.inited || !g.have_frame || iters == 0) return -1;
     FrameGPU F = g.last_frame;
@@ -557,33 +593,41 @@ int32_t vt_bench_frames(uint32_t warmup, uint32_t iters, float* kernel_ms, float
     for (uint32_t i = 0; i < warmup; ++i)
         if (render_frame(F) != 0) return -1;
     hipStreamSynchronize(g.stream);
-    std::vector<hipEvent_t> ev(2 * (size_t)iters);
+    const size_t per = 2 * (size_t)VT_MAX_STAGES;
+    std::vector<hipEvent_t> ev(per * iters);
     for (auto& e : ev) hipEventCreate(&e);
     hipEvent_t start, stop;
     hipEventCreate(&start);
     hipEventCreate(&stop);
+    uint8_t kk[VT_MAX_STAGES] = {};
+    uint32_t ns = 0;
     hipEventRecord(start, g.stream);
     int rc = 0;
-    for (uint32_t i = 0; i < iters && rc == 0; ++i) {
-        hipEventRecord(ev[2 * i], g.stream);
-        rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.wave, g.stats, g.stream);
-        hipEventRecord(ev[2 * i + 1], g.stream);
-    }
+    for (uint32_t i = 0; i < iters && rc == 0; ++i)
+        rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.wave, g.stats,
+                             &ev[per * i], kk, &ns, g.stream);
     hipEventRecord(stop, g.stream);
     hipEventSynchronize(stop);
-    float total = 0.0f, k = 0.0f;
+    float total = 0.0f, ksum = 0.0f;
     hipEventElapsedTime(&total, start, stop);
-    for (uint32_t i = 0; i < iters; ++i) {
-        float t = 0.0f;
-        hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]);
-        k += t;
-    }
+    std::vector<float> sm(VT_MAX_STAGES, 0.0f);
+    for (uint32_t i = 0; i < iters && rc == 0; ++i)
+        for (uint32_t st = 0; st < ns; ++st) {
+            float t = 0.0f;
+            hipEventElapsedTime(&t, ev[per * i + 2 * st], ev[per * i + 2 * st + 1]);
+            sm[st] += t;
+            ksum += t;
+        }
     for (auto& e : ev) hipEventDestroy(e);
     hipEventDestroy(start);
     hipEventDestroy(stop);
-    if (kernel_ms) *kernel_ms = k / (float)iters;
+    if (kernel_ms) *kernel_ms = ksum / (float)iters;
     if (wall_ms) *wall_ms = total / (float)iters;
-    return rc;
+    for (uint32_t st = 0; st < ns && st < max_stages; ++st) {
+        if (stage_ms) stage_ms[st] = sm[st] / (float)iters;
+        if (kinds) kinds[st] = kk[st];
+    }
+    return rc == 0 ? (int32_t)ns : -1;
 }
 
 int32_t vt_volume_info(int32_t id, uint32_t* info8) {

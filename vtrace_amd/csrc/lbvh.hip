@@ -305,6 +305,19 @@ __global__ void k_refit(uint32_t n, BvhNode* nodes, const int32_t* parent, uint3
     }
 }
 
+// Depth of the tree (edges on the longest root-leaf path): one thread per leaf walks up.
+__global__ void k_depth(uint32_t n, const int32_t* parent, uint32_t* depth_out) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t d = 0;
+    int node = parent[n - 1 + k];
+    while (node >= 0 && d < 4096u) {
+        ++d;
+        node = parent[node];
+    }
+    atomicMax(depth_out, d);
+}
+
 }  // namespace
 
 int vt_lbvh_reserve(LbvhWork& w, uint32_t n) {
@@ -319,7 +332,7 @@ int vt_lbvh_reserve(LbvhWork& w, uint32_t n) {
               hipMalloc(&w.keys[0], cap * 4) == hipSuccess && hipMalloc(&w.keys[1], cap * 4) == hipSuccess &&
               hipMalloc(&w.vals[0], cap * 4) == hipSuccess && hipMalloc(&w.vals[1], cap * 4) == hipSuccess &&
               hipMalloc(&w.parent, nn * 4) == hipSuccess && hipMalloc(&w.counters, cap * 4) == hipSuccess &&
-              hipMalloc(&w.bounds, 6 * 4) == hipSuccess;
+              hipMalloc(&w.bounds, 6 * 4) == hipSuccess && hipMalloc(&w.depth, 4) == hipSuccess;
     if (!ok) {
         vt_lbvh_free(w);
         return -1;
@@ -331,7 +344,7 @@ int vt_lbvh_reserve(LbvhWork& w, uint32_t n) {
 void vt_lbvh_free(LbvhWork& w) {
     hipFree(w.inst); hipFree(w.nodes); hipFree(w.cen);
     hipFree(w.keys[0]); hipFree(w.keys[1]); hipFree(w.vals[0]); hipFree(w.vals[1]);
-    hipFree(w.parent); hipFree(w.counters); hipFree(w.bounds);
+    hipFree(w.parent); hipFree(w.counters); hipFree(w.bounds); hipFree(w.depth);
     w = LbvhWork{};
 }
 
@@ -347,9 +360,11 @@ int vt_lbvh_build(LbvhWork& w, const float* records_dev, uint32_t n, const Volum
     hipLaunchKernelGGL(k_morton, dim3(nb), dim3(tb), 0, s, w.cen, n, w.bounds, w.keys[0], w.vals[0]);
     hipLaunchKernelGGL(k_radix_sort, dim3(1), dim3(kSortThreads), 0, s, w.keys[0], w.vals[0], w.keys[1], w.vals[1], n);
     hipLaunchKernelGGL(k_leaves, dim3(nb), dim3(tb), 0, s, w.vals[0], n, leaves_tmp, w.nodes);
+    hipMemsetAsync(w.depth, 0, 4, s);
     if (n > 1) {
         hipLaunchKernelGGL(k_karras, dim3(nb), dim3(tb), 0, s, w.keys[0], (int)n, w.nodes, w.parent, w.counters);
         hipLaunchKernelGGL(k_refit, dim3(nb), dim3(tb), 0, s, n, w.nodes, w.parent, w.counters);
+        hipLaunchKernelGGL(k_depth, dim3(nb), dim3(tb), 0, s, n, w.parent, w.depth);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

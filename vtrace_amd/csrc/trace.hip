@@ -37,7 +37,7 @@
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kStackLds = 20;   // LBVH stack entries per thread held in LDS
+constexpr int kStackLds = 16;   // LBVH stack entries per thread held in LDS
 
 // GLSL min/max as "y if y < x else x" (also fixes the NaN cases the DDA can produce)
 __device__ __forceinline__ float gmin(float x, float y) { return (y < x) ? y : x; }
@@ -97,20 +97,55 @@ struct Lds {
     __device__ __forceinline__ uint32_t& at(int k) const { return base[k * kBlock]; }
 };
 
+// LBVH traversal stack: in LDS (interleaved per thread) when the tree is shallow enough,
+// otherwise a private array (trees deeper than kStackLds - 1 levels).
+template <bool DEEP>
 struct TravStack {
     Lds lds;
-    uint32_t spill[48];
     int sp;
-    __device__ __forceinline__ void push(uint32_t x) {
-        if (sp < kStackLds) lds.at(sp) = x;
-        else spill[sp - kStackLds] = x;
-        ++sp;
-    }
-    __device__ __forceinline__ uint32_t pop() {
-        --sp;
-        return sp < kStackLds ? lds.at(sp) : spill[sp - kStackLds];
-    }
+    __device__ __forceinline__ void push(uint32_t x) { lds.at(sp++) = x; }
+    __device__ __forceinline__ uint32_t pop() { return lds.at(--sp); }
 };
+template <>
+struct TravStack<true> {
+    Lds lds;
+    int sp;
+    uint32_t a[64];
+    __device__ __forceinline__ void push(uint32_t x) { a[sp++] = x; }
+    __device__ __forceinline__ uint32_t pop() { return a[--sp]; }
+};
+
+// Loads through the global address space (pointers held in structs are otherwise generic
+// and compile to flat accesses, which also wait on LDS traffic).
+typedef unsigned int vt_u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint32_t gld(const uint32_t* p) { return *(const __attribute__((address_space(1))) uint32_t*)(p); }
+__device__ __forceinline__ float gld(const float* p) { return *(const __attribute__((address_space(1))) float*)(p); }
+__device__ __forceinline__ uint4 gld(const uint4* p) {
+    const vt_u32x4 x = *(const __attribute__((address_space(1))) vt_u32x4*)(p);
+    return make_uint4(x.x, x.y, x.z, x.w);
+}
+__device__ __forceinline__ BvhNode gld(const BvhNode* p) {
+    const uint4 a = gld(reinterpret_cast<const uint4*>(p)), b = gld(reinterpret_cast<const uint4*>(p) + 1);
+    BvhNode n;
+    n.lo[0] = __uint_as_float(a.x); n.lo[1] = __uint_as_float(a.y); n.lo[2] = __uint_as_float(a.z);
+    n.hi[0] = __uint_as_float(a.w); n.hi[1] = __uint_as_float(b.x); n.hi[2] = __uint_as_float(b.y);
+    n.left = b.z;
+    n.right = b.w;
+    return n;
+}
+
+// Stage a brick's 64 B occupancy mask (16 u32: z-slice z is words 2z, 2z+1) in the LDS slot.
+__device__ __forceinline__ void stage_mask(const uint32_t* masks, uint32_t entry, const Lds& mslot) {
+    const uint4* src = reinterpret_cast<const uint4*>(masks + (size_t)entry * 16);
+    const uint4 w0 = gld(src + 0), w1 = gld(src + 1), w2 = gld(src + 2), w3 = gld(src + 3);
+    mslot.at(0) = w0.x; mslot.at(1) = w0.y; mslot.at(2) = w0.z; mslot.at(3) = w0.w;
+    mslot.at(4) = w1.x; mslot.at(5) = w1.y; mslot.at(6) = w1.z; mslot.at(7) = w1.w;
+    mslot.at(8) = w2.x; mslot.at(9) = w2.y; mslot.at(10) = w2.z; mslot.at(11) = w2.w;
+    mslot.at(12) = w3.x; mslot.at(13) = w3.y; mslot.at(14) = w3.z; mslot.at(15) = w3.w;
+}
+__device__ __forceinline__ uint64_t mask_slice(const Lds& mslot, int z) {
+    return (uint64_t)mslot.at(2 * z) | ((uint64_t)mslot.at(2 * z + 1) << 32);
+}
 
 // ---- small vector helpers (explicit evaluation order) -------------------------------
 struct V3 {
@@ -177,6 +212,8 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
     const float sgx = gsign(mdir.x), sgy = gsign(mdir.y), sgz = gsign(mdir.z);
     const int stx = (int)sgx, sty = (int)sgy, stz = (int)sgz;
     const float dx = fabsf(len / mdir.x), dy = fabsf(len / mdir.y), dz = fabsf(len / mdir.z);
+    // float(mask) * delta of trace.frag:84 as a select: delta, or 0 * delta (0, or NaN for inf)
+    const float zx = 0.0f * dx, zy = 0.0f * dy, zz = 0.0f * dz;
     float sx = ((sgx * ((float)vx - mpos[0]) + sgx * 0.5f) + 0.5f) * dx;
     float sy = ((sgy * ((float)vy - mpos[1]) + sgy * 0.5f) + 0.5f) * dy;
     float sz = ((sgz * ((float)vz - mpos[2]) + sgz * 0.5f) + 0.5f) * dz;
@@ -186,13 +223,28 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
     uint32_t steps = 0, mask = 0, entry = VT_EMPTY, li = 0;
     float t_last = 0.0f;
     bool hit = false;
+#define VT_DDA_STEP()                                             \
+    {                                                             \
+        const bool mx = sx <= gmin(sy, sz);                       \
+        const bool my = sy <= gmin(sz, sx);                       \
+        const bool mz = sz <= gmin(sx, sy);                       \
+        t_last = mx ? sx : (my ? sy : sz);                        \
+        sx += mx ? dx : zx;                                       \
+        sy += my ? dy : zy;                                       \
+        sz += mz ? dz : zz;                                       \
+        vx += mx ? stx : 0;                                       \
+        vy += my ? sty : 0;                                       \
+        vz += mz ? stz : 0;                                       \
+        mask = (mx ? 1u : 0u) | (my ? 2u : 0u) | (mz ? 4u : 0u);  \
+        ++steps;                                                  \
+    }
     while (steps < max_steps && (uint32_t)vx < (uint32_t)nx && (uint32_t)vy < (uint32_t)ny &&
            (uint32_t)vz < (uint32_t)nz) {
         // entering a brick: the current voxel is inside the volume
         const int bx8 = vx & ~7, by8 = vy & ~7, bz8 = vz & ~7;
         // extent of this brick inside the volume, per axis (the loop stays in brick and volume)
         const uint32_t ex = (uint32_t)min(8, nx - bx8), ey = (uint32_t)min(8, ny - by8), ez = (uint32_t)min(8, nz - bz8);
-        entry = V.brick_map[(uint32_t)(bx8 >> 3) + bdx * ((uint32_t)(by8 >> 3) + bdy * (uint32_t)(bz8 >> 3))];
+        entry = gld(V.brick_map + (uint32_t)(bx8 >> 3) + bdx * ((uint32_t)(by8 >> 3) + bdy * (uint32_t)(bz8 >> 3)));
         ctr.brick(entry != VT_EMPTY);
         if (entry == VT_EMPTY) {
             do {
@@ -201,21 +253,18 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
                      (uint32_t)(vz - bz8) < ez);
             continue;
         }
-        // stage the brick's 64 B occupancy mask in this thread's LDS slot
-        {
-            const uint4* src = reinterpret_cast<const uint4*>(V.masks + (size_t)entry * 16);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint4 w4 = src[q];
-                mslot.at(4 * q + 0) = w4.x;
-                mslot.at(4 * q + 1) = w4.y;
-                mslot.at(4 * q + 2) = w4.z;
-                mslot.at(4 * q + 3) = w4.w;
-            }
-        }
+        stage_mask(V.masks, entry, mslot);
+        int cur_z = -1;
+        uint64_t slice = 0;
         for (;;) {
-            li = (uint32_t)(vx - bx8) + 8u * (uint32_t)(vy - by8) + 64u * (uint32_t)(vz - bz8);
-            if ((mslot.at((int)(li >> 5)) >> (li & 31u)) & 1u) {
+            const int lz = vz - bz8;
+            if (lz != cur_z) {
+                cur_z = lz;
+                slice = mask_slice(mslot, lz);
+            }
+            const uint32_t bit = (uint32_t)(vx - bx8) + 8u * (uint32_t)(vy - by8);
+            if ((slice >> bit) & 1ull) {
+                li = bit + 64u * (uint32_t)lz;
                 hit = true;
                 break;
             }
@@ -226,6 +275,7 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
         }
         if (hit) break;
     }
+#undef VT_DDA_STEP
     ctr.steps(steps);
     r.v[0] = vx; r.v[1] = vy; r.v[2] = vz;
     r.steps = steps;
@@ -271,7 +321,7 @@ __device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], 
     for (uint32_t it = 0; it < guard; ++it) {
         if (!((uint32_t)v[0] < (uint32_t)n[0] && (uint32_t)v[1] < (uint32_t)n[1] && (uint32_t)v[2] < (uint32_t)n[2])) break;
         const int b8[3] = {v[0] & ~7, v[1] & ~7, v[2] & ~7};
-        entry = V.brick_map[(uint32_t)(b8[0] >> 3) + bdx * ((uint32_t)(b8[1] >> 3) + bdy * (uint32_t)(b8[2] >> 3))];
+        entry = gld(V.brick_map + (uint32_t)(b8[0] >> 3) + bdx * ((uint32_t)(b8[1] >> 3) + bdy * (uint32_t)(b8[2] >> 3)));
         ctr.brick(entry != VT_EMPTY);
         if (entry == VT_EMPTY) {
             float t[3];
@@ -305,24 +355,22 @@ __device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], 
             ++steps;
             continue;
         }
-        {
-            const uint4* src = reinterpret_cast<const uint4*>(V.masks + (size_t)entry * 16);
-#pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                const uint4 w4 = src[w];
-                mslot.at(4 * w + 0) = w4.x;
-                mslot.at(4 * w + 1) = w4.y;
-                mslot.at(4 * w + 2) = w4.z;
-                mslot.at(4 * w + 3) = w4.w;
-            }
-        }
+        stage_mask(V.masks, entry, mslot);
+        int cur_z = -1;
+        uint64_t slice = 0;
         float sx = ((sg[0] * ((float)v[0] - q[0]) + sg[0] * 0.5f) + 0.5f) * dl[0];
         float sy = ((sg[1] * ((float)v[1] - q[1]) + sg[1] * 0.5f) + 0.5f) * dl[1];
         float sz = ((sg[2] * ((float)v[2] - q[2]) + sg[2] * 0.5f) + 0.5f) * dl[2];
         float tl = 0.0f;
         for (;;) {
-            li = (uint32_t)(v[0] - b8[0]) + 8u * (uint32_t)(v[1] - b8[1]) + 64u * (uint32_t)(v[2] - b8[2]);
-            if ((mslot.at((int)(li >> 5)) >> (li & 31u)) & 1u) {
+            const int lz = v[2] - b8[2];
+            if (lz != cur_z) {
+                cur_z = lz;
+                slice = mask_slice(mslot, lz);
+            }
+            const uint32_t bit = (uint32_t)(v[0] - b8[0]) + 8u * (uint32_t)(v[1] - b8[1]);
+            if ((slice >> bit) & 1ull) {
+                li = bit + 64u * (uint32_t)lz;
                 hit = true;
                 t_last = tl;
                 break;
@@ -438,10 +486,10 @@ template <bool PRIMARY, bool ST>
 __device__ bool enter_instance(const SceneGPU& S, const FrameGPU& F, uint32_t i, V3 O, V3 D, HitG& h, Ctr<ST>& ctr,
                                const Lds& mslot) {
     const InstanceGPU* I = &S.inst[i];
-    if (!I->valid) return false;
+    if (!gld(&I->valid)) return false;
     float inv[12];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) inv[k] = I->inv[k];
+    for (int k = 0; k < 12; ++k) inv[k] = gld(&I->inv[k]);
     const V3 o = xf34(inv, O, 1.0f);
     const V3 d = xf34(inv, D, 0.0f);
     SlabR s = slab_unit(o, d);
@@ -461,7 +509,7 @@ __device__ bool enter_instance(const SceneGPU& S, const FrameGPU& F, uint32_t i,
     if (PRIMARY) {
         float M[12];
 #pragma unroll
-        for (int k = 0; k < 12; ++k) M[k] = I->M[k];
+        for (int k = 0; k < 12; ++k) M[k] = gld(&I->M[k]);
         const V3 w = xf34(M, V3{mp[0], mp[1], mp[2]}, 1.0f);
         const float4 sp = mv4(F.PC, make_float4(w.x, w.y, w.z, 1.0f));
         if (!(sp.z >= 0.0f && sp.z <= sp.w)) return false;
@@ -506,7 +554,7 @@ __device__ __forceinline__ bool better(const HitG& c, const HitG& b) {
 __device__ __forceinline__ float prune_bound(float t) { return t + fabsf(t) * 0.0009765625f + 0.0009765625f; }
 
 // Closest (or any) hit over all instances except `skip`, through the LBVH.
-template <bool PRIMARY, bool ANY, bool ST>
+template <bool PRIMARY, bool ANY, bool ST, bool DEEP>
 __device__ HitG query(const SceneGPU& S, const FrameGPU& F, V3 O, V3 D, uint32_t skip, Ctr<ST>& ctr,
                       const Lds& stk_lds, const Lds& mslot) {
     HitG best;
@@ -515,13 +563,13 @@ __device__ HitG query(const SceneGPU& S, const FrameGPU& F, V3 O, V3 D, uint32_t
     best.tn = INFINITY;
     if (S.ninst == 0) return best;
     const V3 inv_d{1.0f / D.x, 1.0f / D.y, 1.0f / D.z};
-    TravStack stk;
+    TravStack<DEEP> stk;
     stk.lds = stk_lds;
     stk.sp = 0;
     stk.push(0u);
     while (stk.sp > 0) {
         const uint32_t ni = stk.pop();
-        const BvhNode N = S.nodes[ni];
+        const BvhNode N = gld(S.nodes + ni);
         ctr.node();
         float tn, tf;
         if (!slab_node(O, D, inv_d, N, &tn, &tf) || tf < 0.0f) continue;
@@ -573,7 +621,7 @@ __device__ void surface_of(const SceneGPU& S, const HitG& h, SurfaceRec& sf) {
     }
     float M[12], inv[12];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) { M[k] = I->M[k]; inv[k] = I->inv[k]; }
+    for (int k = 0; k < 12; ++k) { M[k] = gld(&I->M[k]); inv[k] = gld(&I->inv[k]); }
     const V3 mp{sf.p[0] / I->size[0] - 0.5f, sf.p[1] / I->size[1] - 0.5f, sf.p[2] / I->size[2] - 0.5f};
     const V3 X = xf34(M, mp, 1.0f);
     const float nm = neg ? -1.0f : 1.0f;
@@ -585,7 +633,7 @@ __device__ void surface_of(const SceneGPU& S, const HitG& h, SurfaceRec& sf) {
     sf.face = h.face;
 }
 
-template <bool ANY, bool ST>
+template <bool ANY, bool ST, bool DEEP>
 __device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const SurfaceRec& sf, V3 w, Ctr<ST>& ctr,
                           const Lds& stk_lds, const Lds& mslot) {
     ctr.ray();
@@ -595,7 +643,7 @@ __device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const SurfaceRec
     const InstanceGPU* I = &S.inst[origin];
     float inv[12];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) inv[k] = I->inv[k];
+    for (int k = 0; k < 12; ++k) inv[k] = gld(&I->inv[k]);
     const V3 mdir = xf34(inv, w, 0.0f);
     ctr.instance();
     const VolumeDesc V = S.vols[I->tex];
@@ -617,7 +665,7 @@ __device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const SurfaceRec
         ctr.hit();
         return h;
     }
-    h = query<false, ANY, ST>(S, F, V3{sf.X[0], sf.X[1], sf.X[2]}, w, origin, ctr, stk_lds, mslot);
+    h = query<false, ANY, ST, DEEP>(S, F, V3{sf.X[0], sf.X[1], sf.X[2]}, w, origin, ctr, stk_lds, mslot);
     if (h.hit) ctr.hit();
     return h;
 }
@@ -740,7 +788,7 @@ __device__ __forceinline__ bool work_pixel(const FrameGPU& F, uint32_t w, uint32
     return *px < F.width && *py < F.height;
 }
 
-template <bool ST>
+template <bool ST, bool DEEP>
 __global__ __launch_bounds__(kBlock) void k_primary(const FrameGPU F, const SceneGPU S, const float* __restrict__ srgb_dec,
                                                     HitRecord* __restrict__ hits, WF wf, uint32_t nwork,
                                                     unsigned long long* stats) {
@@ -762,7 +810,7 @@ __global__ __launch_bounds__(kBlock) void k_primary(const FrameGPU F, const Scen
         }
         const V3 D = camera_ray(F, (float)px + jx, (float)py + jy);
         ctr.ray();
-        HitG h = query<true, false, ST>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, VT_EMPTY, ctr, stk, mslot);
+        HitG h = query<true, false, ST, DEEP>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, VT_EMPTY, ctr, stk, mslot);
         float4 L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float beta[3] = {1.0f, 1.0f, 1.0f};
         uint8_t shadowed = 0;
@@ -810,7 +858,7 @@ __global__ __launch_bounds__(kBlock) void k_primary(const FrameGPU F, const Scen
     if (ST) ctr.flush(stats);
 }
 
-template <bool ST>
+template <bool ST, bool DEEP>
 __global__ __launch_bounds__(kBlock) void k_shadow(const FrameGPU F, const SceneGPU S, WF wf, uint32_t q, uint32_t depth,
                                                    uint32_t fetch_idx, unsigned long long* stats) {
     __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
@@ -822,7 +870,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(const FrameGPU F, const Scene
         const float4 c = wf.contrib[q][slot];
         if (c.w == 0.0f) return;
         const SurfaceRec sf = wf.surf[q][slot];
-        HitG h = secondary<true, ST>(S, F, sf, V3{F.sun[0], F.sun[1], F.sun[2]}, ctr, stk, mslot);
+        HitG h = secondary<true, ST, DEEP>(S, F, sf, V3{F.sun[0], F.sun[1], F.sun[2]}, ctr, stk, mslot);
         if (!h.hit) {
             float4 L = wf.L[sf.path];
             L.x = L.x + c.x;
@@ -836,7 +884,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(const FrameGPU F, const Scene
     if (ST) ctr.flush(stats);
 }
 
-template <bool ST>
+template <bool ST, bool DEEP>
 __global__ __launch_bounds__(kBlock) void k_bounce(const FrameGPU F, const SceneGPU S, const float* __restrict__ srgb_dec,
                                                    WF wf, uint32_t q, uint32_t depth, uint32_t fetch_idx,
                                                    unsigned long long* stats) {
@@ -851,7 +899,7 @@ __global__ __launch_bounds__(kBlock) void k_bounce(const FrameGPU F, const Scene
         if (dw.w == 0.0f) return;
         const SurfaceRec sf = wf.surf[q][slot];
         const uint32_t path = sf.path;
-        HitG h = secondary<false, ST>(S, F, sf, V3{dw.x, dw.y, dw.z}, ctr, stk, mslot);
+        HitG h = secondary<false, ST, DEEP>(S, F, sf, V3{dw.x, dw.y, dw.z}, ctr, stk, mslot);
         const float4 b4 = wf.beta[path];
         if (!h.hit) {
             float4 L = wf.L[path];
@@ -903,7 +951,7 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
 }
 
 // Mode 0: one thread per pixel, samples looped in-thread (primary colour only).
-template <bool ST>
+template <bool ST, bool DEEP>
 __global__ __launch_bounds__(kBlock) void k_render0(const FrameGPU F, const SceneGPU S, const float* __restrict__ srgb_dec,
                                                     HitRecord* __restrict__ hits, float4* __restrict__ rad,
                                                     uint32_t* fetch, uint32_t nwork, unsigned long long* stats) {
@@ -929,7 +977,7 @@ __global__ __launch_bounds__(kBlock) void k_render0(const FrameGPU F, const Scen
             }
             const V3 D = camera_ray(F, (float)px + jx, (float)py + jy);
             ctr.ray();
-            HitG h = query<true, false, ST>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, VT_EMPTY, ctr, stk, mslot);
+            HitG h = query<true, false, ST, DEEP>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, VT_EMPTY, ctr, stk, mslot);
             if (h.hit) {
                 ctr.hit();
                 uint8_t pal = 0;
@@ -1030,40 +1078,71 @@ static WF to_wf(const WaveBuffers& wb) {
     return w;
 }
 
-template <bool ST>
+template <bool ST, bool DEEP>
 static int launch_all(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, const WaveBuffers& wb,
-                      unsigned long long* stats, hipStream_t s) {
+                      unsigned long long* stats, hipEvent_t* ev, uint8_t* kinds, uint32_t* nstages, hipStream_t s) {
     const uint32_t spp = F.spp ? F.spp : 1u;
     const uint32_t tiles = ((F.width + 7u) / 8u) * ((F.height + 7u) / 8u);
     const dim3 grid(g_persistent_blocks), block(kBlock);
     WF wf = to_wf(wb);
+    uint32_t stage = 0;
+    auto begin = [&](uint8_t kind) {
+        if (ev) hipEventRecord(ev[2 * stage], s);
+        if (kinds) kinds[stage] = kind;
+    };
+    auto end = [&]() {
+        if (ev) hipEventRecord(ev[2 * stage + 1], s);
+        ++stage;
+    };
+    auto st_at = [&](uint32_t k) { return stats ? stats + (size_t)k * VT_STAT_COUNT : nullptr; };
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, wb.counters, 64u);
     if (F.mode == 0) {
-        hipLaunchKernelGGL((k_render0<ST>), grid, block, 0, s, F, S, g_srgb_dec, hits, rad, &wb.counters[2],
-                           tiles * 64u, stats);
+        begin(VT_STAGE_RENDER0);
+        hipLaunchKernelGGL((k_render0<ST, DEEP>), grid, block, 0, s, F, S, g_srgb_dec, hits, rad, &wb.counters[2],
+                           tiles * 64u, st_at(stage));
+        end();
+        if (nstages) *nstages = stage;
         return hipGetLastError() == hipSuccess ? 0 : -1;
     }
     const uint32_t nwork = tiles * 64u * spp;
     if ((size_t)F.width * F.height * spp > wb.npaths) return -1;
-    hipLaunchKernelGGL((k_primary<ST>), grid, block, 0, s, F, S, g_srgb_dec, hits, wf, nwork, stats);
+    begin(VT_STAGE_PRIMARY);
+    hipLaunchKernelGGL((k_primary<ST, DEEP>), grid, block, 0, s, F, S, g_srgb_dec, hits, wf, nwork, st_at(stage));
+    end();
     const uint32_t max_b = F.mode == 2 ? F.max_bounces : 0u;
     uint32_t fetch = 3;
     for (uint32_t d = 0; d <= max_b && fetch + 2 < 64; ++d) {
         const uint32_t q = d & 1u;
-        hipLaunchKernelGGL((k_shadow<ST>), grid, block, 0, s, F, S, wf, q, d, fetch++, stats);
-        if (d < max_b) hipLaunchKernelGGL((k_bounce<ST>), grid, block, 0, s, F, S, g_srgb_dec, wf, q, d, fetch++, stats);
+        begin(VT_STAGE_SHADOW);
+        hipLaunchKernelGGL((k_shadow<ST, DEEP>), grid, block, 0, s, F, S, wf, q, d, fetch++, st_at(stage));
+        end();
+        if (d < max_b) {
+            begin(VT_STAGE_BOUNCE);
+            hipLaunchKernelGGL((k_bounce<ST, DEEP>), grid, block, 0, s, F, S, g_srgb_dec, wf, q, d, fetch++,
+                               st_at(stage));
+            end();
+        }
     }
     const uint32_t npix = F.width * F.height;
+    begin(VT_STAGE_ACCUMULATE);
     hipLaunchKernelGGL(k_accumulate, dim3((npix + 255) / 256), dim3(256), 0, s, F, wf, hits, rad);
+    end();
+    if (nstages) *nstages = stage;
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, const WaveBuffers& wb,
-                    unsigned long long* stats, hipStream_t s) {
+                    unsigned long long* stats, hipEvent_t* ev, uint8_t* kinds, uint32_t* nstages, hipStream_t s) {
     if (!g_srgb_dec || F.mode > 2) return -1;
-    if (F.max_bounces > 29) return -1;  // fetch counters available per frame
+    if (F.max_bounces > 29) return -1;  // stages and fetch counters available per frame
     const bool st = F.stats != 0 && stats != nullptr;
-    return st ? launch_all<true>(F, S, hits, rad, wb, stats, s) : launch_all<false>(F, S, hits, rad, wb, stats, s);
+    // the LDS stack holds kStackLds entries; a DFS needs at most depth + 1
+    const bool deep = S.bvh_depth + 1u > (uint32_t)kStackLds;
+    if (deep)
+        return st ? launch_all<true, true>(F, S, hits, rad, wb, stats, ev, kinds, nstages, s)
+                  : launch_all<false, true>(F, S, hits, rad, wb, stats, ev, kinds, nstages, s);
+    return st ? launch_all<true, false>(F, S, hits, rad, wb, stats, ev, kinds, nstages, s)
+              : launch_all<false, false>(F, S, hits, rad, wb, stats, ev, kinds, nstages, s);
 }
 
 int vt_encode_srgb8(const float4* rad, uint32_t* bgra, uint32_t n, hipStream_t s) {

@@ -38,6 +38,7 @@ struct LbvhWork {
     int32_t* parent = nullptr;  // 2*cap - 1
     uint32_t* counters = nullptr;  // cap
     float* bounds = nullptr;    // 6
+    uint32_t* depth = nullptr;  // tree depth (edges on the longest root-leaf path)
 };
 int vt_lbvh_reserve(LbvhWork& w, uint32_t n);
 void vt_lbvh_free(LbvhWork& w);
@@ -60,8 +61,11 @@ size_t vt_wave_bytes(uint32_t npaths);
 int vt_wave_bind(WaveBuffers& wb, void* mem, uint32_t npaths);
 void vt_trace_set_tables(const float* dec_dev, const float* thr_dev);
 void vt_trace_set_persistent_blocks(uint32_t blocks);
+// Launches one frame.  stats: VT_MAX_STAGES x VT_STAT_COUNT counters (stage k at k*VT_STAT_COUNT).
+// ev (optional): 2 events per stage recorded around each launch.  kinds/nstages describe the
+// stages launched.
 int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, const WaveBuffers& wb,
-                    unsigned long long* stats, hipStream_t s);
+                    unsigned long long* stats, hipEvent_t* ev, uint8_t* kinds, uint32_t* nstages, hipStream_t s);
 int vt_encode_srgb8(const float4* rad, uint32_t* bgra, uint32_t n, hipStream_t s);
 
 // host math shared by the runtime (frame setup); FP64 inverse rounded to FP32.

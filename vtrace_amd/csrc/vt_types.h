@@ -80,9 +80,15 @@ struct SceneGPU {
     const VolumeDesc* vols;
     uint32_t ninst;
     uint32_t nvol;
+    uint32_t bvh_depth;         // selects the traversal-stack variant
 };
 
-// stats buffer layout (u64 each)
+// Launch stages of one frame (stats and timings are kept per stage):
+// mode 0: [render0]; modes 1/2: [primary, shadow d0, bounce d0, ..., shadow dB, accumulate].
+#define VT_MAX_STAGES 64
+enum VtStageKind { VT_STAGE_RENDER0 = 0, VT_STAGE_PRIMARY, VT_STAGE_SHADOW, VT_STAGE_BOUNCE, VT_STAGE_ACCUMULATE };
+
+// stats buffer layout (u64 each, VT_STAT_COUNT per stage)
 enum VtStat {
     VT_STAT_RAYS = 0,
     VT_STAT_NODES,

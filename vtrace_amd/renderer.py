@@ -228,6 +228,27 @@ class Renderer:
             raise RendererError(self.error())
         return {k: int(v) for k, v in zip(STAT_NAMES, out)}
 
+    STAGE_KINDS = ("render", "primary", "shadow", "bounce", "accumulate")
+
+    def stage_stats(self) -> list:
+        """Per-launch-stage counters of the last counted frame: [(kind, {counter: value})]."""
+        out = np.zeros((64, 7), dtype=np.uint64)
+        kinds = np.zeros(64, dtype=np.uint8)
+        n = self.lib.vt_read_stage_stats(out.ctypes.data, 64, kinds.ctypes.data)
+        if n < 0:
+            raise RendererError(self.error())
+        return [(self.STAGE_KINDS[int(kinds[i])], {k: int(v) for k, v in zip(STAT_NAMES, out[i])}) for i in range(n)]
+
+    def bench_stages(self, warmup: int, iters: int):
+        """(kernel_ms, wall_ms, [(kind, mean ms)]) over `iters` frames."""
+        k, w = C.c_float(0), C.c_float(0)
+        sm = np.zeros(64, dtype=np.float32)
+        kinds = np.zeros(64, dtype=np.uint8)
+        n = self.lib.vt_bench_stages(warmup, iters, C.byref(k), C.byref(w), sm.ctypes.data, 64, kinds.ctypes.data)
+        if n < 0:
+            raise RendererError(self.error())
+        return k.value, w.value, [(self.STAGE_KINDS[int(kinds[i])], float(sm[i])) for i in range(n)]
+
     def last_frame_ms(self) -> float:
         return float(self.lib.vt_last_frame_ms())
 
