@@ -1024,6 +1024,25 @@ uint32_t orc_srgb_encode(float lin) {
 }
 uint32_t orc_nearest_texel(int v, uint32_t dim) { return nearest_texel(v, (float)dim, dim); }
 
+// DDA test hooks: run the reference DDA (level 0) or the two-level secondary DDA (level 1)
+// on one volume from position mpos / start voxel v0 along mdir.  out: hit, x, y, z, steps, mask.
+int orc_dda_test(const orc_volume* v, int level, const float* mpos, const int* v0, const float* mdir, int* out,
+                 float* t_last_q) {
+    Vol V;
+    init_vol(V, v);
+    Counters C{nullptr};
+    DdaResult r = level == 0 ? dda(V, mpos, v0, mdir, C) : dda2(V, mpos, v0, mdir, C);
+    out[0] = r.hit ? 1 : 0;
+    out[1] = r.v[0]; out[2] = r.v[1]; out[3] = r.v[2];
+    out[4] = (int)r.steps;
+    out[5] = (int)r.mask;
+    if (t_last_q) {
+        t_last_q[0] = r.t_last;
+        t_last_q[1] = r.q[0]; t_last_q[2] = r.q[1]; t_last_q[3] = r.q[2];
+    }
+    return 0;
+}
+
 // Instance preparation: records (16 floats each, render.rs:74-78 layout) ->
 // inverse matrices, padded world AABBs (lo,hi), texture ids, valid flags.
 void orc_instance_prep(uint32_t n, const float* records, uint32_t nvol, float* inv_out, float* aabb_out,

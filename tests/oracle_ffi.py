@@ -82,6 +82,9 @@ def lib():
         L.orc_scene_render_rows.restype = C.c_int
         L.orc_scene_render_rows.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                             C.c_void_p, C.POINTER(OrcStats)]
+        L.orc_dda_test.restype = C.c_int
+        L.orc_dda_test.argtypes = [C.POINTER(OrcVolume), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]
         _lib = L
     return _lib
 
@@ -226,3 +229,16 @@ class OracleScene:
 
     def __del__(self):
         self.close()
+
+
+def dda(volume, level, mpos, v0, mdir):
+    """Oracle DDA on one volume: level 0 = reference DDA, 1 = two-level secondary DDA.
+    Returns (hit, (x, y, z), steps, mask, t_last, q)."""
+    vols, keep = _vol_structs([volume])
+    mp = np.ascontiguousarray(mpos, dtype=np.float32)
+    vv = np.ascontiguousarray(v0, dtype=np.int32)
+    md = np.ascontiguousarray(mdir, dtype=np.float32)
+    out = np.zeros(6, dtype=np.int32)
+    tq = np.zeros(4, dtype=np.float32)
+    lib().orc_dda_test(vols, level, mp.ctypes.data, vv.ctypes.data, md.ctypes.data, out.ctypes.data, tq.ctypes.data)
+    return bool(out[0]), (int(out[1]), int(out[2]), int(out[3])), int(out[4]), int(out[5]), float(tq[0]), tq[1:].copy()
