@@ -106,8 +106,14 @@ void default_params(vt_render_params& p) {
     p.sky[0] = 0.53f; p.sky[1] = 0.81f; p.sky[2] = 0.92f;  // lib/command.c:57-60
 }
 
+// Path-state capacity of a frame: pixel samples padded to whole 64-pixel tiles of any shape
+// (tw x th = 64 pads at most 64 * (W + H) + 64 pixels).
+size_t padded_paths(uint32_t w, uint32_t h, uint32_t spp) {
+    return ((size_t)w * h + 64 * ((size_t)w + h) + 64) * (spp ? spp : 1u);
+}
+
 int ensure_wave_buffers() {
-    const size_t paths = (size_t)g.params.width * g.params.height * (g.params.spp ? g.params.spp : 1u);
+    const size_t paths = padded_paths(g.params.width, g.params.height, g.params.spp);
     if (g.params.mode == 0) return 0;
     if (paths <= g.wave_cap && g.wave_mem) return 0;
     if (paths > 0xFFFFFFFFull / 2) {
@@ -448,6 +454,7 @@ void cleanup(void) {
     hipEventDestroy(g.ev0); hipEventDestroy(g.ev1); hipEventDestroy(g.inst_copied);
     hipStreamDestroy(g.stream);
     vt_trace_set_tables(nullptr, nullptr);
+    vt_trace_release();
     g = State{};
 }
 
@@ -581,7 +588,7 @@ int32_t vt_read_stage_stats(uint64_t* dst, uint32_t max_stages, uint8_t* kinds) 
 float vt_last_frame_ms(void) { return g.last_ms; }
 
 int32_t vt_bench_frames(uint32_t warmup, uint32_t iters, float* kernel_ms, float* wall_ms) {
-    return vt_bench_stages(warmup, iters, kernel_ms, wall_ms, nullptr, 0, nullptr);
+    return vt_bench_stages(warmup, iters, kernel_ms, wall_ms, nullptr, 0, nullptr) < 0 ? -1 : 0;
 }
 
 int32_t vt_bench_stages(uint32_t warmup, uint32_t iters, float* kernel_ms, float* wall_ms, float* stage_ms,

@@ -32,6 +32,8 @@
 // (instance, voxel, face, palette index, steps) bit-exact with the CPU oracle.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "vt_internal.h"
 
 namespace {
@@ -775,7 +777,7 @@ __device__ __forceinline__ void shade(const FrameGPU& F, const float* srgb_dec, 
     }
 }
 
-// Work index -> (pixel, sample): waves cover an 8 x 8 pixel tile at one sample index.
+// Work index -> (pixel, sample) for mode 0: waves cover an 8 x 8 pixel tile.
 __device__ __forceinline__ bool work_pixel(const FrameGPU& F, uint32_t w, uint32_t spp, uint32_t* px, uint32_t* py,
                                            uint32_t* s) {
     const uint32_t tiles_x = (F.width + 7u) / 8u;
@@ -788,20 +790,68 @@ __device__ __forceinline__ bool work_pixel(const FrameGPU& F, uint32_t w, uint32
     return *px < F.width && *py < F.height;
 }
 
+// ---- batched wavefront ----------------------------------------------------------------------
+// Request k of a batch owns the global work items [work_base[k], work_base[k+1]); work item
+// w is pixel sample (pixel, s) of that request and also the index of its path state, so a
+// wave's 64 consecutive items are one pixel tile at one sample of one request.
+
+// request owning global index x in a prefix table base[0..n]
+__device__ __forceinline__ uint32_t find_req(const uint32_t* base, uint32_t n, uint32_t x) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (base[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+struct TileMap {
+    uint32_t tw, th, tiles_x, ns;
+    __device__ __forceinline__ TileMap(const ReqGPU& R) {
+        tw = R.tile_w;
+        th = 64u / tw;
+        tiles_x = (R.F.width + tw - 1u) / tw;
+        ns = R.s1 - R.s0;
+    }
+    // local work index -> pixel and sample
+    __device__ __forceinline__ void pixel(uint32_t lw, uint32_t s0, uint32_t* px, uint32_t* py, uint32_t* s) const {
+        const uint32_t per_tile = 64u * ns;
+        const uint32_t tile = lw / per_tile, rem = lw % per_tile;
+        *s = s0 + rem / 64u;
+        const uint32_t pt = rem % 64u;
+        *px = (tile % tiles_x) * tw + pt % tw;
+        *py = (tile / tiles_x) * th + pt / tw;
+    }
+    // pixel and sample -> local work index
+    __device__ __forceinline__ uint32_t index(uint32_t px, uint32_t py, uint32_t s, uint32_t s0) const {
+        const uint32_t tile = (py / th) * tiles_x + px / tw;
+        const uint32_t pt = (py % th) * tw + px % tw;
+        return tile * (64u * ns) + (s - s0) * 64u + pt;
+    }
+};
+
 template <bool ST, bool DEEP>
-__global__ __launch_bounds__(kBlock) void k_primary(const FrameGPU F, const SceneGPU S, const float* __restrict__ srgb_dec,
-                                                    HitRecord* __restrict__ hits, WF wf, uint32_t nwork,
+__global__ __launch_bounds__(kBlock) void k_primary(const ReqGPU* __restrict__ reqs, const BatchGPU* __restrict__ B,
+                                                    const float* __restrict__ srgb_dec, WF wf,
                                                     unsigned long long* stats) {
     __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
     const Lds stk{&smem[threadIdx.x]}, mslot{&smem[kStackLds * kBlock + threadIdx.x]};
     Ctr<ST> ctr;
-    const uint32_t spp = F.spp ? F.spp : 1u;
-    const uint32_t max_b = F.mode == 2 ? F.max_bounces : 0u;
+    const uint32_t nreq = B->nreq;
+    const uint32_t nwork = B->work_base[nreq];
     for_each_item(&wf.counters[2], nwork, [&](uint32_t w) {
+        // the 64 items of a wave share a request (work ranges are 64-aligned)
+        const uint32_t r = __builtin_amdgcn_readfirstlane(find_req(B->work_base, nreq, w));
+        const ReqGPU& Rq = reqs[r];
+        const FrameGPU& F = Rq.F;
+        const SceneGPU S = Rq.S;
+        const TileMap tm(Rq);
         uint32_t px, py, smp;
-        if (!work_pixel(F, w, spp, &px, &py, &smp)) return;
+        tm.pixel(w - B->work_base[r], Rq.s0, &px, &py, &smp);
+        if (px >= F.width || py >= F.height) return;
         const uint32_t pix = py * F.width + px;
-        const uint32_t path = pix * spp + smp;
+        const uint32_t max_b = F.mode == 2 ? F.max_bounces : 0u;
         Rng rng{pixel_seed(F.seed, F.frame, pix, smp)};
         float jx = 0.5f, jy = 0.5f;
         if (F.jitter) {
@@ -820,7 +870,7 @@ __global__ __launch_bounds__(kBlock) void k_primary(const FrameGPU F, const Scen
                 HitRecord rec;
                 rec.inst = VT_EMPTY; rec.voxel = VT_EMPTY; rec.steps = 0; rec.face = 0xFF;
                 rec.pal = 0; rec.mask = 0; rec.flags = 0; rec.pad = 0;
-                hits[pix] = rec;
+                Rq.hits[pix] = rec;
             }
         } else {
             ctr.hit();
@@ -837,10 +887,10 @@ __global__ __launch_bounds__(kBlock) void k_primary(const FrameGPU F, const Scen
                 rec.mask = (uint8_t)h.d.mask;
                 rec.flags = 1;
                 rec.pad = 0;
-                hits[pix] = rec;
+                Rq.hits[pix] = rec;
             }
             surface_of(S, h, sf);
-            sf.path = path;
+            sf.path = w;
             float4 contrib, dirw;
             bool no_sun;
             shade(F, srgb_dec, sf, 0u, max_b, beta, rng, contrib, dirw, no_sun);
@@ -850,27 +900,31 @@ __global__ __launch_bounds__(kBlock) void k_primary(const FrameGPU F, const Scen
             wf.contrib[0][slot] = contrib;
             wf.dir[0][slot] = dirw;
         }
-        wf.L[path] = L;
-        wf.beta[path] = make_float4(beta[0], beta[1], beta[2], 0.0f);
-        wf.rng[path] = rng.s;
-        wf.shadowed[path] = shadowed;
+        wf.L[w] = L;
+        wf.beta[w] = make_float4(beta[0], beta[1], beta[2], 0.0f);
+        wf.rng[w] = rng.s;
+        wf.shadowed[w] = shadowed;
     });
     if (ST) ctr.flush(stats);
 }
 
 template <bool ST, bool DEEP>
-__global__ __launch_bounds__(kBlock) void k_shadow(const FrameGPU F, const SceneGPU S, WF wf, uint32_t q, uint32_t depth,
-                                                   uint32_t fetch_idx, unsigned long long* stats) {
+__global__ __launch_bounds__(kBlock) void k_shadow(const ReqGPU* __restrict__ reqs, const BatchGPU* __restrict__ B,
+                                                   WF wf, uint32_t q, uint32_t depth, uint32_t fetch_idx,
+                                                   unsigned long long* stats) {
     __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
     const Lds stk{&smem[threadIdx.x]}, mslot{&smem[kStackLds * kBlock + threadIdx.x]};
     Ctr<ST> ctr;
     const uint32_t count = wf.counters[q];
+    const uint32_t nreq = B->nreq;
     if (blockIdx.x == 0 && threadIdx.x == 0) wf.counters[1u - q] = 0u;  // next queue starts empty
     for_each_item(&wf.counters[fetch_idx], count, [&](uint32_t slot) {
         const float4 c = wf.contrib[q][slot];
         if (c.w == 0.0f) return;
         const SurfaceRec sf = wf.surf[q][slot];
-        HitG h = secondary<true, ST, DEEP>(S, F, sf, V3{F.sun[0], F.sun[1], F.sun[2]}, ctr, stk, mslot);
+        const ReqGPU& Rq = reqs[find_req(B->work_base, nreq, sf.path)];
+        const SceneGPU S = Rq.S;
+        HitG h = secondary<true, ST, DEEP>(S, Rq.F, sf, V3{Rq.F.sun[0], Rq.F.sun[1], Rq.F.sun[2]}, ctr, stk, mslot);
         if (!h.hit) {
             float4 L = wf.L[sf.path];
             L.x = L.x + c.x;
@@ -885,20 +939,23 @@ __global__ __launch_bounds__(kBlock) void k_shadow(const FrameGPU F, const Scene
 }
 
 template <bool ST, bool DEEP>
-__global__ __launch_bounds__(kBlock) void k_bounce(const FrameGPU F, const SceneGPU S, const float* __restrict__ srgb_dec,
-                                                   WF wf, uint32_t q, uint32_t depth, uint32_t fetch_idx,
-                                                   unsigned long long* stats) {
+__global__ __launch_bounds__(kBlock) void k_bounce(const ReqGPU* __restrict__ reqs, const BatchGPU* __restrict__ B,
+                                                   const float* __restrict__ srgb_dec, WF wf, uint32_t q,
+                                                   uint32_t depth, uint32_t fetch_idx, unsigned long long* stats) {
     __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
     const Lds stk{&smem[threadIdx.x]}, mslot{&smem[kStackLds * kBlock + threadIdx.x]};
     Ctr<ST> ctr;
     const uint32_t count = wf.counters[q];
     const uint32_t nq = 1u - q;
-    const uint32_t max_b = F.max_bounces;
+    const uint32_t nreq = B->nreq;
     for_each_item(&wf.counters[fetch_idx], count, [&](uint32_t slot) {
         const float4 dw = wf.dir[q][slot];
         if (dw.w == 0.0f) return;
         const SurfaceRec sf = wf.surf[q][slot];
         const uint32_t path = sf.path;
+        const ReqGPU& Rq = reqs[find_req(B->work_base, nreq, path)];
+        const SceneGPU S = Rq.S;
+        const FrameGPU& F = Rq.F;
         HitG h = secondary<false, ST, DEEP>(S, F, sf, V3{dw.x, dw.y, dw.z}, ctr, stk, mslot);
         const float4 b4 = wf.beta[path];
         if (!h.hit) {
@@ -918,6 +975,7 @@ __global__ __launch_bounds__(kBlock) void k_bounce(const FrameGPU F, const Scene
         Rng rng{wf.rng[path]};
         float4 contrib, dirw;
         bool no_sun;
+        const uint32_t max_b = F.mode == 2 ? F.max_bounces : 0u;
         shade(F, srgb_dec, ns, depth + 1u, max_b, beta, rng, contrib, dirw, no_sun);
         const uint32_t ns_slot = append_slot(&wf.counters[nq]);
         wf.surf[nq][ns_slot] = ns;
@@ -929,20 +987,42 @@ __global__ __launch_bounds__(kBlock) void k_bounce(const FrameGPU F, const Scene
     if (ST) ctr.flush(stats);
 }
 
-__global__ void k_accumulate(const FrameGPU F, WF wf, HitRecord* hits, float4* rad) {
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= F.width * F.height) return;
-    const uint32_t spp = F.spp ? F.spp : 1u;
+// Per pixel of every request: add this launch's samples to the running sum in sample order;
+// the last chunk writes radiance = sum / spp and the shadow flag of sample 0.
+__device__ void accumulate_pixel(const ReqGPU* __restrict__ reqs, const BatchGPU* __restrict__ B, WF wf, uint32_t t) {
+    const uint32_t nreq = B->nreq;
+    const uint32_t r = find_req(B->pix_base, nreq, t);
+    const ReqGPU& Rq = reqs[r];
+    const uint32_t p = t - B->pix_base[r];
+    const uint32_t W = Rq.F.width;
+    const uint32_t px = p % W, py = p / W;
+    const TileMap tm(Rq);
+    const uint32_t wb = B->work_base[r];
     float acc[3] = {0.0f, 0.0f, 0.0f};
-    for (uint32_t s = 0; s < spp; ++s) {
-        const float4 L = wf.L[pix * spp + s];
+    if (Rq.s0 > 0) {
+        const float4 a = Rq.acc[p];
+        acc[0] = a.x; acc[1] = a.y; acc[2] = a.z;
+    }
+    for (uint32_t s = Rq.s0; s < Rq.s1; ++s) {
+        const float4 L = wf.L[wb + tm.index(px, py, s, Rq.s0)];
         acc[0] += L.x;
         acc[1] += L.y;
         acc[2] += L.z;
     }
-    const float fs = (float)spp;
-    rad[pix] = make_float4(acc[0] / fs, acc[1] / fs, acc[2] / fs, 1.0f);
-    if (wf.shadowed[pix * spp] && (hits[pix].flags & 1u)) hits[pix].flags |= 2u;
+    if (Rq.final_chunk) {
+        const float fs = (float)(Rq.F.spp ? Rq.F.spp : 1u);
+        Rq.rad[p] = make_float4(acc[0] / fs, acc[1] / fs, acc[2] / fs, 1.0f);
+    } else {
+        Rq.acc[p] = make_float4(acc[0], acc[1], acc[2], 0.0f);
+    }
+    if (Rq.s0 == 0 && wf.shadowed[wb + tm.index(px, py, 0u, 0u)] && (Rq.hits[p].flags & 1u)) Rq.hits[p].flags |= 2u;
+}
+
+__global__ void k_accumulate(const ReqGPU* __restrict__ reqs, const BatchGPU* __restrict__ B, WF wf) {
+    const uint32_t nreq = B->nreq;
+    const uint32_t total = B->pix_base[nreq];
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x)
+        accumulate_pixel(reqs, B, wf, t);
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -1039,7 +1119,8 @@ void vt_trace_set_tables(const float* dec_dev, const float* thr_dev) {
 void vt_trace_set_persistent_blocks(uint32_t blocks) { g_persistent_blocks = blocks ? blocks : 2048; }
 
 size_t vt_wave_bytes(uint32_t npaths) {
-    return (size_t)npaths * (16 + 16 + 4 + 1) + 2 * (size_t)npaths * (64 + 16 + 16) + 64 * 4 + 16 * 256;
+    return (size_t)npaths * (16 + 16 + 4 + 1) + 2 * (size_t)npaths * (64 + 16 + 16) + 64 * 4 + 16 * 256 +
+           sizeof(ReqGPU) * VT_MAX_BATCH + sizeof(BatchGPU);
 }
 
 int vt_wave_bind(WaveBuffers& wb, void* mem, uint32_t npaths) {
@@ -1060,6 +1141,7 @@ int vt_wave_bind(WaveBuffers& wb, void* mem, uint32_t npaths) {
         wb.dir[q] = take((size_t)npaths * 16);
     }
     wb.counters = reinterpret_cast<uint32_t*>(take(64 * 4));
+    wb.tables = take(sizeof(ReqGPU) * VT_MAX_BATCH + sizeof(BatchGPU));
     return 0;
 }
 
@@ -1078,12 +1160,65 @@ static WF to_wf(const WaveBuffers& wb) {
     return w;
 }
 
+// Request tables: one device copy per wave-buffer block (stream order protects it), staged
+// through a ring of pinned host buffers (an event per slot guards reuse).
+struct TableRing {
+    static constexpr int kSlots = 8;
+    void* host[kSlots] = {};
+    hipEvent_t ev[kSlots] = {};
+    int next = 0;
+    bool ready = false;
+};
+static TableRing g_ring;
+static constexpr size_t kTableBytes = sizeof(ReqGPU) * VT_MAX_BATCH + sizeof(BatchGPU);
+
+static int ring_init() {
+    if (g_ring.ready) return 0;
+    for (int i = 0; i < TableRing::kSlots; ++i) {
+        if (hipHostMalloc(&g_ring.host[i], kTableBytes, hipHostMallocDefault) != hipSuccess) return -1;
+        if (hipEventCreateWithFlags(&g_ring.ev[i], hipEventDisableTiming) != hipSuccess) return -1;
+    }
+    g_ring.ready = true;
+    return 0;
+}
+
+void vt_trace_release() {
+    if (!g_ring.ready) return;
+    for (int i = 0; i < TableRing::kSlots; ++i) {
+        hipEventSynchronize(g_ring.ev[i]);
+        hipHostFree(g_ring.host[i]);
+        hipEventDestroy(g_ring.ev[i]);
+    }
+    g_ring = TableRing{};
+}
+
+// Work and pixel prefix tables of a batch; returns false if it does not fit `npaths`.
+bool vt_batch_layout(const ReqGPU* reqs, uint32_t n, uint32_t npaths, BatchGPU& B) {
+    B.nreq = n;
+    B.max_bounces = 0;
+    uint64_t w = 0, px = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const ReqGPU& R = reqs[k];
+        const uint32_t tw = R.tile_w, th = 64u / R.tile_w;
+        const uint64_t tiles = (uint64_t)((R.F.width + tw - 1) / tw) * ((R.F.height + th - 1) / th);
+        B.work_base[k] = (uint32_t)w;
+        B.pix_base[k] = (uint32_t)px;
+        w += tiles * 64u * (R.s1 - R.s0);
+        px += (uint64_t)R.F.width * R.F.height;
+        const uint32_t mb = R.F.mode == 2 ? R.F.max_bounces : 0u;
+        if (mb > B.max_bounces) B.max_bounces = mb;
+    }
+    B.work_base[n] = (uint32_t)w;
+    B.pix_base[n] = (uint32_t)px;
+    return w <= npaths && px < 0xFFFFFFFFull;
+}
+
 template <bool ST, bool DEEP>
-static int launch_all(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, const WaveBuffers& wb,
-                      unsigned long long* stats, hipEvent_t* ev, uint8_t* kinds, uint32_t* nstages, hipStream_t s) {
-    const uint32_t spp = F.spp ? F.spp : 1u;
-    const uint32_t tiles = ((F.width + 7u) / 8u) * ((F.height + 7u) / 8u);
+static int launch_batch(const BatchGPU& Bh, const WaveBuffers& wb, unsigned long long* stats, hipEvent_t* ev,
+                        uint8_t* kinds, uint32_t* nstages, hipStream_t s) {
     const dim3 grid(g_persistent_blocks), block(kBlock);
+    const ReqGPU* reqs = reinterpret_cast<const ReqGPU*>(wb.tables);
+    const BatchGPU* B = reinterpret_cast<const BatchGPU*>(static_cast<const char*>(wb.tables) + sizeof(ReqGPU) * VT_MAX_BATCH);
     WF wf = to_wf(wb);
     uint32_t stage = 0;
     auto begin = [&](uint8_t kind) {
@@ -1096,53 +1231,129 @@ static int launch_all(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, flo
     };
     auto st_at = [&](uint32_t k) { return stats ? stats + (size_t)k * VT_STAT_COUNT : nullptr; };
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, wb.counters, 64u);
-    if (F.mode == 0) {
-        begin(VT_STAGE_RENDER0);
-        hipLaunchKernelGGL((k_render0<ST, DEEP>), grid, block, 0, s, F, S, g_srgb_dec, hits, rad, &wb.counters[2],
-                           tiles * 64u, st_at(stage));
-        end();
-        if (nstages) *nstages = stage;
-        return hipGetLastError() == hipSuccess ? 0 : -1;
-    }
-    const uint32_t nwork = tiles * 64u * spp;
-    if ((size_t)F.width * F.height * spp > wb.npaths) return -1;
     begin(VT_STAGE_PRIMARY);
-    hipLaunchKernelGGL((k_primary<ST, DEEP>), grid, block, 0, s, F, S, g_srgb_dec, hits, wf, nwork, st_at(stage));
+    hipLaunchKernelGGL((k_primary<ST, DEEP>), grid, block, 0, s, reqs, B, g_srgb_dec, wf, st_at(stage));
     end();
-    const uint32_t max_b = F.mode == 2 ? F.max_bounces : 0u;
     uint32_t fetch = 3;
-    for (uint32_t d = 0; d <= max_b && fetch + 2 < 64; ++d) {
+    for (uint32_t d = 0; d <= Bh.max_bounces && fetch + 2 < 64; ++d) {
         const uint32_t q = d & 1u;
         begin(VT_STAGE_SHADOW);
-        hipLaunchKernelGGL((k_shadow<ST, DEEP>), grid, block, 0, s, F, S, wf, q, d, fetch++, st_at(stage));
+        hipLaunchKernelGGL((k_shadow<ST, DEEP>), grid, block, 0, s, reqs, B, wf, q, d, fetch++, st_at(stage));
         end();
-        if (d < max_b) {
+        if (d < Bh.max_bounces) {
             begin(VT_STAGE_BOUNCE);
-            hipLaunchKernelGGL((k_bounce<ST, DEEP>), grid, block, 0, s, F, S, g_srgb_dec, wf, q, d, fetch++,
+            hipLaunchKernelGGL((k_bounce<ST, DEEP>), grid, block, 0, s, reqs, B, g_srgb_dec, wf, q, d, fetch++,
                                st_at(stage));
             end();
         }
     }
-    const uint32_t npix = F.width * F.height;
     begin(VT_STAGE_ACCUMULATE);
-    hipLaunchKernelGGL(k_accumulate, dim3((npix + 255) / 256), dim3(256), 0, s, F, wf, hits, rad);
+    hipLaunchKernelGGL(k_accumulate, dim3(g_persistent_blocks), dim3(256), 0, s, reqs, B, wf);
     end();
     if (nstages) *nstages = stage;
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Uploads the request table (stream-ordered) and launches the batched pipeline.
+int vt_trace_batch(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, unsigned long long* stats, hipEvent_t* ev,
+                   uint8_t* kinds, uint32_t* nstages, hipStream_t s) {
+    if (!g_srgb_dec || n == 0 || n > VT_MAX_BATCH || !wb.tables) return -1;
+    if (ring_init() != 0) return -1;
+    const int slot = g_ring.next;
+    g_ring.next = (g_ring.next + 1) % TableRing::kSlots;
+    hipEventSynchronize(g_ring.ev[slot]);
+    char* h = static_cast<char*>(g_ring.host[slot]);
+    ReqGPU* rq = reinterpret_cast<ReqGPU*>(h);
+    BatchGPU* Bh = reinterpret_cast<BatchGPU*>(h + sizeof(ReqGPU) * VT_MAX_BATCH);
+    std::memcpy(rq, reqs, sizeof(ReqGPU) * n);
+    if (!vt_batch_layout(reqs, n, wb.npaths, *Bh)) return -1;
+    if (Bh->max_bounces > 29) return -1;  // stages and fetch counters available per launch
+    bool deep = false, st = false;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (reqs[k].F.mode == 0 || reqs[k].tile_w == 0 || 64u % reqs[k].tile_w != 0 || reqs[k].s1 <= reqs[k].s0)
+            return -1;
+        deep = deep || reqs[k].S.bvh_depth + 1u > (uint32_t)kStackLds;
+        st = st || reqs[k].F.stats != 0;
+    }
+    st = st && stats != nullptr;
+    if (hipMemcpyAsync(wb.tables, h, kTableBytes, hipMemcpyHostToDevice, s) != hipSuccess) return -1;
+    hipEventRecord(g_ring.ev[slot], s);
+    if (deep) return st ? launch_batch<true, true>(*Bh, wb, stats, ev, kinds, nstages, s)
+                        : launch_batch<false, true>(*Bh, wb, stats, ev, kinds, nstages, s);
+    return st ? launch_batch<true, false>(*Bh, wb, stats, ev, kinds, nstages, s)
+              : launch_batch<false, false>(*Bh, wb, stats, ev, kinds, nstages, s);
+}
+
+// Captures the launch sequence of a batch (fixed grids, tables read from device memory)
+// into a graph; the table upload stays outside the graph.
+int vt_trace_capture(uint32_t max_bounces, bool deep, const WaveBuffers& wb, hipStream_t s, hipGraphExec_t* out) {
+    BatchGPU Bh{};
+    Bh.max_bounces = max_bounces;
+    hipGraph_t graph;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return -1;
+    int rc = deep ? launch_batch<false, true>(Bh, wb, nullptr, nullptr, nullptr, nullptr, s)
+                  : launch_batch<false, false>(Bh, wb, nullptr, nullptr, nullptr, nullptr, s);
+    if (hipStreamEndCapture(s, &graph) != hipSuccess || rc != 0) return -1;
+    rc = hipGraphInstantiate(out, graph, nullptr, nullptr, 0) == hipSuccess ? 0 : -1;
+    hipGraphDestroy(graph);
+    return rc;
+}
+
+// Uploads a batch's request table only (for a captured graph launched afterwards).
+int vt_trace_upload_tables(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, hipStream_t s, BatchGPU* layout) {
+    if (n == 0 || n > VT_MAX_BATCH || ring_init() != 0) return -1;
+    const int slot = g_ring.next;
+    g_ring.next = (g_ring.next + 1) % TableRing::kSlots;
+    hipEventSynchronize(g_ring.ev[slot]);
+    char* h = static_cast<char*>(g_ring.host[slot]);
+    std::memcpy(h, reqs, sizeof(ReqGPU) * n);
+    BatchGPU* Bh = reinterpret_cast<BatchGPU*>(h + sizeof(ReqGPU) * VT_MAX_BATCH);
+    if (!vt_batch_layout(reqs, n, wb.npaths, *Bh)) return -1;
+    if (layout) *layout = *Bh;
+    if (hipMemcpyAsync(wb.tables, h, kTableBytes, hipMemcpyHostToDevice, s) != hipSuccess) return -1;
+    hipEventRecord(g_ring.ev[slot], s);
+    return 0;
+}
+
+template <bool ST, bool DEEP>
+static int launch_render0(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, const WaveBuffers& wb,
+                          unsigned long long* stats, hipEvent_t* ev, uint8_t* kinds, uint32_t* nstages,
+                          hipStream_t s) {
+    const uint32_t tiles = ((F.width + 7u) / 8u) * ((F.height + 7u) / 8u);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, wb.counters, 64u);
+    if (ev) hipEventRecord(ev[0], s);
+    if (kinds) kinds[0] = VT_STAGE_RENDER0;
+    hipLaunchKernelGGL((k_render0<ST, DEEP>), dim3(g_persistent_blocks), dim3(kBlock), 0, s, F, S, g_srgb_dec, hits,
+                       rad, &wb.counters[2], tiles * 64u, stats);
+    if (ev) hipEventRecord(ev[1], s);
+    if (nstages) *nstages = 1;
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, const WaveBuffers& wb,
                     unsigned long long* stats, hipEvent_t* ev, uint8_t* kinds, uint32_t* nstages, hipStream_t s) {
     if (!g_srgb_dec || F.mode > 2) return -1;
-    if (F.max_bounces > 29) return -1;  // stages and fetch counters available per frame
-    const bool st = F.stats != 0 && stats != nullptr;
-    // the LDS stack holds kStackLds entries; a DFS needs at most depth + 1
-    const bool deep = S.bvh_depth + 1u > (uint32_t)kStackLds;
-    if (deep)
-        return st ? launch_all<true, true>(F, S, hits, rad, wb, stats, ev, kinds, nstages, s)
-                  : launch_all<false, true>(F, S, hits, rad, wb, stats, ev, kinds, nstages, s);
-    return st ? launch_all<true, false>(F, S, hits, rad, wb, stats, ev, kinds, nstages, s)
-              : launch_all<false, false>(F, S, hits, rad, wb, stats, ev, kinds, nstages, s);
+    if (F.mode == 0) {
+        const bool st = F.stats != 0 && stats != nullptr;
+        const bool deep = S.bvh_depth + 1u > (uint32_t)kStackLds;
+        if (deep) return st ? launch_render0<true, true>(F, S, hits, rad, wb, stats, ev, kinds, nstages, s)
+                            : launch_render0<false, true>(F, S, hits, rad, wb, stats, ev, kinds, nstages, s);
+        return st ? launch_render0<true, false>(F, S, hits, rad, wb, stats, ev, kinds, nstages, s)
+                  : launch_render0<false, false>(F, S, hits, rad, wb, stats, ev, kinds, nstages, s);
+    }
+    // modes 1 and 2: a batch of one request holding all samples of the frame
+    ReqGPU R;
+    std::memset(&R, 0, sizeof(R));
+    R.F = F;
+    R.S = S;
+    R.hits = hits;
+    R.acc = nullptr;
+    R.rad = rad;
+    R.s0 = 0;
+    R.s1 = F.spp ? F.spp : 1u;
+    R.final_chunk = 1;
+    R.tile_w = 8;
+    return vt_trace_batch(&R, 1, wb, stats, ev, kinds, nstages, s);
 }
 
 int vt_encode_srgb8(const float4* rad, uint32_t* bgra, uint32_t n, hipStream_t s) {

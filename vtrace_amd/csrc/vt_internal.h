@@ -56,6 +56,7 @@ struct WaveBuffers {
     void* contrib[2] = {nullptr, nullptr};  // float4 per slot
     void* dir[2] = {nullptr, nullptr};      // float4 per slot
     uint32_t* counters = nullptr;           // 64 u32: queue sizes and fetch counters
+    void* tables = nullptr;                 // ReqGPU[VT_MAX_BATCH] then BatchGPU
 };
 size_t vt_wave_bytes(uint32_t npaths);
 int vt_wave_bind(WaveBuffers& wb, void* mem, uint32_t npaths);
@@ -67,6 +68,13 @@ void vt_trace_set_persistent_blocks(uint32_t blocks);
 int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, const WaveBuffers& wb,
                     unsigned long long* stats, hipEvent_t* ev, uint8_t* kinds, uint32_t* nstages, hipStream_t s);
 int vt_encode_srgb8(const float4* rad, uint32_t* bgra, uint32_t n, hipStream_t s);
+// Batched path tracing (modes 1/2): n requests, their sample chunks traced in one pipeline.
+bool vt_batch_layout(const ReqGPU* reqs, uint32_t n, uint32_t npaths, BatchGPU& B);
+int vt_trace_batch(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, unsigned long long* stats, hipEvent_t* ev,
+                   uint8_t* kinds, uint32_t* nstages, hipStream_t s);
+int vt_trace_capture(uint32_t max_bounces, bool deep, const WaveBuffers& wb, hipStream_t s, hipGraphExec_t* out);
+int vt_trace_upload_tables(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, hipStream_t s, BatchGPU* layout);
+void vt_trace_release();
 
 // host math shared by the runtime (frame setup); FP64 inverse rounded to FP32.
 bool vt_inverse4(const float* m, float* out);

@@ -83,6 +83,28 @@ struct SceneGPU {
     uint32_t bvh_depth;         // selects the traversal-stack variant
 };
 
+// One request (frame, or a chunk of its samples) in a batched launch of the path tracer.
+struct ReqGPU {
+    FrameGPU F;
+    SceneGPU S;
+    HitRecord* hits;        // per pixel, written by the chunk that holds sample 0
+    float4* acc;            // per pixel running radiance sum over samples (FP32, in sample order)
+    float4* rad;            // per pixel final radiance, written by the last chunk (acc / spp)
+    uint32_t s0, s1;        // samples [s0, s1) are traced in this launch
+    uint32_t final_chunk;   // 1: s1 == spp
+    uint32_t tile_w;        // pixel tile of 64 work items: tile_w x (64 / tile_w)
+};
+
+#define VT_MAX_BATCH 64
+// Work and pixel spaces of a batch: request k owns work items [work_base[k], work_base[k+1])
+// (multiples of 64) and accumulate threads [pix_base[k], pix_base[k+1]).
+struct BatchGPU {
+    uint32_t nreq;
+    uint32_t max_bounces;   // bounce stages launched (max over the requests)
+    uint32_t work_base[VT_MAX_BATCH + 1];
+    uint32_t pix_base[VT_MAX_BATCH + 1];
+};
+
 // Launch stages of one frame (stats and timings are kept per stage):
 // mode 0: [render0]; modes 1/2: [primary, shadow d0, bounce d0, ..., shadow dB, accumulate].
 #define VT_MAX_STAGES 64
