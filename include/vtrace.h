@@ -124,6 +124,55 @@ int32_t vt_instance_count(void);
 int32_t vt_synchronize(void);
 const char* vt_last_error(void);
 
+/* ---------------- serving loop (continuous batching over resident scenes) ---------------- */
+
+typedef struct vt_serve_config {
+    uint32_t max_slots;         /* requests admitted at once (<= 64) */
+    uint32_t spp_chunk;         /* samples of one request traced per iteration */
+    uint64_t memory_budget;     /* bytes for admitted requests' buffers (48 B per pixel) */
+    uint32_t max_work;          /* pixel samples traced per iteration (all requests) */
+    uint32_t tile_w;            /* pixel tile of 64 samples: tile_w x (64 / tile_w); 0 = 8 */
+    uint32_t max_bounces;       /* bounce stages per iteration (largest request max_bounces) */
+    uint32_t use_graph;         /* 1: replay the iteration's launch sequence from a hipGraph */
+} vt_serve_config;
+
+typedef struct vt_request {
+    uint32_t scene;             /* id from vt_serve_add_scene */
+    uint32_t width, height, spp, max_bounces, seed, frame;
+    uint32_t viewport_square;
+    float perspective[16];      /* column-major, as render_tick_info */
+    float camera[16];
+    float sun_dir[3];
+    float sun_radiance[3];
+    float sky[3];
+} vt_request;
+
+typedef struct vt_serve_status {
+    uint32_t queued, active, done_unread, tile_w;
+    uint64_t memory_in_use;
+    uint64_t iterations;
+    uint64_t samples;           /* pixel samples traced so far */
+    float last_iteration_ms;
+    uint32_t pad;
+} vt_serve_stats;
+
+/* Sets up the serving loop (replaces any previous one). */
+int32_t vt_serve_init(const vt_serve_config* config);
+/* A scene = n instance records (render.rs layout) over volumes already uploaded with
+ * add_texture / vt_add_volume_indexed; its LBVH stays resident.  Returns the scene id. */
+int32_t vt_serve_add_scene(const float* records, uint32_t n);
+/* Queues a request; returns its id (> 0) or -1. */
+int64_t vt_serve_submit(const vt_request* request);
+/* One iteration: admit queued requests (slots, memory budget), trace the next sample chunk
+ * of every admitted request in one batched launch, retire finished requests.  Writes up to
+ * max_completed finished ids; returns the number finished or -1. */
+int32_t vt_serve_step(int64_t* completed, uint32_t max_completed);
+/* Copies a finished request's RGBA32F radiance and/or hit records (16 B per pixel each)
+ * and releases its buffers. */
+int32_t vt_serve_read(int64_t id, float* rgba, uint64_t bytes, void* hits, uint64_t hit_bytes);
+int32_t vt_serve_status(vt_serve_stats* status);
+void vt_serve_shutdown(void);
+
 #ifdef __cplusplus
 }
 #endif

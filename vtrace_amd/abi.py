@@ -13,7 +13,9 @@ REFERENCE_EXPORTS = ("entry", "render_tick", "get_input_data_pointer", "add_text
 EXTENSION_EXPORTS = ("vt_set_params", "vt_get_params", "vt_add_volume_indexed", "vt_read_hits", "vt_read_radiance",
                      "vt_read_bgra8", "vt_read_stats", "vt_read_stage_stats", "vt_last_frame_ms", "vt_bench_frames",
                      "vt_bench_stages", "vt_read_volume",
-                     "vt_volume_info", "vt_read_lbvh", "vt_instance_count", "vt_synchronize", "vt_last_error")
+                     "vt_volume_info", "vt_read_lbvh", "vt_instance_count", "vt_synchronize", "vt_last_error",
+                     "vt_serve_init", "vt_serve_add_scene", "vt_serve_submit", "vt_serve_step", "vt_serve_read",
+                     "vt_serve_status", "vt_serve_shutdown")
 
 
 class RenderTickInfo(C.Structure):
@@ -84,4 +86,6 @@ def declare(lib: C.CDLL) -> None:
     lib.vt_synchronize.argtypes = []
     lib.vt_last_error.restype = C.c_char_p
     lib.vt_last_error.argtypes = []
+    from . import serving
+    serving.declare(lib)
     lib._vt_declared = True

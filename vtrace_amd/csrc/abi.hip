@@ -262,7 +262,9 @@ int upload_volume(const void* data, size_t bytes, uint32_t fmt, const uint32_t d
     return id;
 }
 
-bool build_frame(const float* P, const float* C, FrameGPU& F) {
+}  // namespace
+
+bool vt_build_frame(const float* P, const float* C, const vt_render_params& p, FrameGPU& F) {
     float invP[16], invC[16], invCc[16], Cc[16];
     std::memcpy(Cc, C, 64);
     Cc[12] = Cc[13] = Cc[14] = 0.0f;  // centered_camera (shaders/trace.frag:55-58)
@@ -276,7 +278,6 @@ bool build_frame(const float* P, const float* C, FrameGPU& F) {
     F.O[0] = o[0]; F.O[1] = o[1]; F.O[2] = o[2];
     F.ip00 = invP[0];
     F.ip11 = invP[5];
-    const vt_render_params& p = g.params;
     F.W = (float)p.width;
     F.Hv = p.viewport_square ? (float)p.width : (float)p.height;
     F.width = p.width;
@@ -295,6 +296,10 @@ bool build_frame(const float* P, const float* C, FrameGPU& F) {
     F.stats = p.stats;
     return true;
 }
+
+namespace {
+
+bool build_frame(const float* P, const float* C, FrameGPU& F) { return vt_build_frame(P, C, g.params, F); }
 
 SceneGPU scene_gpu() {
     SceneGPU S;
@@ -322,6 +327,19 @@ int render_frame(const FrameGPU& F) {
     g.have_frame = true;
     return 0;
 }
+
+}  // namespace
+
+// ---- runtime accessors used by the serving loop (serve.hip) ----------------------------
+std::mutex& vt_rt_lock() { return g_lock; }
+bool vt_rt_inited() { return g.inited; }
+int vt_rt_device() { return g.device; }
+hipStream_t vt_rt_stream() { return g.stream; }
+const VolumeDesc* vt_rt_vol_table() { return g.vol_table; }
+uint32_t vt_rt_nvol() { return (uint32_t)g.vols.size(); }
+void vt_rt_set_error(const char* what) { set_err(what); }
+
+namespace {
 
 struct Guard {
     std::lock_guard<std::mutex> lk;
@@ -442,6 +460,7 @@ void cleanup(void) {
     if (!g.inited) return;
     hipSetDevice(g.device);
     hipStreamSynchronize(g.stream);
+    vt_serve_release();
     for (auto& v : g.vols) {
         hipFree(v.brick_map); hipFree(v.masks); hipFree(v.payload); hipFree(v.palette);
     }

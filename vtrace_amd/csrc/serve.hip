@@ -1,0 +1,372 @@
+// serve.hip — continuous-batching serving loop over resident scenes (no reference
+// counterpart: the reference renders one interactive frame per render_tick, lib/entry.c:150).
+//
+// Scenes (instance lists over the library's resident volumes, each with its own LBVH) stay
+// in HBM.  Requests (camera, resolution, spp, seed) queue FIFO; vt_serve_step() runs one
+// iteration:
+//   1. admission: the queue head is admitted while a slot is free and its buffers (hit
+//      records, radiance running sum, output: 48 B per pixel) fit the memory budget
+//      (first-fit arena);
+//   2. batching: every active request contributes its next chunk of samples (all pixels x
+//      up to spp_chunk samples) while the iteration's work budget allows;
+//   3. one launch of the batched wavefront pipeline traces all chunks together (optionally
+//      replayed from a captured hipGraph), so requests that arrived while others were in
+//      flight share the device with them;
+//   4. requests whose last chunk ran are complete; their output waits for vt_serve_read.
+// A request's pixels depend only on its own scene, camera, seed and sample indices and
+// its samples are summed in sample order, so its result is bit-identical whether it is
+// rendered alone, in any mix, or by the plain single-frame path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <vector>
+
+#include "vt_internal.h"
+
+namespace {
+
+struct Arena {
+    size_t size = 0;
+    std::map<size_t, size_t> free_;  // offset -> length
+    void reset(size_t n) {
+        size = n;
+        free_.clear();
+        if (n) free_[0] = n;
+    }
+    bool alloc(size_t n, size_t* off) {
+        n = (n + 255) & ~(size_t)255;
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= n) {
+                *off = it->first;
+                const size_t rest = it->second - n, at = it->first + n;
+                free_.erase(it);
+                if (rest) free_[at] = rest;
+                return true;
+            }
+        return false;
+    }
+    void release(size_t off, size_t n) {
+        n = (n + 255) & ~(size_t)255;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t in_use() const {
+        size_t f = 0;
+        for (auto& kv : free_) f += kv.second;
+        return size - f;
+    }
+};
+
+struct ServeScene {
+    LbvhWork bvh;
+    float* records = nullptr;
+    uint32_t n = 0;
+    uint32_t depth = 0;
+};
+
+enum { kQueued = 0, kActive = 1, kDone = 2 };
+
+struct ServeReq {
+    vt_request q{};
+    FrameGPU F{};
+    size_t off = 0, bytes = 0;
+    uint32_t next_s = 0;
+    int state = kQueued;
+};
+
+struct Serve {
+    bool on = false;
+    vt_serve_config cfg{};
+    char* arena_mem = nullptr;
+    Arena arena;
+    void* wave_mem = nullptr;
+    WaveBuffers wave;
+    std::vector<ServeScene> scenes;
+    std::deque<int64_t> queue;
+    std::vector<int64_t> active;
+    std::map<int64_t, ServeReq> reqs;
+    int64_t next_id = 1;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipEvent_t done = nullptr;
+    uint64_t iterations = 0, samples = 0;
+    float last_ms = 0.0f;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+};
+
+Serve sv;
+
+size_t req_bytes(const vt_request& q) { return (size_t)q.width * q.height * 48; }
+
+uint32_t tiles_of(uint32_t w, uint32_t h, uint32_t tw) {
+    const uint32_t th = 64u / tw;
+    return ((w + tw - 1) / tw) * ((h + th - 1) / th);
+}
+
+void free_all() {
+    for (auto& s : sv.scenes) {
+        vt_lbvh_free(s.bvh);
+        hipFree(s.records);
+    }
+    sv.scenes.clear();
+    for (auto& gx : sv.graph)
+        if (gx) hipGraphExecDestroy(gx);
+    hipFree(sv.arena_mem);
+    hipFree(sv.wave_mem);
+    if (sv.done) hipEventDestroy(sv.done);
+    if (sv.t0) hipEventDestroy(sv.t0);
+    if (sv.t1) hipEventDestroy(sv.t1);
+    sv = Serve{};
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t vt_serve_init(const vt_serve_config* cfg) {
+    std::lock_guard<std::mutex> lk(vt_rt_lock());
+    if (!vt_rt_inited() || !cfg) return -1;
+    hipSetDevice(vt_rt_device());
+    if (sv.on) free_all();
+    vt_serve_config c = *cfg;
+    if (c.max_slots == 0 || c.max_slots > VT_MAX_BATCH || c.memory_budget < 48 * 64 || c.max_work < 64 ||
+        c.spp_chunk == 0 || c.max_bounces > 29 || (c.tile_w && 64u % c.tile_w != 0)) {
+        vt_rt_set_error("vt_serve_init: invalid configuration");
+        return -1;
+    }
+    if (c.tile_w == 0) c.tile_w = 8;
+    // path state: the work budget, or one sample of the largest frame the budget admits
+    const size_t max_px = c.memory_budget / 48;
+    const size_t one_spp = max_px + 64 * 2 * 16384 + 64;
+    const size_t paths = std::max<size_t>(c.max_work, one_spp);
+    if (paths > 0x7FFFFFFFull) {
+        vt_rt_set_error("vt_serve_init: work budget too large");
+        return -1;
+    }
+    if (hipMalloc(&sv.arena_mem, c.memory_budget) != hipSuccess ||
+        hipMalloc(&sv.wave_mem, vt_wave_bytes((uint32_t)paths)) != hipSuccess ||
+        hipEventCreateWithFlags(&sv.done, hipEventDisableTiming) != hipSuccess || hipEventCreate(&sv.t0) != hipSuccess ||
+        hipEventCreate(&sv.t1) != hipSuccess) {
+        vt_rt_set_error("vt_serve_init: allocation failed");
+        free_all();
+        return -1;
+    }
+    vt_wave_bind(sv.wave, sv.wave_mem, (uint32_t)paths);
+    sv.arena.reset(c.memory_budget);
+    sv.cfg = c;
+    sv.on = true;
+    return 0;
+}
+
+int32_t vt_serve_add_scene(const float* records, uint32_t n) {
+    std::lock_guard<std::mutex> lk(vt_rt_lock());
+    if (!sv.on || !records || n == 0) return -1;
+    hipSetDevice(vt_rt_device());
+    hipStream_t s = vt_rt_stream();
+    ServeScene sc;
+    sc.n = n;
+    if (hipMalloc(&sc.records, (size_t)n * 64) != hipSuccess) return -1;
+    hipMemcpyAsync(sc.records, records, (size_t)n * 64, hipMemcpyHostToDevice, s);
+    if (vt_lbvh_build(sc.bvh, sc.records, n, vt_rt_vol_table(), vt_rt_nvol(), s) != 0) {
+        hipFree(sc.records);
+        vt_lbvh_free(sc.bvh);
+        return -1;
+    }
+    hipMemcpyAsync(&sc.depth, sc.bvh.depth, 4, hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    sv.scenes.push_back(sc);
+    return (int32_t)sv.scenes.size() - 1;
+}
+
+int64_t vt_serve_submit(const vt_request* q) {
+    std::lock_guard<std::mutex> lk(vt_rt_lock());
+    if (!sv.on || !q || q->scene >= sv.scenes.size() || q->width == 0 || q->height == 0 || q->width > 16384 ||
+        q->height > 16384 || q->spp == 0 || q->max_bounces > sv.cfg.max_bounces || req_bytes(*q) > sv.cfg.memory_budget) {
+        vt_rt_set_error("vt_serve_submit: invalid request");
+        return -1;
+    }
+    ServeReq r;
+    r.q = *q;
+    vt_render_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.width = q->width;
+    p.height = q->height;
+    p.viewport_square = q->viewport_square;
+    p.mode = 2;
+    p.spp = q->spp;
+    p.max_bounces = q->max_bounces;
+    p.jitter = 1;
+    p.seed = q->seed;
+    p.frame = q->frame;
+    for (int k = 0; k < 3; ++k) {
+        p.sun_dir[k] = q->sun_dir[k];
+        p.sun_radiance[k] = q->sun_radiance[k];
+        p.sky[k] = q->sky[k];
+    }
+    if (!vt_build_frame(q->perspective, q->camera, p, r.F)) {
+        vt_rt_set_error("vt_serve_submit: singular projection or camera");
+        return -1;
+    }
+    const int64_t id = sv.next_id++;
+    sv.reqs[id] = r;
+    sv.queue.push_back(id);
+    return id;
+}
+
+int32_t vt_serve_step(int64_t* completed, uint32_t max_completed) {
+    std::lock_guard<std::mutex> lk(vt_rt_lock());
+    if (!sv.on) return -1;
+    hipSetDevice(vt_rt_device());
+    hipStream_t s = vt_rt_stream();
+    const vt_serve_config& c = sv.cfg;
+    // 1. admission (FIFO: the head waits for memory, later requests wait behind it)
+    while (!sv.queue.empty() && sv.active.size() < c.max_slots) {
+        ServeReq& r = sv.reqs[sv.queue.front()];
+        size_t off;
+        if (!sv.arena.alloc(req_bytes(r.q), &off)) break;
+        r.off = off;
+        r.bytes = req_bytes(r.q);
+        r.state = kActive;
+        sv.active.push_back(sv.queue.front());
+        sv.queue.pop_front();
+    }
+    // 2. batching: next sample chunk of every active request, within the work budget
+    std::vector<ReqGPU> batch;
+    std::vector<int64_t> ids;
+    uint64_t work = 0;
+    for (int64_t id : sv.active) {
+        ServeReq& r = sv.reqs[id];
+        if (batch.size() >= VT_MAX_BATCH) break;
+        const uint64_t per_s = (uint64_t)tiles_of(r.q.width, r.q.height, c.tile_w) * 64u;
+        uint32_t ns = std::min(c.spp_chunk, r.q.spp - r.next_s);
+        if (work + per_s * ns > c.max_work) {
+            const uint64_t fit = work < c.max_work ? (c.max_work - work) / per_s : 0;
+            if (fit == 0 && !batch.empty()) continue;
+            ns = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ns, fit));
+        }
+        if (work + per_s * ns > sv.wave.npaths) continue;
+        ReqGPU R;
+        std::memset(&R, 0, sizeof(R));
+        R.F = r.F;
+        const ServeScene& sc = sv.scenes[r.q.scene];
+        R.S.inst = sc.bvh.inst;
+        R.S.nodes = sc.bvh.nodes;
+        R.S.vols = vt_rt_vol_table();
+        R.S.ninst = sc.n;
+        R.S.nvol = vt_rt_nvol();
+        R.S.bvh_depth = sc.depth;
+        const size_t npx = (size_t)r.q.width * r.q.height;
+        char* base = sv.arena_mem + r.off;
+        R.hits = reinterpret_cast<HitRecord*>(base);
+        R.acc = reinterpret_cast<float4*>(base + npx * 16);
+        R.rad = reinterpret_cast<float4*>(base + npx * 32);
+        R.s0 = r.next_s;
+        R.s1 = r.next_s + ns;
+        R.final_chunk = R.s1 == r.q.spp ? 1u : 0u;
+        R.tile_w = c.tile_w;
+        batch.push_back(R);
+        ids.push_back(id);
+        work += per_s * ns;
+    }
+    if (batch.empty()) return 0;
+    // 3. one launch for all chunks
+    bool deep = false;
+    for (auto& R : batch) deep = deep || R.S.bvh_depth + 1u > 16u;
+    hipEventRecord(sv.t0, s);
+    int rc;
+    if (c.use_graph) {
+        hipGraphExec_t& gx = sv.graph[deep ? 1 : 0];
+        rc = 0;
+        if (!gx) rc = vt_trace_capture(c.max_bounces, deep, sv.wave, s, &gx);
+        if (rc == 0) rc = vt_trace_upload_tables(batch.data(), (uint32_t)batch.size(), sv.wave, s, nullptr);
+        if (rc == 0) rc = hipGraphLaunch(gx, s) == hipSuccess ? 0 : -1;
+    } else {
+        rc = vt_trace_batch(batch.data(), (uint32_t)batch.size(), sv.wave, nullptr, nullptr, nullptr, nullptr, s);
+    }
+    hipEventRecord(sv.t1, s);
+    if (rc != 0) {
+        vt_rt_set_error("vt_serve_step: launch failed");
+        return -1;
+    }
+    if (hipEventSynchronize(sv.t1) != hipSuccess) return -1;
+    hipEventElapsedTime(&sv.last_ms, sv.t0, sv.t1);
+    ++sv.iterations;
+    sv.samples += work;
+    // 4. retire finished requests
+    uint32_t n_done = 0;
+    for (size_t k = 0; k < batch.size(); ++k) {
+        ServeReq& r = sv.reqs[ids[k]];
+        r.next_s = batch[k].s1;
+        if (r.next_s == r.q.spp) {
+            r.state = kDone;
+            sv.active.erase(std::find(sv.active.begin(), sv.active.end(), ids[k]));
+            if (completed && n_done < max_completed) completed[n_done] = ids[k];
+            ++n_done;
+        }
+    }
+    return (int32_t)n_done;
+}
+
+int32_t vt_serve_read(int64_t id, float* rgba, uint64_t bytes, void* hits, uint64_t hit_bytes) {
+    std::lock_guard<std::mutex> lk(vt_rt_lock());
+    if (!sv.on) return -1;
+    auto it = sv.reqs.find(id);
+    if (it == sv.reqs.end() || it->second.state != kDone) return -1;
+    ServeReq& r = it->second;
+    const size_t npx = (size_t)r.q.width * r.q.height;
+    if ((rgba && bytes < npx * 16) || (hits && hit_bytes < npx * 16)) return -1;
+    hipSetDevice(vt_rt_device());
+    hipStream_t s = vt_rt_stream();
+    const char* base = sv.arena_mem + r.off;
+    if (rgba) hipMemcpyAsync(rgba, base + npx * 32, npx * 16, hipMemcpyDeviceToHost, s);
+    if (hits) hipMemcpyAsync(hits, base, npx * 16, hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    sv.arena.release(r.off, r.bytes);
+    sv.reqs.erase(it);
+    return 0;
+}
+
+int32_t vt_serve_status(vt_serve_stats* st) {
+    std::lock_guard<std::mutex> lk(vt_rt_lock());
+    if (!sv.on || !st) return -1;
+    std::memset(st, 0, sizeof(*st));
+    st->queued = (uint32_t)sv.queue.size();
+    st->active = (uint32_t)sv.active.size();
+    uint32_t done = 0;
+    for (auto& kv : sv.reqs) done += kv.second.state == kDone ? 1u : 0u;
+    st->done_unread = done;
+    st->memory_in_use = sv.arena.in_use();
+    st->iterations = sv.iterations;
+    st->samples = sv.samples;
+    st->last_iteration_ms = sv.last_ms;
+    st->tile_w = sv.cfg.tile_w;
+    return 0;
+}
+
+void vt_serve_shutdown(void) {
+    std::lock_guard<std::mutex> lk(vt_rt_lock());
+    vt_serve_release();
+}
+
+}  // extern "C"
+
+// Frees the serving state; the caller holds the runtime lock (cleanup() calls it too).
+void vt_serve_release() {
+    if (!sv.on) return;
+    hipSetDevice(vt_rt_device());
+    hipStreamSynchronize(vt_rt_stream());
+    free_all();
+}

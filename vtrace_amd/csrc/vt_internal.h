@@ -5,6 +5,9 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <mutex>
+
+#include "../../include/vtrace.h"
 #include "vt_types.h"
 
 // Device storage of one bricked volume.
@@ -78,3 +81,16 @@ void vt_trace_release();
 
 // host math shared by the runtime (frame setup); FP64 inverse rounded to FP32.
 bool vt_inverse4(const float* m, float* out);
+
+// abi.hip: runtime state shared with the serving loop
+std::mutex& vt_rt_lock();
+bool vt_rt_inited();
+int vt_rt_device();
+hipStream_t vt_rt_stream();
+const VolumeDesc* vt_rt_vol_table();
+uint32_t vt_rt_nvol();
+void vt_rt_set_error(const char* what);
+// frame constants from the projection / camera matrices and render parameters
+bool vt_build_frame(const float* P, const float* C, const vt_render_params& p, FrameGPU& F);
+// serve.hip: frees the serving loop (caller holds the runtime lock)
+void vt_serve_release();

@@ -1,0 +1,293 @@
+"""Serving loop driver: resident scenes, a Poisson request stream, continuous batching.
+
+The scheduler itself is native (vt_serve_* in librender_hip, serve.hip): FIFO admission by
+slots and HBM budget, one batched launch per iteration holding the next sample chunk of
+every admitted request, optional hipGraph replay.  This module feeds it requests at their
+arrival times, measures per-request latency (arrival -> result available) on the wall
+clock, and compares against the plain path serving the same stream one request at a time.
+
+The request stream (seeded): resolution uniform over {256x256, 640x480, 1280x720,
+1920x1080}, spp uniform in 1..16, scene uniform over the resident scenes, camera pose
+seeded per request, Poisson arrivals at a rate set for a target utilisation of the
+measured batched capacity.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+import os
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import scenes
+from .renderer import HIT_DTYPE, RendererError
+
+RESOLUTIONS = ((256, 256), (640, 480), (1280, 720), (1920, 1080))
+
+
+class VtServeConfig(C.Structure):
+    _fields_ = [("max_slots", C.c_uint32), ("spp_chunk", C.c_uint32), ("memory_budget", C.c_uint64),
+                ("max_work", C.c_uint32), ("tile_w", C.c_uint32), ("max_bounces", C.c_uint32),
+                ("use_graph", C.c_uint32)]
+
+
+class VtRequest(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("max_bounces", C.c_uint32), ("seed", C.c_uint32), ("frame", C.c_uint32),
+                ("viewport_square", C.c_uint32), ("perspective", C.c_float * 16), ("camera", C.c_float * 16),
+                ("sun_dir", C.c_float * 3), ("sun_radiance", C.c_float * 3), ("sky", C.c_float * 3)]
+
+
+class VtServeStats(C.Structure):
+    _fields_ = [("queued", C.c_uint32), ("active", C.c_uint32), ("done_unread", C.c_uint32), ("tile_w", C.c_uint32),
+                ("memory_in_use", C.c_uint64), ("iterations", C.c_uint64), ("samples", C.c_uint64),
+                ("last_iteration_ms", C.c_float), ("pad", C.c_uint32)]
+
+
+def declare(lib):
+    if getattr(lib, "_vt_serve_declared", False):
+        return
+    lib.vt_serve_init.restype = C.c_int32
+    lib.vt_serve_init.argtypes = [C.POINTER(VtServeConfig)]
+    lib.vt_serve_add_scene.restype = C.c_int32
+    lib.vt_serve_add_scene.argtypes = [C.c_void_p, C.c_uint32]
+    lib.vt_serve_submit.restype = C.c_int64
+    lib.vt_serve_submit.argtypes = [C.POINTER(VtRequest)]
+    lib.vt_serve_step.restype = C.c_int32
+    lib.vt_serve_step.argtypes = [C.c_void_p, C.c_uint32]
+    lib.vt_serve_read.restype = C.c_int32
+    lib.vt_serve_read.argtypes = [C.c_int64, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64]
+    lib.vt_serve_status.restype = C.c_int32
+    lib.vt_serve_status.argtypes = [C.POINTER(VtServeStats)]
+    lib.vt_serve_shutdown.restype = None
+    lib.vt_serve_shutdown.argtypes = []
+    lib._vt_serve_declared = True
+
+
+@dataclass
+class Request:
+    scene: int
+    width: int
+    height: int
+    spp: int
+    P: np.ndarray
+    C: np.ndarray
+    seed: int
+    max_bounces: int = 4
+    sun_dir: tuple = (0.3, -0.8, 0.5)
+    sun_radiance: tuple = (2.5, 2.5, 2.5)
+    sky: tuple = scenes.SKY
+    viewport_square: int = 0
+    arrival: float = 0.0
+
+    def params(self) -> "scenes.RenderParams":
+        return scenes.RenderParams(self.width, self.height, viewport_square=self.viewport_square, mode=2,
+                                   spp=self.spp, max_bounces=self.max_bounces, jitter=1, seed=self.seed,
+                                   sun_dir=self.sun_dir, sun_radiance=self.sun_radiance, sky=self.sky)
+
+    def to_c(self) -> VtRequest:
+        q = VtRequest()
+        q.scene, q.width, q.height, q.spp = self.scene, self.width, self.height, self.spp
+        q.max_bounces, q.seed, q.frame, q.viewport_square = self.max_bounces, self.seed & 0xFFFFFFFF, 0, \
+            self.viewport_square
+        for k in range(16):
+            q.perspective[k] = float(self.P[k])
+            q.camera[k] = float(self.C[k])
+        for k in range(3):
+            q.sun_dir[k], q.sun_radiance[k], q.sky[k] = self.sun_dir[k], self.sun_radiance[k], self.sky[k]
+        return q
+
+
+class ServingLoop:
+    def __init__(self, renderer, max_slots=32, spp_chunk=4, memory_budget=8 << 30, max_work=8 << 20, tile_w=8,
+                 max_bounces=4, use_graph=True):
+        self.r = renderer
+        self.lib = renderer.lib
+        declare(self.lib)
+        cfg = VtServeConfig(max_slots, spp_chunk, memory_budget, max_work, tile_w, max_bounces, 1 if use_graph else 0)
+        if self.lib.vt_serve_init(C.byref(cfg)) != 0:
+            raise RendererError(f"vt_serve_init failed: {renderer.error()}")
+        self.cfg = cfg
+        self._done = (C.c_int64 * 64)()
+
+    def add_scene(self, records: np.ndarray) -> int:
+        recs = np.ascontiguousarray(records, dtype=np.float32).reshape(-1, 16)
+        sid = self.lib.vt_serve_add_scene(recs.ctypes.data, recs.shape[0])
+        if sid < 0:
+            raise RendererError(f"vt_serve_add_scene failed: {self.r.error()}")
+        return sid
+
+    def submit(self, req: Request) -> int:
+        q = req.to_c()
+        rid = self.lib.vt_serve_submit(C.byref(q))
+        if rid < 0:
+            raise RendererError(f"vt_serve_submit failed: {self.r.error()}")
+        return int(rid)
+
+    def step(self) -> list:
+        n = self.lib.vt_serve_step(self._done, 64)
+        if n < 0:
+            raise RendererError(f"vt_serve_step failed: {self.r.error()}")
+        return [int(self._done[i]) for i in range(min(n, 64))]
+
+    def read(self, rid: int, width: int, height: int, hits: bool = False):
+        rad = np.zeros((height, width, 4), dtype=np.float32)
+        hb = np.zeros((height, width), dtype=HIT_DTYPE) if hits else None
+        rc = self.lib.vt_serve_read(rid, rad.ctypes.data, rad.nbytes, hb.ctypes.data if hits else None,
+                                    hb.nbytes if hits else 0)
+        if rc != 0:
+            raise RendererError(f"vt_serve_read failed for {rid}")
+        return (rad, hb) if hits else rad
+
+    def status(self) -> dict:
+        st = VtServeStats()
+        self.lib.vt_serve_status(C.byref(st))
+        return {n: getattr(st, n) for n, _ in VtServeStats._fields_ if n != "pad"}
+
+    def run_all(self, reqs: list) -> dict:
+        """Submits every request at once and steps until all are done; returns id -> radiance."""
+        ids = {self.submit(q): q for q in reqs}
+        out = {}
+        pending = set(ids)
+        while pending:
+            for rid in self.step():
+                q = ids[rid]
+                out[rid] = self.read(rid, q.width, q.height)
+                pending.discard(rid)
+        return {list(ids).index(k): v for k, v in out.items()}
+
+    def run_stream(self, reqs: list, timeout_s: float = 600.0) -> dict:
+        """Feeds requests at their arrival times (seconds from start); latency = result ready - arrival."""
+        order = sorted(range(len(reqs)), key=lambda i: reqs[i].arrival)
+        t0 = time.perf_counter()
+        nxt = 0
+        idmap = {}
+        lat = np.zeros(len(reqs))
+        done = 0
+        samples = 0
+        while done < len(reqs):
+            now = time.perf_counter() - t0
+            if now > timeout_s:
+                raise RuntimeError("serving stream timed out")
+            while nxt < len(order) and reqs[order[nxt]].arrival <= now:
+                i = order[nxt]
+                idmap[self.submit(reqs[i])] = i
+                nxt += 1
+            st = self.status()
+            if st["queued"] == 0 and st["active"] == 0:
+                if nxt < len(order):  # idle until the next arrival
+                    time.sleep(max(0.0, reqs[order[nxt]].arrival - (time.perf_counter() - t0)))
+                continue
+            for rid in self.step():
+                i = idmap[rid]
+                q = reqs[i]
+                self.read(rid, q.width, q.height)
+                lat[i] = (time.perf_counter() - t0) - q.arrival
+                samples += q.width * q.height * q.spp
+                done += 1
+        wall = time.perf_counter() - t0
+        return {"latency_s": lat, "wall_s": wall, "samples": samples}
+
+    def close(self):
+        self.lib.vt_serve_shutdown()
+
+
+def plain_one_at_a_time(renderer, scene_list, reqs: list, timeout_s: float = 600.0) -> dict:
+    """The same stream served FIFO by the plain single-frame path, one request at a time."""
+    order = sorted(range(len(reqs)), key=lambda i: reqs[i].arrival)
+    t0 = time.perf_counter()
+    lat = np.zeros(len(reqs))
+    cur_scene = None
+    for i in order:
+        q = reqs[i]
+        now = time.perf_counter() - t0
+        if now > timeout_s:
+            raise RuntimeError("plain stream timed out")
+        if q.arrival > now:
+            time.sleep(q.arrival - now)
+        if cur_scene != q.scene:
+            renderer.set_instances(scene_list[q.scene])
+            cur_scene = q.scene
+        renderer.set_params(q.params())
+        renderer.render(q.P, q.C)
+        renderer.radiance()
+        lat[i] = (time.perf_counter() - t0) - q.arrival
+    return {"latency_s": lat, "wall_s": time.perf_counter() - t0}
+
+
+# ---- workload -------------------------------------------------------------------------------
+def resident_volumes(count=32, sizes=(64, 128, 256, 512), seed=4, fill=(0.05, 0.40)):
+    """count seeded noise volumes with sizes drawn from `sizes` and fill U[fill]."""
+    rng = scenes.Rng(seed)
+    vols = []
+    for i in range(count):
+        n = sizes[rng.randint(0, len(sizes) - 1)]
+        f = rng.uniform(*fill)
+        vols.append(scenes.Volume(scenes.gen_volume((n, n, n), seed * 100 + i, f), scenes.gen_palette(seed * 100 + i)))
+    return vols
+
+
+def scene_records(tex_ids, dims) -> list:
+    """One instance per scene: volume k at the origin, one world unit per voxel."""
+    return [scenes.instance_record(scenes.model_matrix((0.0, 0.0, 0.0), float(n)), t)[None, :]
+            for t, n in zip(tex_ids, dims)]
+
+
+def request_stream(nreq: int, nscenes: int, dims: list, rate_hz: float, seed: int = 5, max_bounces: int = 4,
+                   resolutions=RESOLUTIONS, spp_range=(1, 16)) -> list:
+    rng = scenes.Rng(seed)
+    t = 0.0
+    out = []
+    for i in range(nreq):
+        t += -math.log(max(1e-12, 1.0 - rng.uniform())) / rate_hz
+        w, h = resolutions[rng.randint(0, len(resolutions) - 1)]
+        spp = rng.randint(spp_range[0], spp_range[1])
+        sc = rng.randint(0, nscenes - 1)
+        n = dims[sc]
+        eye, target = scenes.orbit_camera(rng, (0.0, 0.0, 0.0), rng.uniform(1.2, 2.0) * n, 0.1 * n)
+        out.append(Request(sc, w, h, spp, scenes.perspective(scenes.FOV, w / h), scenes.look_at(eye, target),
+                           seed=1000 + i, max_bounces=max_bounces, sun_dir=scenes.sun_direction(rng), arrival=t))
+    return out
+
+
+def percentiles(lat) -> dict:
+    lat = np.asarray(lat)
+    return {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3), "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+            "mean_ms": round(float(lat.mean()) * 1e3, 3), "max_ms": round(float(lat.max()) * 1e3, 3)}
+
+
+# ---- autotuner ------------------------------------------------------------------------------
+TILE_CANDIDATES = (4, 8, 16, 32, 64)
+
+
+def autotune_tile(renderer, probe_reqs: list, scene_recs: list, cache_path: str | None = None, key: str = "",
+                  iters: int = 2, **cfg) -> dict:
+    """Times the batched loop on `probe_reqs` for every tile shape; returns the fastest.
+
+    Results are cached in cache_path (JSON) under `key` (e.g. the device name and workload)."""
+    if cache_path and os.path.exists(cache_path):
+        cache = json.load(open(cache_path))
+        if key in cache:
+            return cache[key]
+    timings = {}
+    for tw in TILE_CANDIDATES:
+        loop = ServingLoop(renderer, tile_w=tw, **cfg)
+        for recs in scene_recs:
+            loop.add_scene(recs)
+        best = float("inf")
+        for _ in range(iters):
+            t0 = time.perf_counter()
+            loop.run_all(probe_reqs)
+            best = min(best, time.perf_counter() - t0)
+        timings[tw] = best
+        loop.close()
+    tw = min(timings, key=timings.get)
+    res = {"tile_w": tw, "tile": f"{tw}x{64 // tw}", "seconds": {str(k): round(v, 5) for k, v in timings.items()}}
+    if cache_path:
+        cache = json.load(open(cache_path)) if os.path.exists(cache_path) else {}
+        cache[key] = res
+        json.dump(cache, open(cache_path, "w"), indent=1)
+    return res
