@@ -22,6 +22,11 @@ profiles/traffic.json when present (null otherwise).
 cpu_baseline: the CPU oracle (C++ restatement of the same path, host threads over rows) on a
 band of rows of the same frame, scaled to Mrays/s.
 
+serving: a seeded Poisson stream of requests (resolution from {256x256, 640x480, 1280x720,
+1920x1080}, spp 1..16, one of 32 resident noise volumes of 64^3..512^3) at 75% of the batching
+loop's measured capacity; p50/p99 latency (arrival -> radiance in host memory) for the
+continuous-batching loop and for the plain path serving the same arrivals one at a time.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
@@ -45,27 +50,36 @@ def algorithmic_bytes(st: dict) -> int:
 
 
 def cpu_baseline(scene, target_s: float = 15.0) -> dict:
-    """Times the oracle on every k-th row of the frame (k sized for about target_s)."""
+    """Times the oracle on whole frames (or every k-th row when a frame would exceed target_s)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi as O  # the oracle is the CPU baseline's timed code, never the product
 
     threads = max(1, min(16, os.cpu_count() or 1))
     osc = O.OracleScene(scene, use_bvh=True)  # volumes / LBVH prepared outside the timed sample
     H = scene.params.height
-    stride = 64
+    probe = list(range(32, H, 64))
     t0 = time.perf_counter()
-    osc.render_rows(list(range(stride // 2, H, stride)), threads=threads)
-    dt = time.perf_counter() - t0
-    nrows = max(threads, min(H, int(len(range(stride // 2, H, stride)) * target_s / max(dt, 1e-3))))
-    stride = max(1, H // nrows)
-    rows = list(range(stride // 2, H, stride))
+    osc.render_rows(probe, threads=threads)
+    frame_est = (time.perf_counter() - t0) * H / len(probe)
+    if frame_est <= target_s:
+        frames = max(1, min(10, int(target_s / max(frame_est, 1e-3))))
+        rows = list(range(H))
+        what = f"{frames} whole frame(s)"
+    else:
+        frames = 1
+        stride = max(2, int(round(frame_est / target_s)))
+        rows = list(range(stride // 2, H, stride))
+        what = f"every {stride}th row ({len(rows)} rows) of the frame"
+    rays = 0
     t0 = time.perf_counter()
-    st = osc.render_rows(rows, threads=threads)
+    for _ in range(frames):
+        rays += osc.render_rows(rows, threads=threads)["rays"]
     elapsed = time.perf_counter() - t0
     osc.close()
-    return {"value": round(st["rays"] / elapsed / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C++ restatement, -O2, {threads} threads) on every {stride}th row ({len(rows)} rows) of "
-                      f"the same 1920x1080 4-spp frame: {st['rays']} rays in {elapsed:.1f} s"}
+    p = scene.params
+    return {"value": round(rays / elapsed / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C++ restatement of the same path, -O2, {threads} host threads over rows), {what} of "
+                      f"the {p.width}x{p.height} {p.spp}-spp workload: {rays} rays in {elapsed:.1f} s"}
 
 
 def calibrate_copy(nbytes: int = 256 << 20):
@@ -100,6 +114,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true", help="run only a plain copy kernel (PMC scaling)")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--serving-requests", type=int, default=2000,
+                    help="requests in the serving stream (0 skips the serving measurement)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the renderer has no multi-GPU path); got --gpus {args.gpus}",
@@ -199,6 +215,10 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc)
+    if args.serving_requests > 0:
+        from vtrace_amd import serving
+        out["serving"] = serving.benchmark(r, nreq=args.serving_requests,
+                                           log=lambda m: print("# " + m, file=sys.stderr, flush=True))
     print(json.dumps(out), flush=True)
 
 

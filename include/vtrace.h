@@ -97,6 +97,12 @@ int32_t vt_add_volume_indexed(const uint8_t* indices, uint32_t width, uint32_t h
 int32_t vt_read_hits(void* dst, uint64_t bytes);
 int32_t vt_read_radiance(float* dst, uint64_t bytes);
 int32_t vt_read_bgra8(uint32_t* dst, uint64_t bytes);
+/* Optional G-buffer of frames in modes 1 and 2 (denoiser features), taken from each
+ * pixel's sample 0: 2 x float4 per pixel, (albedo rgb linear, distance to the camera) and
+ * (world unit normal, 1); a miss gives (sky rgb, 0) and (0, 0, 0, 0).  Enabling it costs
+ * 32 B per pixel of writes per frame.  bytes >= width*height*32. */
+int32_t vt_enable_gbuffer(uint32_t on);
+int32_t vt_read_gbuffer(float* dst, uint64_t bytes);
 /* 7 counters of the last frame rendered with stats = 1: rays, LBVH nodes, instances,
  * bricks, non-empty bricks, hits, DDA steps. */
 int32_t vt_read_stats(uint64_t* dst7);

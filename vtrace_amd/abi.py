@@ -15,7 +15,7 @@ EXTENSION_EXPORTS = ("vt_set_params", "vt_get_params", "vt_add_volume_indexed", 
                      "vt_bench_stages", "vt_read_volume",
                      "vt_volume_info", "vt_read_lbvh", "vt_instance_count", "vt_synchronize", "vt_last_error",
                      "vt_serve_init", "vt_serve_add_scene", "vt_serve_submit", "vt_serve_step", "vt_serve_read",
-                     "vt_serve_status", "vt_serve_shutdown")
+                     "vt_serve_status", "vt_serve_shutdown", "vt_enable_gbuffer", "vt_read_gbuffer")
 
 
 class RenderTickInfo(C.Structure):
@@ -59,7 +59,9 @@ def declare(lib: C.CDLL) -> None:
     lib.vt_get_params.argtypes = [C.POINTER(VtRenderParams)]
     lib.vt_add_volume_indexed.restype = C.c_int32
     lib.vt_add_volume_indexed.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
-    for name in ("vt_read_hits", "vt_read_radiance", "vt_read_bgra8"):
+    lib.vt_enable_gbuffer.restype = C.c_int32
+    lib.vt_enable_gbuffer.argtypes = [C.c_uint32]
+    for name in ("vt_read_hits", "vt_read_radiance", "vt_read_bgra8", "vt_read_gbuffer"):
         f = getattr(lib, name)
         f.restype = C.c_int32
         f.argtypes = [C.c_void_p, C.c_uint64]

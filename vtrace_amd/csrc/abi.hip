@@ -49,6 +49,8 @@ struct State {
     HitRecord* hits = nullptr;
     float4* rad = nullptr;
     uint32_t* bgra = nullptr;
+    float4* gbuf = nullptr;     // optional G-buffer, 2 x float4 per pixel
+    bool gbuf_on = false;
     size_t frame_cap = 0;
     void* wave_mem = nullptr;   // wavefront path state + queues
     size_t wave_cap = 0;        // pixel samples it holds
@@ -141,12 +143,13 @@ int ensure_frame_buffers() {
         g.wave_cap = 1;
     }
     size_t n = (size_t)g.params.width * g.params.height;
-    if (n <= g.frame_cap && g.hits) return 0;
-    hipFree(g.hits); hipFree(g.rad); hipFree(g.bgra);
-    g.hits = nullptr; g.rad = nullptr; g.bgra = nullptr;
-    hipError_t e;
+    if (n <= g.frame_cap && g.hits && (g.gbuf || !g.gbuf_on)) return 0;
+    hipFree(g.hits); hipFree(g.rad); hipFree(g.bgra); hipFree(g.gbuf);
+    g.hits = nullptr; g.rad = nullptr; g.bgra = nullptr; g.gbuf = nullptr;
+    hipError_t e = hipSuccess;
     if ((e = hipMalloc(&g.hits, n * sizeof(HitRecord))) != hipSuccess ||
-        (e = hipMalloc(&g.rad, n * sizeof(float4))) != hipSuccess || (e = hipMalloc(&g.bgra, n * 4)) != hipSuccess) {
+        (e = hipMalloc(&g.rad, n * sizeof(float4))) != hipSuccess || (e = hipMalloc(&g.bgra, n * 4)) != hipSuccess ||
+        (g.gbuf_on && (e = hipMalloc(&g.gbuf, n * 2 * sizeof(float4))) != hipSuccess)) {
         set_err("frame buffer allocation", e);
         g.frame_cap = 0;
         return -1;
@@ -316,7 +319,8 @@ int render_frame(const FrameGPU& F) {
     if (ensure_frame_buffers() != 0) return -1;
     if (F.stats) hipMemsetAsync(g.stats, 0, (size_t)VT_MAX_STAGES * VT_STAT_COUNT * 8, g.stream);
     hipEventRecord(g.ev0, g.stream);
-    int rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.wave, g.stats, nullptr,
+    int rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.gbuf_on ? g.gbuf : nullptr,
+                             g.wave, g.stats, nullptr,
                              g.stage_kinds, &g.nstages, g.stream);
     hipEventRecord(g.ev1, g.stream);
     if (rc != 0) {
@@ -468,7 +472,7 @@ void cleanup(void) {
     vt_lbvh_free(g.bvh);
     hipHostFree(g.inst_host);
     hipFree(g.inst_dev); hipFree(g.vol_table); hipFree(g.staging); hipFree(g.remap);
-    hipFree(g.hits); hipFree(g.rad); hipFree(g.bgra); hipFree(g.stats); hipFree(g.srgb_dec); hipFree(g.srgb_thr);
+    hipFree(g.hits); hipFree(g.rad); hipFree(g.bgra); hipFree(g.gbuf); hipFree(g.stats); hipFree(g.srgb_dec); hipFree(g.srgb_thr);
     hipFree(g.wave_mem);
     hipEventDestroy(g.ev0); hipEventDestroy(g.ev1); hipEventDestroy(g.inst_copied);
     hipStreamDestroy(g.stream);
@@ -570,6 +574,24 @@ static int32_t read_frame(const void* src, void* dst, uint64_t bytes, size_t ele
 int32_t vt_read_hits(void* dst, uint64_t bytes) { return read_frame(g.hits, dst, bytes, sizeof(HitRecord)); }
 int32_t vt_read_radiance(float* dst, uint64_t bytes) { return read_frame(g.rad, dst, bytes, sizeof(float4)); }
 
+int32_t vt_enable_gbuffer(uint32_t on) {
+    Guard G;
+    if (!g.inited) return -1;
+    g.gbuf_on = on != 0;
+    return 0;
+}
+
+int32_t vt_read_gbuffer(float* dst, uint64_t bytes) {
+    {
+        Guard G;
+        if (!g.gbuf_on || !g.gbuf || !g.have_frame || g.last_frame.mode == 0) {
+            set_err("no G-buffer: enable it and render a frame in mode 1 or 2");
+            return -1;
+        }
+    }
+    return read_frame(g.gbuf, dst, bytes, 2 * sizeof(float4));
+}
+
 int32_t vt_read_bgra8(uint32_t* dst, uint64_t bytes) {
     {
         Guard G;
@@ -630,7 +652,8 @@ int32_t vt_bench_stages(uint32_t warmup, uint32_t iters, float* kernel_ms, float
     hipEventRecord(start, g.stream);
     int rc = 0;
     for (uint32_t i = 0; i < iters && rc == 0; ++i)
-        rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.wave, g.stats,
+        rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.gbuf_on ? g.gbuf : nullptr,
+                             g.wave, g.stats,
                              &ev[per * i], kk, &ns, g.stream);
     hipEventRecord(stop, g.stream);
     hipEventSynchronize(stop);

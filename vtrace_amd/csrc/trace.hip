@@ -871,6 +871,10 @@ __global__ __launch_bounds__(kBlock) void k_primary(const ReqGPU* __restrict__ r
                 rec.inst = VT_EMPTY; rec.voxel = VT_EMPTY; rec.steps = 0; rec.face = 0xFF;
                 rec.pal = 0; rec.mask = 0; rec.flags = 0; rec.pad = 0;
                 Rq.hits[pix] = rec;
+                if (Rq.gbuf) {
+                    Rq.gbuf[2 * pix] = make_float4(F.sky[0], F.sky[1], F.sky[2], 0.0f);
+                    Rq.gbuf[2 * pix + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
             }
         } else {
             ctr.hit();
@@ -891,6 +895,12 @@ __global__ __launch_bounds__(kBlock) void k_primary(const ReqGPU* __restrict__ r
             }
             surface_of(S, h, sf);
             sf.path = w;
+            if (smp == 0 && Rq.gbuf) {
+                const float ex = sf.X[0] - F.O[0], ey = sf.X[1] - F.O[1], ez = sf.X[2] - F.O[2];
+                Rq.gbuf[2 * pix] = make_float4(srgb_dec[sf.rgba & 255u], srgb_dec[(sf.rgba >> 8) & 255u],
+                                               srgb_dec[(sf.rgba >> 16) & 255u], sqrtf((ex * ex + ey * ey) + ez * ez));
+                Rq.gbuf[2 * pix + 1] = make_float4(sf.n[0], sf.n[1], sf.n[2], 1.0f);
+            }
             float4 contrib, dirw;
             bool no_sun;
             shade(F, srgb_dec, sf, 0u, max_b, beta, rng, contrib, dirw, no_sun);
@@ -1330,8 +1340,9 @@ static int launch_render0(const FrameGPU& F, const SceneGPU& S, HitRecord* hits,
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, const WaveBuffers& wb,
-                    unsigned long long* stats, hipEvent_t* ev, uint8_t* kinds, uint32_t* nstages, hipStream_t s) {
+int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, float4* gbuf,
+                    const WaveBuffers& wb, unsigned long long* stats, hipEvent_t* ev, uint8_t* kinds, uint32_t* nstages,
+                    hipStream_t s) {
     if (!g_srgb_dec || F.mode > 2) return -1;
     if (F.mode == 0) {
         const bool st = F.stats != 0 && stats != nullptr;
@@ -1349,6 +1360,7 @@ int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float
     R.hits = hits;
     R.acc = nullptr;
     R.rad = rad;
+    R.gbuf = gbuf;
     R.s0 = 0;
     R.s1 = F.spp ? F.spp : 1u;
     R.final_chunk = 1;

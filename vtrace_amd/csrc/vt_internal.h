@@ -68,7 +68,8 @@ void vt_trace_set_persistent_blocks(uint32_t blocks);
 // Launches one frame.  stats: VT_MAX_STAGES x VT_STAT_COUNT counters (stage k at k*VT_STAT_COUNT).
 // ev (optional): 2 events per stage recorded around each launch.  kinds/nstages describe the
 // stages launched.
-int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, const WaveBuffers& wb,
+int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, float4* gbuf,
+                    const WaveBuffers& wb,
                     unsigned long long* stats, hipEvent_t* ev, uint8_t* kinds, uint32_t* nstages, hipStream_t s);
 int vt_encode_srgb8(const float4* rad, uint32_t* bgra, uint32_t n, hipStream_t s);
 // Batched path tracing (modes 1/2): n requests, their sample chunks traced in one pipeline.

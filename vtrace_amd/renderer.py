@@ -215,6 +215,18 @@ class Renderer:
             raise RendererError(self.error())
         return out
 
+    def enable_gbuffer(self, on: bool = True) -> None:
+        if self.lib.vt_enable_gbuffer(1 if on else 0) != 0:
+            raise RendererError(self.error())
+
+    def gbuffer(self) -> np.ndarray:
+        """(h, w, 8) float32: albedo rgb, distance, normal xyz, hit flag (sample 0 of each pixel)."""
+        w, h = self.window_width, self.window_height
+        out = np.zeros((h, w, 8), dtype=np.float32)
+        if self.lib.vt_read_gbuffer(out.ctypes.data, out.nbytes) != 0:
+            raise RendererError(self.error())
+        return out
+
     def bgra8(self) -> np.ndarray:
         w, h = self.window_width, self.window_height
         out = np.zeros((h, w), dtype=np.uint32)

@@ -287,7 +287,80 @@ def autotune_tile(renderer, probe_reqs: list, scene_recs: list, cache_path: str 
     tw = min(timings, key=timings.get)
     res = {"tile_w": tw, "tile": f"{tw}x{64 // tw}", "seconds": {str(k): round(v, 5) for k, v in timings.items()}}
     if cache_path:
-        cache = json.load(open(cache_path)) if os.path.exists(cache_path) else {}
-        cache[key] = res
-        json.dump(cache, open(cache_path, "w"), indent=1)
+        try:
+            os.makedirs(os.path.dirname(os.path.abspath(cache_path)), exist_ok=True)
+            cache = json.load(open(cache_path)) if os.path.exists(cache_path) else {}
+            cache[key] = res
+            with open(cache_path, "w") as f:
+                json.dump(cache, f, indent=1)
+        except (OSError, ValueError):
+            pass  # an unwritable cache only costs a re-tune next time
     return res
+
+
+# ---- benchmark ------------------------------------------------------------------------------
+def default_tune_cache() -> str:
+    base = os.environ.get("XDG_CACHE_HOME") or os.path.join(os.path.expanduser("~"), ".cache")
+    return os.path.join(base, "vtrace_amd", "autotune.json")
+
+
+def benchmark(renderer, nreq: int = 2000, util: float = 0.75, seed: int = 11, probe: int = 24, max_slots: int = 32,
+              spp_chunk: int = 4, memory_budget: int = 8 << 30, max_work: int = 8 << 20, max_bounces: int = 4,
+              use_graph: bool = True, tune_cache: str | None = None, plain: bool = True, log=None) -> dict:
+    """The serving workload: 32 resident noise volumes (64^3..512^3, fill 5-40%), a seeded Poisson
+    stream of nreq requests at `util` of the measured batched capacity, served by the batching loop
+    and then by the plain path one request at a time (same arrivals).  Latency = arrival -> result
+    read back to host memory, wall clock."""
+    say = log or (lambda *a: None)
+    t0 = time.perf_counter()
+    vols = resident_volumes()
+    dims = [v.dims[0] for v in vols]
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ids = [renderer.upload(v) for v in vols]
+    renderer.lib.vt_synchronize()
+    t_up = time.perf_counter() - t0
+    resident = sum(renderer.volume_info(i)["bytes"] for i in ids)
+    del vols
+    recs = scene_records(ids, dims)
+    say(f"serving: 32 volumes generated in {t_gen:.1f} s, bricked in {t_up:.1f} s, {resident / 2**20:.0f} MiB resident")
+    cfg = dict(max_slots=max_slots, spp_chunk=spp_chunk, memory_budget=memory_budget, max_work=max_work,
+               max_bounces=max_bounces)
+    probe_reqs = request_stream(probe, len(recs), dims, rate_hz=1.0, seed=seed + 1, max_bounces=max_bounces)
+    tune = autotune_tile(renderer, probe_reqs[: max(4, probe // 3)], recs, cache_path=tune_cache or default_tune_cache(),
+                         key=f"serve32:{max_bounces}b:{spp_chunk}c", **cfg)
+    say(f"serving: tile {tune['tile']} ({tune['seconds']})")
+    loop = ServingLoop(renderer, tile_w=tune["tile_w"], use_graph=use_graph, **cfg)
+    for rc in recs:
+        loop.add_scene(rc)
+    loop.run_all(probe_reqs[:4])  # warm-up (graph capture, first-touch of buffers)
+    t0 = time.perf_counter()
+    loop.run_all(probe_reqs)
+    t_probe = time.perf_counter() - t0
+    service = t_probe / len(probe_reqs)
+    rate = util / service
+    reqs = request_stream(nreq, len(recs), dims, rate_hz=rate, seed=seed, max_bounces=max_bounces)
+    say(f"serving: capacity {1 / service:.1f} req/s (probe of {probe}), stream of {nreq} at {rate:.1f} req/s")
+    st0 = loop.status()
+    res = loop.run_stream(reqs)
+    st1 = loop.status()
+    loop.close()
+    p = percentiles(res["latency_s"])
+    out = {"requests": nreq, "arrival_rate_hz": round(rate, 3), "target_utilisation": util,
+           "capacity_req_s": round(1 / service, 3), "tile": tune["tile"],
+           "batched": dict(p, wall_s=round(res["wall_s"], 3), throughput_req_s=round(nreq / res["wall_s"], 3),
+                           msamples_s=round(res["samples"] / res["wall_s"] / 1e6, 2),
+                           iterations=int(st1["iterations"] - st0["iterations"]),
+                           mean_batch_samples=int((st1["samples"] - st0["samples"]) /
+                                                  max(1, st1["iterations"] - st0["iterations"]))),
+           "resident_volume_mib": round(resident / 2**20, 1),
+           "config": {"max_slots": max_slots, "spp_chunk": spp_chunk, "memory_budget_gib": memory_budget / 2**30,
+                      "max_work": max_work, "graph": bool(use_graph), "max_bounces": max_bounces}}
+    say(f"serving: batched p50 {p['p50_ms']} ms p99 {p['p99_ms']} ms over {res['wall_s']:.1f} s")
+    if plain:
+        pr = plain_one_at_a_time(renderer, recs, reqs)
+        pp = percentiles(pr["latency_s"])
+        out["plain"] = dict(pp, wall_s=round(pr["wall_s"], 3), throughput_req_s=round(nreq / pr["wall_s"], 3),
+                            msamples_s=round(res["samples"] / pr["wall_s"] / 1e6, 2))
+        say(f"serving: plain p50 {pp['p50_ms']} ms p99 {pp['p99_ms']} ms over {pr['wall_s']:.1f} s")
+    return out
