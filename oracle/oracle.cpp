@@ -16,8 +16,8 @@
 //                                     lib/pipeline.c:129-137 (src-alpha blend)
 //   * instance record:                src/render.rs:31-35,74-78 (texture id in m[3][3])
 // The reference has no LBVH, bricks, RNG, shading or path tracing; those parts follow
-// this project's own specification (documented in DESIGN.md of the repository) and are
-// pinned in tests by brute force, closed forms and invariants rather than by the reference.
+// this project's own specification (stated at each function below) and are pinned in
+// tests by brute force, closed forms and invariants rather than by the reference.
 //
 // Numerics: compiled with -ffp-contract=off so every FP32 expression below is evaluated
 // as written, one IEEE-rounded operation at a time (GLSL evaluation order, left to right).
