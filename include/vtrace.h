@@ -153,7 +153,7 @@ typedef struct vt_request {
     float sky[3];
 } vt_request;
 
-typedef struct vt_serve_status {
+typedef struct vt_serve_stats {
     uint32_t queued, active, done_unread, tile_w;
     uint64_t memory_in_use;
     uint64_t iterations;
@@ -178,6 +178,43 @@ int32_t vt_serve_step(int64_t* completed, uint32_t max_completed);
 int32_t vt_serve_read(int64_t id, float* rgba, uint64_t bytes, void* hits, uint64_t hit_bytes);
 int32_t vt_serve_status(vt_serve_stats* status);
 void vt_serve_shutdown(void);
+
+/* ---------------- learned denoiser (5-level conv U-Net, BF16 MFMA, FP32 accumulate) -------- */
+
+typedef struct vt_dn_config {
+    uint32_t enc[5];            /* encoder widths per level (level 4 = bottleneck) */
+    uint32_t dec[4];            /* decoder widths per level; all multiples of 16, <= 96 */
+} vt_dn_config;
+
+/* Builds the network (parameters zero until vt_dn_set_params). */
+int32_t vt_dn_init(const vt_dn_config* config);
+/* Layer table: 4 u32 per layer {cin, cout, weight offset, bias offset} into the FP32
+ * parameter vector (weights [cout][3][3][cin] row-major).  Returns the layer count (19). */
+int32_t vt_dn_layers(uint32_t* out4, uint32_t max_layers);
+uint64_t vt_dn_param_count(void);
+/* Sets / reads the FP32 parameters (set also resets the Adam state). */
+int32_t vt_dn_set_params(const float* params, uint64_t n);
+int32_t vt_dn_get_params(float* params, uint64_t n);
+/* One forward + backward pass without an update: x = n x h x w x 16 FP32 features
+ * (rounded to BF16), target = n x h x w x 3; writes the output (n x h x w x 3), the FP32
+ * gradient of the mean relMSE loss for every parameter, and the loss.  h, w % 16 == 0. */
+int32_t vt_dn_grad(const float* x, const float* target, uint32_t n, uint32_t h, uint32_t w, float* y, float* grad,
+                   float* loss);
+/* The 16 input features of a frame (radiance RGBA32F + vt_read_gbuffer layout), as FP32
+ * values of the BF16 tensor the network reads. */
+int32_t vt_dn_features(const float* rad_rgba, const float* gbuf8, uint32_t h, uint32_t w, float* out16);
+/* Training frames kept in device memory: input radiance + G-buffer, target radiance. */
+int32_t vt_dn_add_frame(const float* rad_rgba, const float* gbuf8, const float* target_rgba, uint32_t h, uint32_t w);
+int32_t vt_dn_clear_frames(void);
+/* iters Adam steps on random crop x crop patches (batch per step, seeded); losses[iters]. */
+int32_t vt_dn_train(uint32_t iters, uint32_t batch, uint32_t crop, float lr, uint32_t seed, float* losses);
+/* Denoises an h x w frame (host buffers) into RGBA32F; gpu_ms = device time. */
+int32_t vt_dn_denoise(const float* rad_rgba, const float* gbuf8, uint32_t h, uint32_t w, float* out_rgba,
+                      float* gpu_ms);
+/* Denoises the last frame of render_tick (mode 1/2 with the G-buffer enabled) in place on
+ * the device and copies the result out. */
+int32_t vt_dn_denoise_frame(float* out_rgba, float* gpu_ms);
+void vt_dn_shutdown(void);
 
 #ifdef __cplusplus
 }

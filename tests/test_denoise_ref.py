@@ -1,0 +1,122 @@
+"""CPU: pins of the FP64 reference of the denoiser network (oracle/unet_ref.py).
+
+The reference is the checker of the HIP denoiser; here it is checked itself: convolution
+against scipy.signal.correlate (an independent implementation), every gradient against
+central finite differences in FP64, and BF16 rounding against hand-computed values."""
+import os
+import sys
+
+import numpy as np
+import pytest
+from scipy import signal
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import unet_ref as U  # noqa: E402
+
+
+def test_conv3x3_matches_scipy_correlate():
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(2, 7, 9, 5))
+    w = rng.normal(size=(4, 3, 3, 5))
+    b = rng.normal(size=4)
+    y = U.conv3x3(x, w, b)
+    for n in range(2):
+        for co in range(4):
+            ref = b[co] + sum(signal.correlate(x[n, :, :, ci], w[co, :, :, ci], mode="same") for ci in range(5))
+            assert np.allclose(y[n, :, :, co], ref, atol=1e-12)
+
+
+def test_conv3x3_backward_is_the_adjoint():
+    # <conv(x), g> is linear in x and w: its gradients must satisfy the adjoint identities
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=(1, 6, 5, 3))
+    w = rng.normal(size=(2, 3, 3, 3))
+    g = rng.normal(size=(1, 6, 5, 2))
+    zb = np.zeros(2)
+    dx, dw, db = U.conv3x3_backward(x, w, g)
+    assert np.isclose(np.sum(U.conv3x3(x, w, zb) * g), np.sum(dx * x))
+    assert np.isclose(np.sum(U.conv3x3(x, w, zb) * g), np.sum(dw * w))
+    assert np.allclose(db, g.sum(axis=(0, 1, 2)))
+
+
+def test_bf16_rounding_known_values():
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -9, -2.5, 3.0e-39, 1.0 + 2 ** -8 + 2 ** -20])
+    want = np.array([1.0, 1.0, 1.0 + 4 * 2 ** -8, 1.0, -2.5, U.bf16_round(np.array([3.0e-39]))[0], 1.0 + 2 ** -7])
+    assert np.array_equal(U.bf16_round(v), want)
+    # every BF16 value is a fixed point
+    r = U.bf16_round(np.random.default_rng(2).normal(size=1000))
+    assert np.array_equal(U.bf16_round(r), r)
+
+
+def _tiny():
+    enc, dec = (16, 16, 16, 16, 16), (16, 16, 16, 16)
+    layers = U.layer_table(enc, dec)
+    n = layers[-1][3] + layers[-1][1]
+    rng = np.random.default_rng(3)
+    params = np.zeros(n)
+    for i, (cin, cout, wo, bo) in enumerate(layers):
+        params[wo:wo + cout * 9 * cin] = rng.normal(0.0, np.sqrt(2.0 / (9 * cin)), cout * 9 * cin)
+        params[bo:bo + cout] = rng.normal(0.0, 0.05, cout)
+    return U.UNetRef(layers, enc, dec), params, rng
+
+
+def test_layer_table_offsets_are_contiguous():
+    layers = U.layer_table((32, 32, 48, 64, 96), (64, 48, 32, 32))
+    off = 0
+    for cin, cout, wo, bo in layers:
+        assert wo == off and bo == wo + cout * 9 * cin
+        off = bo + cout
+    assert len(layers) == 19 and layers[10][0] == 96 + 64 and layers[16][0] == 48 + 32
+    assert 450_000 < off < 600_000  # about 0.5 M parameters
+
+
+def test_gradients_match_finite_differences():
+    net, params, rng = _tiny()
+    x = rng.normal(size=(1, 16, 16, 16))
+    x[..., 10:] = 0.0
+    t = np.abs(rng.normal(size=(1, 16, 16, 3)))
+    _, loss, g = net.loss_and_grads(params, x, t)
+    # every layer: its first weight, a random weight and its bias
+    idx = []
+    for cin, cout, wo, bo in net.layers:
+        idx += [wo, wo + int(rng.integers(cout * 9 * cin)), bo + int(rng.integers(cout))]
+    eps = 1e-6
+    for i in idx:
+        p1, p2 = params.copy(), params.copy()
+        p1[i] += eps
+        p2[i] -= eps
+        l1 = net.loss_and_grads(p1, x, t)[1]
+        l2 = net.loss_and_grads(p2, x, t)[1]
+        fd = (l1 - l2) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
+
+
+def test_bf16_mode_stays_close_to_fp64():
+    # BF16 rounding of the stored tensors perturbs the output by well under a percent; the
+    # backward pass's own rounding (on the FP64 forward state) perturbs every layer's gradient
+    # by about a percent.  (BF16 in the forward pass also flips a few ReLU masks, which moves
+    # the gradients of tiny layers by more: that is a property of the network, not an error.)
+    net, params, rng = _tiny()
+    x = U.bf16_round(rng.normal(size=(1, 16, 16, 16)))
+    t = np.abs(rng.normal(size=(1, 16, 16, 3)))
+    y64, c64 = net.forward(params, x)
+    y16, _ = net.forward(params, x, bf16=True)
+    assert np.linalg.norm(y16 - y64) / np.linalg.norm(y64) < 3e-2
+    _, gy = net.loss(y64, t)
+    g64 = net.backward(params, c64, gy)
+    g16 = net.backward(params, c64, gy, bf16=True)
+    for cin, cout, wo, bo in net.layers:
+        a, b = g64[wo:bo + cout], g16[wo:bo + cout]
+        assert np.linalg.norm(b - a) / np.linalg.norm(a) < 3e-2
+
+
+def test_features_formula():
+    rad = np.zeros((2, 3, 4), dtype=np.float32)
+    rad[0, 0, :3] = [0.0, 1.0, np.e - 1.0]
+    rad[0, 1, :3] = [-1.0, 0.5, 2.0]
+    gb = np.zeros((2, 3, 8), dtype=np.float32)
+    gb[0, 0] = [0.1, 0.2, 0.3, np.e ** 8 - 1.0, 0.0, 1.0, 0.0, 1.0]
+    f = U.features(rad, gb)
+    assert np.allclose(f[0, 0, :10], [0.0, np.log(2.0), 1.0, 0.1, 0.2, 0.3, 0.0, 1.0, 0.0, 1.0], atol=1e-6)
+    assert f[0, 1, 0] == 0.0 and np.isclose(f[0, 1, 2], np.log(3.0))
+    assert np.all(f[..., 10:] == 0.0)

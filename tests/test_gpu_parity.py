@@ -123,3 +123,57 @@ def test_invalid_texture_id_and_zero_instances():
     o_hits, _, _ = O.render(sc)
     assert_hits_equal(g_hits, o_hits, "invalid tex")
     assert not (g_hits["inst"] == 1).any()
+
+
+def test_gbuffer_consistent_with_hits():
+    """G-buffer of sample 0: albedo = sRGB-decoded palette colour of the hit voxel, normal =
+    the hit face's outward axis (translate-scale instances), and the point at `distance` along
+    the pixel-centre camera ray lies on that face of the hit voxel; misses give the sky."""
+    from vtrace_amd import renderer as R
+    sc = scenes.multi_volume_scene(count=4, n=64, world=384, seed=13, width=160, height=96, spp=1, max_bounces=2)
+    p = copy.copy(sc.params)
+    p.jitter = 0
+    sc.params = p
+    r = R.reset_renderer()
+    R.load_scene(r, sc)
+    r.enable_gbuffer(True)
+    _say("case: G-buffer 4x64^3 160x96")
+    r.render(sc.P, sc.C)
+    hits, gb = r.hits(), r.gbuffer()
+    r.enable_gbuffer(False)
+    hit = hits["flags"] & 1 == 1
+    assert hit.mean() > 0.2
+    assert np.all(gb[~hit][:, 3] == 0.0) and np.all(gb[~hit][:, 4:] == 0.0)
+    assert np.allclose(gb[~hit][:, :3], np.array(p.sky, dtype=np.float32))
+    dec = np.array([O.lib().orc_srgb_decode(v) for v in range(256)], dtype=np.float32)
+    Pm = np.asarray(sc.P, np.float64).reshape(4, 4).T
+    Cm = np.asarray(sc.C, np.float64).reshape(4, 4).T
+    invC = np.linalg.inv(Cm)
+    Rm, O_ = invC[:3, :3], invC[:3, 3]
+    ys, xs = np.nonzero(hit)
+    for y, x in list(zip(ys, xs))[::7]:
+        h = hits[y, x]
+        inst = int(h["inst"])
+        rec = np.asarray(sc.records[inst], np.float32).copy()
+        vol = sc.volumes[int(rec.view(np.uint32)[15])]
+        vx, vy, vz = int(h["voxel"]) & 1023, (int(h["voxel"]) >> 10) & 1023, (int(h["voxel"]) >> 20) & 1023
+        rgba = int(vol.palette[int(h["pal"])])
+        assert np.array_equal(gb[y, x, :3], dec[[rgba & 255, (rgba >> 8) & 255, (rgba >> 16) & 255]])
+        face = int(h["face"])
+        a, neg = face >> 1, (face & 1) == 0
+        n = np.zeros(3, np.float32)
+        n[a] = -1.0 if neg else 1.0
+        assert np.allclose(gb[y, x, 4:7], n, atol=1e-6) and gb[y, x, 7] == 1.0
+        nx = 2.0 * (x + 0.5) / p.width - 1.0
+        ny = 2.0 * (y + 0.5) / p.height - 1.0
+        D = Rm @ np.array([nx / Pm[0, 0], ny / Pm[1, 1], -1.0])
+        X = O_ + gb[y, x, 3] * D / np.linalg.norm(D)
+        M = rec.astype(np.float64).reshape(4, 4).T
+        M[3, 3] = 1.0
+        Xm = np.linalg.solve(M, np.append(X, 1.0))[:3]
+        v = (Xm + 0.5) * np.array(vol.dims, np.float64)
+        vox = np.array([vx, vy, vz], np.float64)
+        want = vox[a] + (0.0 if neg else 1.0)
+        assert abs(v[a] - want) < 0.05, (y, x, v, vox, face)
+        others = [k for k in range(3) if k != a]
+        assert np.all(v[others] > vox[others] - 0.05) and np.all(v[others] < vox[others] + 1.05)

@@ -15,7 +15,10 @@ EXTENSION_EXPORTS = ("vt_set_params", "vt_get_params", "vt_add_volume_indexed", 
                      "vt_bench_stages", "vt_read_volume",
                      "vt_volume_info", "vt_read_lbvh", "vt_instance_count", "vt_synchronize", "vt_last_error",
                      "vt_serve_init", "vt_serve_add_scene", "vt_serve_submit", "vt_serve_step", "vt_serve_read",
-                     "vt_serve_status", "vt_serve_shutdown", "vt_enable_gbuffer", "vt_read_gbuffer")
+                     "vt_serve_status", "vt_serve_shutdown", "vt_enable_gbuffer", "vt_read_gbuffer",
+                     "vt_dn_init", "vt_dn_layers", "vt_dn_param_count", "vt_dn_set_params", "vt_dn_get_params",
+                     "vt_dn_grad", "vt_dn_features", "vt_dn_add_frame", "vt_dn_clear_frames", "vt_dn_train",
+                     "vt_dn_denoise", "vt_dn_denoise_frame", "vt_dn_shutdown")
 
 
 class RenderTickInfo(C.Structure):
@@ -88,6 +91,7 @@ def declare(lib: C.CDLL) -> None:
     lib.vt_synchronize.argtypes = []
     lib.vt_last_error.restype = C.c_char_p
     lib.vt_last_error.argtypes = []
-    from . import serving
+    from . import denoise, serving
     serving.declare(lib)
+    denoise.declare(lib)
     lib._vt_declared = True

@@ -342,6 +342,14 @@ hipStream_t vt_rt_stream() { return g.stream; }
 const VolumeDesc* vt_rt_vol_table() { return g.vol_table; }
 uint32_t vt_rt_nvol() { return (uint32_t)g.vols.size(); }
 void vt_rt_set_error(const char* what) { set_err(what); }
+bool vt_rt_frame(const float4** rad, const float4** gbuf, uint32_t* width, uint32_t* height) {
+    if (!g.inited || !g.have_frame || g.last_frame.mode == 0 || !g.gbuf_on || !g.gbuf) return false;
+    *rad = reinterpret_cast<const float4*>(g.rad);
+    *gbuf = g.gbuf;
+    *width = g.last_frame.width;
+    *height = g.last_frame.height;
+    return true;
+}
 
 namespace {
 
@@ -465,6 +473,7 @@ void cleanup(void) {
     hipSetDevice(g.device);
     hipStreamSynchronize(g.stream);
     vt_serve_release();
+    vt_dn_release();
     for (auto& v : g.vols) {
         hipFree(v.brick_map); hipFree(v.masks); hipFree(v.payload); hipFree(v.palette);
     }

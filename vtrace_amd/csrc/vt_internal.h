@@ -91,7 +91,11 @@ hipStream_t vt_rt_stream();
 const VolumeDesc* vt_rt_vol_table();
 uint32_t vt_rt_nvol();
 void vt_rt_set_error(const char* what);
+// last plain-path frame's radiance and G-buffer (device); false if there is none (caller holds the lock)
+bool vt_rt_frame(const float4** rad, const float4** gbuf, uint32_t* width, uint32_t* height);
 // frame constants from the projection / camera matrices and render parameters
 bool vt_build_frame(const float* P, const float* C, const vt_render_params& p, FrameGPU& F);
 // serve.hip: frees the serving loop (caller holds the runtime lock)
 void vt_serve_release();
+// denoise.hip: frees the denoiser (caller holds the runtime lock)
+void vt_dn_release();

@@ -1,0 +1,209 @@
+"""Learned denoiser: thin wrapper over the vt_dn_* functions of librender_hip (denoise.hip).
+
+The network (5-level conv U-Net, BF16 MFMA with FP32 accumulation), its training loop
+(Adam on random crops of device-resident frames) and inference all run in the library on
+the GPU; this module declares the C functions, initialises parameters, renders training
+pairs with the plain path and measures quality (relMSE against a high-spp render).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+
+import numpy as np
+
+from . import scenes
+from .renderer import RendererError
+
+ENC = (32, 32, 48, 64, 96)
+DEC = (64, 48, 32, 32)
+N_FEATURES = 10   # of the 16 input channels; the rest are zero padding
+
+
+class VtDnConfig(C.Structure):
+    _fields_ = [("enc", C.c_uint32 * 5), ("dec", C.c_uint32 * 4)]
+
+
+def declare(lib):
+    if getattr(lib, "_vt_dn_declared", False):
+        return
+    f = C.POINTER(C.c_float)
+    lib.vt_dn_init.restype = C.c_int32
+    lib.vt_dn_init.argtypes = [C.POINTER(VtDnConfig)]
+    lib.vt_dn_layers.restype = C.c_int32
+    lib.vt_dn_layers.argtypes = [C.c_void_p, C.c_uint32]
+    lib.vt_dn_param_count.restype = C.c_uint64
+    lib.vt_dn_param_count.argtypes = []
+    lib.vt_dn_set_params.restype = C.c_int32
+    lib.vt_dn_set_params.argtypes = [C.c_void_p, C.c_uint64]
+    lib.vt_dn_get_params.restype = C.c_int32
+    lib.vt_dn_get_params.argtypes = [C.c_void_p, C.c_uint64]
+    lib.vt_dn_grad.restype = C.c_int32
+    lib.vt_dn_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, f]
+    lib.vt_dn_features.restype = C.c_int32
+    lib.vt_dn_features.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+    lib.vt_dn_add_frame.restype = C.c_int32
+    lib.vt_dn_add_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]
+    lib.vt_dn_clear_frames.restype = C.c_int32
+    lib.vt_dn_clear_frames.argtypes = []
+    lib.vt_dn_train.restype = C.c_int32
+    lib.vt_dn_train.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, C.c_void_p]
+    lib.vt_dn_denoise.restype = C.c_int32
+    lib.vt_dn_denoise.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, f]
+    lib.vt_dn_denoise_frame.restype = C.c_int32
+    lib.vt_dn_denoise_frame.argtypes = [C.c_void_p, f]
+    lib.vt_dn_shutdown.restype = None
+    lib.vt_dn_shutdown.argtypes = []
+    lib._vt_dn_declared = True
+
+
+def relmse(x: np.ndarray, ref: np.ndarray) -> float:
+    """mean((x - r)^2 / (r^2 + 0.01)) over pixels and RGB."""
+    x = np.asarray(x, dtype=np.float64)[..., :3]
+    r = np.asarray(ref, dtype=np.float64)[..., :3]
+    return float(np.mean((x - r) ** 2 / (r * r + 0.01)))
+
+
+class Denoiser:
+    def __init__(self, renderer, enc=ENC, dec=DEC, seed: int | None = 0):
+        self.r = renderer
+        self.lib = renderer.lib
+        declare(self.lib)
+        self.enc, self.dec = tuple(enc), tuple(dec)
+        cfg = VtDnConfig((C.c_uint32 * 5)(*self.enc), (C.c_uint32 * 4)(*self.dec))
+        if self.lib.vt_dn_init(C.byref(cfg)) != 0:
+            raise RendererError(f"vt_dn_init failed: {renderer.error()}")
+        tab = np.zeros((32, 4), dtype=np.uint32)
+        n = self.lib.vt_dn_layers(tab.ctypes.data, 32)
+        self.layers = [tuple(int(v) for v in tab[i]) for i in range(n)]
+        self.nparam = int(self.lib.vt_dn_param_count())
+        if seed is not None:
+            self.set_params(self.init_params(seed))
+
+    @property
+    def real_param_count(self) -> int:
+        """Parameters excluding the zero padding of the input channels."""
+        cin, cout = self.layers[0][0], self.layers[0][1]
+        return self.nparam - cout * 9 * (cin - N_FEATURES)
+
+    def init_params(self, seed: int) -> np.ndarray:
+        """He-normal weights (fan-in of the real inputs), zero biases, zero padded channels."""
+        rng = np.random.default_rng(seed)
+        p = np.zeros(self.nparam, dtype=np.float32)
+        for i, (cin, cout, wo, bo) in enumerate(self.layers):
+            fan = 9 * (N_FEATURES if i == 0 else cin)
+            std = math.sqrt(2.0 / fan) if i < len(self.layers) - 1 else math.sqrt(1.0 / fan)
+            w = rng.normal(0.0, std, size=(cout, 3, 3, cin)).astype(np.float32)
+            if i == 0:
+                w[..., N_FEATURES:] = 0.0
+            p[wo:wo + w.size] = w.reshape(-1)
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.nparam or self.lib.vt_dn_set_params(p.ctypes.data, p.size) != 0:
+            raise RendererError("vt_dn_set_params failed")
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.nparam, dtype=np.float32)
+        if self.lib.vt_dn_get_params(p.ctypes.data, p.size) != 0:
+            raise RendererError("vt_dn_get_params failed")
+        return p
+
+    def grad(self, x: np.ndarray, target: np.ndarray):
+        """x (n,h,w,16), target (n,h,w,3) -> (y, grads, loss) of one forward+backward pass."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        n, h, w, _ = x.shape
+        y = np.zeros((n, h, w, 3), dtype=np.float32)
+        g = np.zeros(self.nparam, dtype=np.float32)
+        loss = C.c_float(0.0)
+        if self.lib.vt_dn_grad(x.ctypes.data, t.ctypes.data, n, h, w, y.ctypes.data, g.ctypes.data,
+                               C.byref(loss)) != 0:
+            raise RendererError(f"vt_dn_grad failed: {self.r.error()}")
+        return y, g, float(loss.value)
+
+    def features(self, rad: np.ndarray, gbuf: np.ndarray) -> np.ndarray:
+        rad = np.ascontiguousarray(rad, dtype=np.float32)
+        gbuf = np.ascontiguousarray(gbuf, dtype=np.float32)
+        h, w = rad.shape[:2]
+        out = np.zeros((h, w, 16), dtype=np.float32)
+        if self.lib.vt_dn_features(rad.ctypes.data, gbuf.ctypes.data, h, w, out.ctypes.data) != 0:
+            raise RendererError("vt_dn_features failed")
+        return out
+
+    def add_frame(self, rad: np.ndarray, gbuf: np.ndarray, target: np.ndarray) -> int:
+        rad = np.ascontiguousarray(rad, dtype=np.float32)
+        gbuf = np.ascontiguousarray(gbuf, dtype=np.float32)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        h, w = rad.shape[:2]
+        k = self.lib.vt_dn_add_frame(rad.ctypes.data, gbuf.ctypes.data, target.ctypes.data, h, w)
+        if k < 0:
+            raise RendererError(f"vt_dn_add_frame failed: {self.r.error()}")
+        return k
+
+    def clear_frames(self):
+        self.lib.vt_dn_clear_frames()
+
+    def train(self, iters: int, batch: int = 8, crop: int = 128, lr: float = 2e-3, seed: int = 1) -> np.ndarray:
+        losses = np.zeros(iters, dtype=np.float32)
+        if self.lib.vt_dn_train(iters, batch, crop, lr, seed, losses.ctypes.data) != 0:
+            raise RendererError(f"vt_dn_train failed: {self.r.error()}")
+        return losses
+
+    def denoise(self, rad: np.ndarray, gbuf: np.ndarray):
+        rad = np.ascontiguousarray(rad, dtype=np.float32)
+        gbuf = np.ascontiguousarray(gbuf, dtype=np.float32)
+        h, w = rad.shape[:2]
+        out = np.zeros((h, w, 4), dtype=np.float32)
+        ms = C.c_float(0.0)
+        if self.lib.vt_dn_denoise(rad.ctypes.data, gbuf.ctypes.data, h, w, out.ctypes.data, C.byref(ms)) != 0:
+            raise RendererError(f"vt_dn_denoise failed: {self.r.error()}")
+        return out, float(ms.value)
+
+    def denoise_frame(self):
+        """Denoises the renderer's last frame (G-buffer enabled) without a host round trip."""
+        w, h = self.r.window_width, self.r.window_height
+        out = np.zeros((h, w, 4), dtype=np.float32)
+        ms = C.c_float(0.0)
+        if self.lib.vt_dn_denoise_frame(out.ctypes.data, C.byref(ms)) != 0:
+            raise RendererError(f"vt_dn_denoise_frame failed: {self.r.error()}")
+        return out, float(ms.value)
+
+    def close(self):
+        self.lib.vt_dn_shutdown()
+
+
+# ---- rendering helpers ----------------------------------------------------------------------
+def render_spp(r, scene: "scenes.Scene", spp: int, seed: int, chunk: int = 16, gbuffer: bool = False):
+    """Radiance of `spp` samples per pixel as the mean of frames of `chunk` samples (frame
+    index = independent RNG stream).  Returns (radiance (h,w,4), G-buffer or None, GPU ms)."""
+    p = scene.params
+    per = min(spp, chunk)
+    nfr = max(1, spp // per)
+    r.enable_gbuffer(gbuffer)
+    acc = None
+    gb = None
+    ms = 0.0
+    for k in range(nfr):
+        prm = scenes.RenderParams(p.width, p.height, viewport_square=p.viewport_square, mode=2, spp=per,
+                                  max_bounces=p.max_bounces, jitter=1, seed=seed, frame=k, sun_dir=p.sun_dir,
+                                  sun_radiance=p.sun_radiance, sky=p.sky)
+        r.set_params(prm)
+        r.render(scene.P, scene.C)
+        ms += r.last_frame_ms()
+        rad = r.radiance().astype(np.float64)
+        acc = rad if acc is None else acc + rad
+        if gbuffer and k == 0:
+            gb = r.gbuffer()
+    r.enable_gbuffer(False)
+    return (acc / nfr).astype(np.float32), gb, ms
+
+
+def time_to_quality(curve: list, threshold: float):
+    """First (time, relMSE) point of a (time, relMSE) curve at or below threshold, else None."""
+    for t, e in curve:
+        if e <= threshold:
+            return t
+    return None
