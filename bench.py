@@ -27,6 +27,10 @@ serving: a seeded Poisson stream of requests (resolution from {256x256, 640x480,
 loop's measured capacity; p50/p99 latency (arrival -> radiance in host memory) for the
 continuous-batching loop and for the plain path serving the same arrivals one at a time.
 
+denoiser: a 5-level BF16 conv U-Net trained on the GPU (own kernels) on (1/4/16 spp, 1024 spp)
+pairs of other seeded scenes of the same kind; device time to reach relMSE <= 1e-3 against a
+4096-spp render of the benchmark frame, denoised vs plain.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
@@ -116,6 +120,8 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--serving-requests", type=int, default=2000,
                     help="requests in the serving stream (0 skips the serving measurement)")
+    ap.add_argument("--denoiser-iters", type=int, default=2000,
+                    help="training iterations of the denoiser (0 skips the denoiser measurement)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the renderer has no multi-GPU path); got --gpus {args.gpus}",
@@ -219,6 +225,10 @@ def main():
         from vtrace_amd import serving
         out["serving"] = serving.benchmark(r, nreq=args.serving_requests,
                                            log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+    if args.denoiser_iters > 0:
+        from vtrace_amd import denoise
+        out["denoiser"] = denoise.benchmark(r, sc, iters=args.denoiser_iters,
+                                            log=lambda m: print("# " + m, file=sys.stderr, flush=True))
     print(json.dumps(out), flush=True)
 
 

@@ -97,10 +97,10 @@ int32_t vt_add_volume_indexed(const uint8_t* indices, uint32_t width, uint32_t h
 int32_t vt_read_hits(void* dst, uint64_t bytes);
 int32_t vt_read_radiance(float* dst, uint64_t bytes);
 int32_t vt_read_bgra8(uint32_t* dst, uint64_t bytes);
-/* Optional G-buffer of frames in modes 1 and 2 (denoiser features), taken from each
- * pixel's sample 0: 2 x float4 per pixel, (albedo rgb linear, distance to the camera) and
- * (world unit normal, 1); a miss gives (sky rgb, 0) and (0, 0, 0, 0).  Enabling it costs
- * 32 B per pixel of writes per frame.  bytes >= width*height*32. */
+/* Optional G-buffer of frames in modes 1 and 2 (denoiser features), the mean over each
+ * pixel's samples of: (albedo rgb linear, distance to the camera) and (world unit normal,
+ * 1) for a hit, (sky rgb, 0) and (0, 0, 0, 0) for a miss -- 2 x float4 per pixel.  Enabling
+ * it adds 32 B per pixel sample of device memory and traffic.  bytes >= width*height*32. */
 int32_t vt_enable_gbuffer(uint32_t on);
 int32_t vt_read_gbuffer(float* dst, uint64_t bytes);
 /* 7 counters of the last frame rendered with stats = 1: rays, LBVH nodes, instances,
@@ -196,21 +196,25 @@ uint64_t vt_dn_param_count(void);
 int32_t vt_dn_set_params(const float* params, uint64_t n);
 int32_t vt_dn_get_params(float* params, uint64_t n);
 /* One forward + backward pass without an update: x = n x h x w x 16 FP32 features
- * (rounded to BF16), target = n x h x w x 3; writes the output (n x h x w x 3), the FP32
- * gradient of the mean relMSE loss for every parameter, and the loss.  h, w % 16 == 0. */
+ * (rounded to BF16), target = n x h x w x 3 radiance; writes the network output
+ * (n x h x w x 3 irradiance), the FP32 gradient of the mean relMSE loss of the remodulated
+ * output for every parameter, and the loss.  h, w % 16 == 0. */
 int32_t vt_dn_grad(const float* x, const float* target, uint32_t n, uint32_t h, uint32_t w, float* y, float* grad,
                    float* loss);
-/* The 16 input features of a frame (radiance RGBA32F + vt_read_gbuffer layout), as FP32
- * values of the BF16 tensor the network reads. */
-int32_t vt_dn_features(const float* rad_rgba, const float* gbuf8, uint32_t h, uint32_t w, float* out16);
-/* Training frames kept in device memory: input radiance + G-buffer, target radiance. */
-int32_t vt_dn_add_frame(const float* rad_rgba, const float* gbuf8, const float* target_rgba, uint32_t h, uint32_t w);
+/* The 16 input features of a frame rendered with spp samples per pixel (radiance RGBA32F +
+ * vt_read_gbuffer layout), as FP32 values of the BF16 tensor the network reads:
+ * log(1 + radiance / (albedo + 0.01)) x3, albedo x3, normal x3, log(1 + distance) / 8,
+ * log2(spp) / 8, zeros.  The network predicts irradiance; output = (albedo + 0.01) x that. */
+int32_t vt_dn_features(const float* rad_rgba, const float* gbuf8, uint32_t h, uint32_t w, uint32_t spp, float* out16);
+/* Training frames kept in device memory: input radiance (spp samples) + G-buffer, target. */
+int32_t vt_dn_add_frame(const float* rad_rgba, const float* gbuf8, const float* target_rgba, uint32_t h, uint32_t w,
+                        uint32_t spp);
 int32_t vt_dn_clear_frames(void);
 /* iters Adam steps on random crop x crop patches (batch per step, seeded); losses[iters]. */
 int32_t vt_dn_train(uint32_t iters, uint32_t batch, uint32_t crop, float lr, uint32_t seed, float* losses);
-/* Denoises an h x w frame (host buffers) into RGBA32F; gpu_ms = device time. */
-int32_t vt_dn_denoise(const float* rad_rgba, const float* gbuf8, uint32_t h, uint32_t w, float* out_rgba,
-                      float* gpu_ms);
+/* Denoises an h x w frame of spp samples (host buffers) into RGBA32F; gpu_ms = device time. */
+int32_t vt_dn_denoise(const float* rad_rgba, const float* gbuf8, uint32_t h, uint32_t w, uint32_t spp,
+                      float* out_rgba, float* gpu_ms);
 /* Denoises the last frame of render_tick (mode 1/2 with the G-buffer enabled) in place on
  * the device and copies the result out. */
 int32_t vt_dn_denoise_frame(float* out_rgba, float* gpu_ms);

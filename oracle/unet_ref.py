@@ -12,7 +12,9 @@ specification, restated here:
   bottleneck: conv(8) + ReLU, conv(9) + ReLU;
 * decoder level l (l = 3..0): nearest 2x upsampling of the coarser output, concatenation
   [upsampled, skip_l] along channels, conv + ReLU, conv + ReLU; output conv(18), linear;
-* loss = mean over pixels and RGB of (y - t)^2 / (t^2 + 0.01).
+* the network predicts a correction y to the input's log irradiance b (input channels 0..2):
+  irradiance E = exp(min(b + y, 16)) - 1, radiance a * E with a = albedo + 0.01 (albedo =
+  input channels 3..5); loss = mean over pixels and RGB of (a E - t)^2 / (t^2 + 0.01).
 
 `bf16=True` rounds to BF16 (round to nearest even) at the points where the GPU stores BF16
 tensors: the input, every stored activation and activation gradient, the weights; the
@@ -23,6 +25,9 @@ convolution (scipy.signal.correlate) in tests/test_denoise_ref.py.
 from __future__ import annotations
 
 import numpy as np
+
+ALBEDO_EPS = 0.01
+LOG_MAX = 16.0
 
 
 def bf16_round(a: np.ndarray) -> np.ndarray:
@@ -160,10 +165,15 @@ class UNetRef:
         return y, c
 
     @staticmethod
-    def loss(y, t):
+    def loss(y, t, x):
+        """x: the (rounded) input features; returns (loss, dLoss/dy)."""
+        b, a = x[..., 0:3], x[..., 3:6] + ALBEDO_EPS
+        z = b + y
+        dE = np.where(z > LOG_MAX, 0.0, np.exp(np.minimum(z, LOG_MAX)))
+        E = np.expm1(np.minimum(z, LOG_MAX))
         den = t * t + 0.01
-        d = y - t
-        return float(np.mean(d * d / den)), 2.0 * d / den / d.size
+        d = a * E - t
+        return float(np.mean(d * d / den)), 2.0 * d / den * a * dE / d.size
 
     def backward(self, params, c, gy, bf16=False):
         R = bf16_round if bf16 else (lambda a: a)
@@ -204,16 +214,27 @@ class UNetRef:
 
     def loss_and_grads(self, params, x, t, bf16=False):
         y, c = self.forward(params, x, bf16)
-        loss, gy = self.loss(y, t)
+        loss, gy = self.loss(y, t, c["x0"])
         return y, loss, self.backward(params, c, gy, bf16)
 
 
-def features(rad_rgba: np.ndarray, gbuf8: np.ndarray) -> np.ndarray:
-    """The 16 input channels of a frame (FP64, before BF16 rounding)."""
+def features(rad_rgba: np.ndarray, gbuf8: np.ndarray, spp: int = 1) -> np.ndarray:
+    """The 16 input channels of a frame of spp samples (FP64, before BF16 rounding):
+    demodulated log irradiance (with the BF16-rounded albedo the network remodulates with),
+    albedo, normal, log distance, log2(spp)."""
     h, w = rad_rgba.shape[:2]
     f = np.zeros((h, w, 16))
-    f[..., 0:3] = np.log1p(np.maximum(rad_rgba[..., :3].astype(np.float64), 0.0))
+    a = bf16_round(gbuf8[..., 0:3]) + ALBEDO_EPS
+    f[..., 0:3] = np.log1p(np.maximum(rad_rgba[..., :3].astype(np.float64), 0.0) / a)
     f[..., 3:6] = gbuf8[..., 0:3]
     f[..., 6:9] = gbuf8[..., 4:7]
     f[..., 9] = np.log1p(np.maximum(gbuf8[..., 3].astype(np.float64), 0.0)) * 0.125
+    f[..., 10] = np.log2(max(spp, 1)) * 0.125
     return f
+
+
+def radiance(y: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """Network output y (log-irradiance correction) -> radiance, remodulated with the albedo
+    of the (rounded) features x."""
+    E = np.expm1(np.minimum(x[..., 0:3] + y, LOG_MAX))
+    return np.maximum((x[..., 3:6] + ALBEDO_EPS) * E, 0.0)

@@ -73,7 +73,7 @@ def test_layer_table_offsets_are_contiguous():
 def test_gradients_match_finite_differences():
     net, params, rng = _tiny()
     x = rng.normal(size=(1, 16, 16, 16))
-    x[..., 10:] = 0.0
+    x[..., 11:] = 0.0
     t = np.abs(rng.normal(size=(1, 16, 16, 3)))
     _, loss, g = net.loss_and_grads(params, x, t)
     # every layer: its first weight, a random weight and its bias
@@ -102,7 +102,7 @@ def test_bf16_mode_stays_close_to_fp64():
     y64, c64 = net.forward(params, x)
     y16, _ = net.forward(params, x, bf16=True)
     assert np.linalg.norm(y16 - y64) / np.linalg.norm(y64) < 3e-2
-    _, gy = net.loss(y64, t)
+    _, gy = net.loss(y64, t, c64["x0"])
     g64 = net.backward(params, c64, gy)
     g16 = net.backward(params, c64, gy, bf16=True)
     for cin, cout, wo, bo in net.layers:
@@ -112,11 +112,35 @@ def test_bf16_mode_stays_close_to_fp64():
 
 def test_features_formula():
     rad = np.zeros((2, 3, 4), dtype=np.float32)
-    rad[0, 0, :3] = [0.0, 1.0, np.e - 1.0]
+    rad[0, 0, :3] = [0.0, 0.5, 1.0]
     rad[0, 1, :3] = [-1.0, 0.5, 2.0]
     gb = np.zeros((2, 3, 8), dtype=np.float32)
-    gb[0, 0] = [0.1, 0.2, 0.3, np.e ** 8 - 1.0, 0.0, 1.0, 0.0, 1.0]
-    f = U.features(rad, gb)
-    assert np.allclose(f[0, 0, :10], [0.0, np.log(2.0), 1.0, 0.1, 0.2, 0.3, 0.0, 1.0, 0.0, 1.0], atol=1e-6)
-    assert f[0, 1, 0] == 0.0 and np.isclose(f[0, 1, 2], np.log(3.0))
-    assert np.all(f[..., 10:] == 0.0)
+    gb[0, 0] = [0.25, 0.5, 0.99, np.e ** 8 - 1.0, 0.0, 1.0, 0.0, 1.0]
+    f = U.features(rad, gb, spp=16)
+    a = np.array([0.25, 0.5, U.bf16_round(np.array([0.99]))[0]]) + 0.01
+    assert np.allclose(f[0, 0, :3], np.log1p(np.array([0.0, 0.5, 1.0]) / a), atol=1e-12)
+    assert np.allclose(f[0, 0, 3:11], [0.25, 0.5, 0.99, 0.0, 1.0, 0.0, 1.0, 0.5], atol=1e-6)
+    assert f[0, 1, 0] == 0.0 and np.isclose(f[0, 1, 2], np.log(1.0 + 2.0 / 0.01))
+    assert np.all(f[..., 11:] == 0.0)
+    # a zero correction reproduces the input radiance (remodulation inverts demodulation)
+    fb = f.copy()
+    fb[..., 3:6] = U.bf16_round(f[..., 3:6])  # the network reads BF16 features
+    assert np.allclose(U.radiance(np.zeros((2, 3, 3)), fb)[0, 0], [0.0, 0.5, 1.0], atol=1e-9)
+
+
+def test_loss_gradient_includes_remodulation():
+    rng = np.random.default_rng(4)
+    y, t = rng.normal(size=(1, 4, 4, 3)), np.abs(rng.normal(size=(1, 4, 4, 3)))
+    x = np.zeros((1, 4, 4, 16))
+    x[..., 0:3] = rng.uniform(0.0, 2.0, size=(1, 4, 4, 3))
+    x[..., 3:6] = rng.uniform(0.0, 1.0, size=(1, 4, 4, 3))
+    loss, gy = U.UNetRef.loss(y, t, x)
+    E = np.expm1(x[..., 0:3] + y)
+    assert np.isclose(loss, np.mean(((x[..., 3:6] + 0.01) * E - t) ** 2 / (t * t + 0.01)))
+    eps = 1e-7
+    for idx in [(0, 0, 0, 0), (0, 3, 2, 1), (0, 1, 3, 2)]:
+        y1, y2 = y.copy(), y.copy()
+        y1[idx] += eps
+        y2[idx] -= eps
+        fd = (U.UNetRef.loss(y1, t, x)[0] - U.UNetRef.loss(y2, t, x)[0]) / (2 * eps)
+        assert abs(fd - gy[idx]) < 1e-6 * max(1.0, abs(fd))

@@ -59,7 +59,7 @@ def test_layer_table_matches_reference(dn):
 def _case(seed):
     rng = np.random.default_rng(seed)
     x = rng.normal(size=(2, 32, 32, 16)).astype(np.float32)
-    x[..., 10:] = 0.0
+    x[..., 11:] = 0.0
     t = np.abs(rng.normal(size=(2, 32, 32, 3))).astype(np.float32)
     return x, t
 
@@ -71,10 +71,12 @@ def _layer_errs(d, g, ref):
 @pytest.mark.parametrize("biases", ["zero", "random"])
 def test_gradients_match_reference(dn, biases):
     """One-ulp BF16 rounding differences caused by summation order cascade through the 19
-    layers, so two correct references (FP64- and FP32-accumulating, both rounding to BF16 where
-    the GPU does) already disagree by up to a few percent on some layers.  The GPU must stay
-    within twice that spread (+0.2 %) on the output, the loss and every layer's gradient, and
-    near the plain FP64 network (output within 3 %, gradient direction)."""
+    layers: two correct references (FP64- and FP32-accumulating, both rounding to BF16 where
+    the GPU does) disagree on some inputs by up to 2.2e-3 on the output and 6e-2 on a layer's
+    gradient, and on others by 1e-7 (no cascade).  The GPU, whose MFMA summation order is a
+    third one, must stay within twice the measured spread or those cascade sizes (5e-3 output,
+    6e-2 per layer), whichever is larger, and near the plain FP64 network (output within 3 %,
+    gradient cosine > 0.99).  A wrong gradient is off by O(1)."""
     _, d = dn
     p = _params(d, 7) if biases == "random" else d.init_params(7)
     d.set_params(p)
@@ -88,14 +90,14 @@ def test_gradients_match_reference(dn, biases):
     ey, errs = _rel(y, ya), _layer_errs(d, g, ga)
     _say(f"y {ey:.2e} (reference spread {spread_y:.2e}); layer gradients max {errs.max():.2e} "
          f"(spread max {spread.max():.2e}), worst ratio {np.max(errs / np.maximum(spread, 1e-3)):.2f}")
-    assert ey < 2 * spread_y + 1e-4 and abs(loss - la) / la < 2 * abs(lb - la) / la + 1e-5
-    assert np.all(errs < 2 * spread + 2e-3), (errs, spread)
+    assert ey < max(2 * spread_y, 5e-3) and abs(loss - la) / la < max(2 * abs(lb - la) / la, 2e-3)
+    assert np.all(errs < np.maximum(2 * spread, 6e-2)), (errs, spread)
     y64, l64, g64 = U.UNetRef(d.layers, DN.ENC, DN.DEC).loss_and_grads(p64, x64, t64)
     cos = float(np.dot(g, g64) / (np.linalg.norm(g) * np.linalg.norm(g64)))
     cin, cout, wo, _ = d.layers[0]
-    assert np.all(g[wo:wo + cout * 9 * cin].reshape(cout, 3, 3, cin)[..., 10:] == 0.0)
+    assert np.all(g[wo:wo + cout * 9 * cin].reshape(cout, 3, 3, cin)[..., 11:] == 0.0)
     _say(f"vs FP64 reference: y {_rel(y, y64):.2e}, loss {abs(loss - l64) / l64:.2e}, gradient cosine {cos:.4f}")
-    assert _rel(y, y64) < 3e-2 and cos > 0.95
+    assert _rel(y, y64) < 3e-2 and cos > 0.99
 
 
 def test_features_kernel(dn):
@@ -104,8 +106,8 @@ def test_features_kernel(dn):
     rad = rng.normal(1.0, 1.0, size=(20, 24, 4)).astype(np.float32)
     gb = rng.uniform(0.0, 1.0, size=(20, 24, 8)).astype(np.float32)
     gb[..., 3] *= 3000.0
-    f = d.features(rad, gb)
-    ref = U.bf16_round(U.features(rad, gb))
+    f = d.features(rad, gb, spp=8)
+    ref = U.bf16_round(U.features(rad, gb, spp=8))
     assert np.all(np.abs(f - ref) <= 2.0 ** -7 * np.abs(ref) + 1e-30)
 
 
@@ -117,12 +119,12 @@ def test_denoise_path_matches_reference_forward(dn):
     h, w = 40, 24   # padded to 48 x 32 by edge replication
     rad = np.abs(rng.normal(0.5, 0.5, size=(h, w, 4))).astype(np.float32)
     gb = rng.uniform(0.0, 1.0, size=(h, w, 8)).astype(np.float32)
-    out, ms = d.denoise(rad, gb)
-    f = U.features(rad, gb)
+    out, ms = d.denoise(rad, gb, 2)
+    f = U.features(rad, gb, 2)
     fp = np.pad(f, ((0, 8), (0, 8), (0, 0)), mode="edge")[None]
     net = U.UNetRef(d.layers, DN.ENC, DN.DEC)
-    y, _ = net.forward(p.astype(np.float64), fp, bf16=True)
-    want = np.maximum(y[0, :h, :w, :], 0.0)
+    y, c = net.forward(p.astype(np.float64), fp, bf16=True)
+    want = U.radiance(y, c["x0"])[0, :h, :w, :]
     _say(f"denoise path vs reference: {_rel(out[..., :3], want):.2e} ({ms:.3f} ms)")
     assert _rel(out[..., :3], want) < 1e-2
     assert np.all(out[..., 3] == 1.0)
@@ -143,7 +145,7 @@ def test_training_beats_the_noisy_input(dn):
         target, _, _ = DN.render_spp(r, sc, 256, seed=7000 + seed)
         for spp in (1, 4):
             inp, gb, _ = DN.render_spp(r, sc, spp, seed=100 * seed + spp, gbuffer=True)
-            d.add_frame(inp, gb, target)
+            d.add_frame(inp, gb, target, spp)
     _say("case: denoiser training 300 iterations, batch 8, crop 64")
     losses = d.train(300, batch=8, crop=64, lr=2e-3, seed=3)
     assert np.all(np.isfinite(losses))
@@ -152,7 +154,7 @@ def test_training_beats_the_noisy_input(dn):
     R.load_scene(r, sc)
     ref, _, _ = DN.render_spp(r, sc, 1024, seed=99)
     noisy, gb, _ = DN.render_spp(r, sc, 1, seed=5, gbuffer=True)
-    den, _ = d.denoise(noisy, gb)
+    den, _ = d.denoise(noisy, gb, 1)
     e_noisy, e_den = DN.relmse(noisy, ref), DN.relmse(den, ref)
     _say(f"training loss {first:.4f} -> {last:.4f}; held-out relMSE noisy {e_noisy:.4f} -> denoised {e_den:.4f}")
     assert last < 0.7 * first
@@ -164,6 +166,6 @@ def test_training_beats_the_noisy_input(dn):
                                      jitter=1, seed=5, sun_dir=prm.sun_dir))
     r.render(sc.P, sc.C)
     a, _ = d.denoise_frame()
-    b, _ = d.denoise(r.radiance(), r.gbuffer())
+    b, _ = d.denoise(r.radiance(), r.gbuffer(), 1)
     r.enable_gbuffer(False)
     assert np.array_equal(a, b)

@@ -50,6 +50,8 @@ struct State {
     float4* rad = nullptr;
     uint32_t* bgra = nullptr;
     float4* gbuf = nullptr;     // optional G-buffer, 2 x float4 per pixel
+    float4* gsamp = nullptr;    // its per-sample state, 2 x float4 per path
+    size_t gsamp_cap = 0;
     bool gbuf_on = false;
     size_t frame_cap = 0;
     void* wave_mem = nullptr;   // wavefront path state + queues
@@ -132,11 +134,33 @@ int ensure_wave_buffers() {
     }
     vt_wave_bind(g.wave, g.wave_mem, (uint32_t)paths);
     g.wave_cap = paths;
+    g.wave.gsamp = g.gbuf_on ? g.gsamp : nullptr;
+    return 0;
+}
+
+// per-sample G-buffer state for the current wave capacity (only while the G-buffer is on)
+int ensure_gsamp() {
+    if (!g.gbuf_on || g.params.mode == 0) {
+        g.wave.gsamp = nullptr;
+        return 0;
+    }
+    if (g.gsamp_cap < g.wave_cap || !g.gsamp) {
+        hipFree(g.gsamp);
+        g.gsamp = nullptr;
+        g.gsamp_cap = 0;
+        hipError_t e = hipMalloc(&g.gsamp, g.wave_cap * 2 * sizeof(float4));
+        if (e != hipSuccess) {
+            set_err("G-buffer sample allocation", e);
+            return -1;
+        }
+        g.gsamp_cap = g.wave_cap;
+    }
+    g.wave.gsamp = g.gsamp;
     return 0;
 }
 
 int ensure_frame_buffers() {
-    if (ensure_wave_buffers() != 0) return -1;
+    if (ensure_wave_buffers() != 0 || ensure_gsamp() != 0) return -1;
     if (!g.wave_mem) {  // mode 0 still needs the counters block
         if (hipMalloc(&g.wave_mem, vt_wave_bytes(1)) != hipSuccess) return -1;
         vt_wave_bind(g.wave, g.wave_mem, 1);
@@ -342,12 +366,13 @@ hipStream_t vt_rt_stream() { return g.stream; }
 const VolumeDesc* vt_rt_vol_table() { return g.vol_table; }
 uint32_t vt_rt_nvol() { return (uint32_t)g.vols.size(); }
 void vt_rt_set_error(const char* what) { set_err(what); }
-bool vt_rt_frame(const float4** rad, const float4** gbuf, uint32_t* width, uint32_t* height) {
+bool vt_rt_frame(const float4** rad, const float4** gbuf, uint32_t* width, uint32_t* height, uint32_t* spp) {
     if (!g.inited || !g.have_frame || g.last_frame.mode == 0 || !g.gbuf_on || !g.gbuf) return false;
     *rad = reinterpret_cast<const float4*>(g.rad);
     *gbuf = g.gbuf;
     *width = g.last_frame.width;
     *height = g.last_frame.height;
+    *spp = g.last_frame.spp;
     return true;
 }
 
@@ -481,7 +506,7 @@ void cleanup(void) {
     vt_lbvh_free(g.bvh);
     hipHostFree(g.inst_host);
     hipFree(g.inst_dev); hipFree(g.vol_table); hipFree(g.staging); hipFree(g.remap);
-    hipFree(g.hits); hipFree(g.rad); hipFree(g.bgra); hipFree(g.gbuf); hipFree(g.stats); hipFree(g.srgb_dec); hipFree(g.srgb_thr);
+    hipFree(g.hits); hipFree(g.rad); hipFree(g.bgra); hipFree(g.gbuf); hipFree(g.gsamp); hipFree(g.stats); hipFree(g.srgb_dec); hipFree(g.srgb_thr);
     hipFree(g.wave_mem);
     hipEventDestroy(g.ev0); hipEventDestroy(g.ev1); hipEventDestroy(g.inst_copied);
     hipStreamDestroy(g.stream);

@@ -44,6 +44,8 @@ constexpr int kLayers = 19;
 constexpr int kMaxWidth = 96;   // widest layer (LDS tile of the weight-gradient kernel)
 constexpr uint32_t kConvPix = 128;
 constexpr uint32_t kPitch = 40;  // LDS row pitch (elements) of 32-pixel transposed tiles
+constexpr float kAlbedoEps = 0.01f;  // demodulation: irradiance = radiance / (albedo + eps)
+constexpr float kLogMax = 16.0f;     // clamp of the predicted log irradiance
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
     uint32_t u = __float_as_uint(f);
@@ -319,16 +321,20 @@ __global__ void k_up_bwd(const uint16_t* __restrict__ gcat, uint32_t stride, uin
 }
 
 // features of an h x w frame into an hp x wp tensor (edge-replicated padding), 16 channels:
-// log(1 + max(rad, 0)) x3, albedo x3, normal x3, log(1 + distance) / 8, zeros x6
+// log(1 + max(rad, 0) / (albedo + 0.01)) x3 (demodulated irradiance), albedo x3, normal x3,
+// log(1 + distance) / 8, log2(spp) / 8, zeros x5
 __global__ void k_features(const float4* __restrict__ rad, const float4* __restrict__ gbuf, uint32_t h, uint32_t w,
-                           uint32_t hp, uint32_t wp, uint16_t* __restrict__ out) {
+                           uint32_t hp, uint32_t wp, float spp_feature, uint16_t* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= hp * wp) return;
     const uint32_t x = min(i % wp, w - 1u), y = min(i / wp, h - 1u);
     const uint32_t s = y * w + x;
     const float4 L = rad[s], A = gbuf[2 * s], Nn = gbuf[2 * s + 1];
-    float f[16] = {log1pf(fmaxf(L.x, 0.0f)), log1pf(fmaxf(L.y, 0.0f)), log1pf(fmaxf(L.z, 0.0f)), A.x, A.y, A.z,
-                   Nn.x, Nn.y, Nn.z, log1pf(fmaxf(A.w, 0.0f)) * 0.125f, 0, 0, 0, 0, 0, 0};
+    // the albedo the network sees (and remodulates with) is the BF16-rounded one
+    const float ar = bf2f(f2bf(A.x)) + kAlbedoEps, ag = bf2f(f2bf(A.y)) + kAlbedoEps, ab = bf2f(f2bf(A.z)) + kAlbedoEps;
+    float f[16] = {log1pf(fmaxf(L.x, 0.0f) / ar), log1pf(fmaxf(L.y, 0.0f) / ag), log1pf(fmaxf(L.z, 0.0f) / ab),
+                   A.x, A.y, A.z, Nn.x, Nn.y, Nn.z, log1pf(fmaxf(A.w, 0.0f)) * 0.125f, spp_feature,
+                   0, 0, 0, 0, 0};
     u32x4 o0, o1;
     for (int t = 0; t < 4; ++t) {
         o0[t] = (uint32_t)f2bf(f[2 * t]) | ((uint32_t)f2bf(f[2 * t + 1]) << 16);
@@ -376,10 +382,23 @@ __global__ void k_rgba_to_rgb(const float4* __restrict__ in, float* __restrict__
     out[3 * i + 2] = v.z;
 }
 
-// relMSE loss mean((y - t)^2 / (t^2 + 0.01)) over P pixels x 3 channels; gy = dLoss/dy (BF16,
-// stride 8, channels 3..7 zero); per-block partial sums of the loss.
-__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y, const float* __restrict__ t, uint32_t P,
-                                              float scale, uint16_t* __restrict__ gy, float* __restrict__ part) {
+// The network predicts a correction y to the input's log irradiance b (feature channels 0..2
+// of x): irradiance E = exp(min(b + y, kLogMax)) - 1, radiance a * E with a = albedo + eps
+// (feature channels 3..5).  relMSE loss mean((a E - t)^2 / (t^2 + 0.01)) over P pixels x 3
+// channels; gy = dLoss/dy (BF16, stride 8, channels 3..7 zero); per-block partial sums.
+__device__ __forceinline__ float irradiance(float b, float y, float* dE) {
+    const float z = b + y;
+    if (z > kLogMax) {
+        *dE = 0.0f;
+        return expm1f(kLogMax);
+    }
+    *dE = expf(z);
+    return expm1f(z);
+}
+
+__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y, const float* __restrict__ t,
+                                              const uint16_t* __restrict__ x, uint32_t P, float scale,
+                                              uint16_t* __restrict__ gy, float* __restrict__ part) {
     __shared__ float red[256];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     float l = 0.0f;
@@ -387,10 +406,14 @@ __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y, const
         u32x4 o = {0u, 0u, 0u, 0u};
         uint32_t h[3];
         for (int c = 0; c < 3; ++c) {
-            const float d = y[3 * i + c] - t[3 * i + c];
+            const float b = bf2f(x[(size_t)i * 16u + (uint32_t)c]);
+            const float a = bf2f(x[(size_t)i * 16u + 3u + (uint32_t)c]) + kAlbedoEps;
+            float dE;
+            const float E = irradiance(b, y[3 * i + c], &dE);
+            const float d = a * E - t[3 * i + c];
             const float den = t[3 * i + c] * t[3 * i + c] + 0.01f;
             l += d * d / den;
-            h[c] = f2bf(2.0f * d / den * scale);
+            h[c] = f2bf(2.0f * d / den * a * dE * scale);
         }
         o[0] = h[0] | (h[1] << 16);
         o[1] = h[2];
@@ -458,13 +481,22 @@ __global__ void k_prep_weights(const float* __restrict__ params, const LayerDev*
     wt[d.wt_off + (ci * 9u + (8u - tap)) * d.cout8 + co] = b;
 }
 
-// network output (hp x wp x 3, FP32) -> h x w RGBA32F radiance, clamped at 0
-__global__ void k_output(const float* __restrict__ y, uint32_t h, uint32_t w, uint32_t wp, float4* __restrict__ out) {
+// network output (hp x wp x 3 log-irradiance corrections, FP32) -> h x w RGBA32F radiance
+// (irradiance remodulated with the albedo of the features x, clamped at 0)
+__global__ void k_output(const float* __restrict__ y, const uint16_t* __restrict__ x, uint32_t h, uint32_t w,
+                         uint32_t wp, float4* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= h * w) return;
-    const uint32_t x = i % w, yy = i / w;
-    const float* v = y + ((size_t)yy * wp + x) * 3u;
-    out[i] = make_float4(fmaxf(v[0], 0.0f), fmaxf(v[1], 0.0f), fmaxf(v[2], 0.0f), 1.0f);
+    const uint32_t xx = i % w, yy = i / w;
+    const size_t q = (size_t)yy * wp + xx;
+    const float* v = y + q * 3u;
+    const uint16_t* f = x + q * 16u;
+    float r[3];
+    for (int c = 0; c < 3; ++c) {
+        float dE;
+        r[c] = fmaxf((bf2f(f[3 + c]) + kAlbedoEps) * irradiance(bf2f(f[c]), v[c], &dE), 0.0f);
+    }
+    out[i] = make_float4(r[0], r[1], r[2], 1.0f);
 }
 
 // ---- host side ---------------------------------------------------------------------------
@@ -782,8 +814,8 @@ int backward(uint32_t N, uint32_t H, uint32_t W, float* loss_dev, hipStream_t s)
     const size_t P0 = (size_t)N * H * W;
     const float scale = 1.0f / (float)(3 * P0);
     const uint32_t lb = blocks(P0);
-    hipLaunchKernelGGL(k_loss, dim3(lb), dim3(256), 0, s, net.y, net.target, (uint32_t)P0, scale, net.gy.p,
-                       net.lpart);
+    hipLaunchKernelGGL(k_loss, dim3(lb), dim3(256), 0, s, net.y, net.target, net.x0.p, (uint32_t)P0, scale,
+                       net.gy.p, net.lpart);
     hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(256), 0, s, net.lpart, lb, scale, loss_dev);
     int rc = 0;
     // output layer
@@ -853,14 +885,17 @@ uint32_t pcg(uint32_t v) {
 }
 
 // denoise an h x w frame whose radiance and G-buffer are in device memory; out_dev RGBA32F
-int denoise_device(const float4* rad, const float4* gbuf, uint32_t h, uint32_t w, float4* out_dev, float* ms,
-                   hipStream_t s) {
+float spp_feature(uint32_t spp) { return log2f((float)(spp ? spp : 1u)) * 0.125f; }
+
+int denoise_device(const float4* rad, const float4* gbuf, uint32_t h, uint32_t w, uint32_t spp, float4* out_dev,
+                   float* ms, hipStream_t s) {
     const uint32_t hp = (h + 15u) & ~15u, wp = (w + 15u) & ~15u;
     if (ensure_act((size_t)hp * wp) != 0) return -1;
     hipEventRecord(net.e0, s);
-    hipLaunchKernelGGL(k_features, dim3(blocks((size_t)hp * wp)), dim3(256), 0, s, rad, gbuf, h, w, hp, wp, net.x0.p);
+    hipLaunchKernelGGL(k_features, dim3(blocks((size_t)hp * wp)), dim3(256), 0, s, rad, gbuf, h, w, hp, wp,
+                       spp_feature(spp), net.x0.p);
     if (forward(1, hp, wp, s) != 0) return -1;
-    hipLaunchKernelGGL(k_output, dim3(blocks((size_t)h * w)), dim3(256), 0, s, net.y, h, w, wp, out_dev);
+    hipLaunchKernelGGL(k_output, dim3(blocks((size_t)h * w)), dim3(256), 0, s, net.y, net.x0.p, h, w, wp, out_dev);
     hipEventRecord(net.e1, s);
     if (hipStreamSynchronize(s) != hipSuccess) return -1;
     if (ms) hipEventElapsedTime(ms, net.e0, net.e1);
@@ -1022,7 +1057,7 @@ int32_t vt_dn_grad(const float* x, const float* target, uint32_t n, uint32_t h, 
     return hipStreamSynchronize(s) == hipSuccess ? 0 : -1;
 }
 
-int32_t vt_dn_features(const float* rad_rgba, const float* gbuf8, uint32_t h, uint32_t w, float* out16) {
+int32_t vt_dn_features(const float* rad_rgba, const float* gbuf8, uint32_t h, uint32_t w, uint32_t spp, float* out16) {
     std::lock_guard<std::mutex> lk(vt_rt_lock());
     if (!net.on || !rad_rgba || !gbuf8 || !out16 || h == 0 || w == 0) return -1;
     hipSetDevice(vt_rt_device());
@@ -1035,7 +1070,7 @@ int32_t vt_dn_features(const float* rad_rgba, const float* gbuf8, uint32_t h, ui
     uint16_t* f = reinterpret_cast<uint16_t*>(io + npx * 48);
     hipMemcpyAsync(rad, rad_rgba, npx * 16, hipMemcpyHostToDevice, s);
     hipMemcpyAsync(gb, gbuf8, npx * 32, hipMemcpyHostToDevice, s);
-    hipLaunchKernelGGL(k_features, dim3(blocks(npx)), dim3(256), 0, s, rad, gb, h, w, h, w, f);
+    hipLaunchKernelGGL(k_features, dim3(blocks(npx)), dim3(256), 0, s, rad, gb, h, w, h, w, spp_feature(spp), f);
     std::vector<uint16_t> hb(npx * 16);
     hipMemcpyAsync(hb.data(), f, npx * 32, hipMemcpyDeviceToHost, s);
     if (hipStreamSynchronize(s) != hipSuccess) return -1;
@@ -1046,7 +1081,8 @@ int32_t vt_dn_features(const float* rad_rgba, const float* gbuf8, uint32_t h, ui
     return 0;
 }
 
-int32_t vt_dn_add_frame(const float* rad_rgba, const float* gbuf8, const float* target_rgba, uint32_t h, uint32_t w) {
+int32_t vt_dn_add_frame(const float* rad_rgba, const float* gbuf8, const float* target_rgba, uint32_t h, uint32_t w,
+                        uint32_t spp) {
     std::lock_guard<std::mutex> lk(vt_rt_lock());
     if (!net.on || !rad_rgba || !gbuf8 || !target_rgba || h < 16 || w < 16) return -1;
     hipSetDevice(vt_rt_device());
@@ -1069,7 +1105,7 @@ int32_t vt_dn_add_frame(const float* rad_rgba, const float* gbuf8, const float* 
     fd.w = w;
     hipMemcpyAsync(rad, rad_rgba, npx * 16, hipMemcpyHostToDevice, s);
     hipMemcpyAsync(gb, gbuf8, npx * 32, hipMemcpyHostToDevice, s);
-    hipLaunchKernelGGL(k_features, dim3(blocks(npx)), dim3(256), 0, s, rad, gb, h, w, h, w,
+    hipLaunchKernelGGL(k_features, dim3(blocks(npx)), dim3(256), 0, s, rad, gb, h, w, h, w, spp_feature(spp),
                        static_cast<uint16_t*>(mem));
     hipMemcpyAsync(rad, target_rgba, npx * 16, hipMemcpyHostToDevice, s);
     hipLaunchKernelGGL(k_rgba_to_rgb, dim3(blocks(npx)), dim3(256), 0, s, rad, const_cast<float*>(fd.target),
@@ -1166,8 +1202,8 @@ int32_t vt_dn_train(uint32_t iters, uint32_t batch, uint32_t crop, float lr, uin
     return hipStreamSynchronize(s) == hipSuccess ? 0 : -1;
 }
 
-int32_t vt_dn_denoise(const float* rad_rgba, const float* gbuf8, uint32_t h, uint32_t w, float* out_rgba,
-                      float* gpu_ms) {
+int32_t vt_dn_denoise(const float* rad_rgba, const float* gbuf8, uint32_t h, uint32_t w, uint32_t spp,
+                      float* out_rgba, float* gpu_ms) {
     std::lock_guard<std::mutex> lk(vt_rt_lock());
     if (!net.on || !rad_rgba || !gbuf8 || !out_rgba || h == 0 || w == 0) return -1;
     hipSetDevice(vt_rt_device());
@@ -1180,7 +1216,7 @@ int32_t vt_dn_denoise(const float* rad_rgba, const float* gbuf8, uint32_t h, uin
     float4* out = reinterpret_cast<float4*>(io + npx * 48);
     hipMemcpyAsync(rad, rad_rgba, npx * 16, hipMemcpyHostToDevice, s);
     hipMemcpyAsync(gb, gbuf8, npx * 32, hipMemcpyHostToDevice, s);
-    if (denoise_device(rad, gb, h, w, out, gpu_ms, s) != 0) {
+    if (denoise_device(rad, gb, h, w, spp, out, gpu_ms, s) != 0) {
         vt_rt_set_error("vt_dn_denoise: failed");
         return -1;
     }
@@ -1191,8 +1227,8 @@ int32_t vt_dn_denoise(const float* rad_rgba, const float* gbuf8, uint32_t h, uin
 int32_t vt_dn_denoise_frame(float* out_rgba, float* gpu_ms) {
     std::lock_guard<std::mutex> lk(vt_rt_lock());
     const float4 *rad = nullptr, *gb = nullptr;
-    uint32_t w = 0, h = 0;
-    if (!net.on || !out_rgba || !vt_rt_frame(&rad, &gb, &w, &h)) {
+    uint32_t w = 0, h = 0, spp = 1;
+    if (!net.on || !out_rgba || !vt_rt_frame(&rad, &gb, &w, &h, &spp)) {
         vt_rt_set_error("vt_dn_denoise_frame: no network or no frame with a G-buffer");
         return -1;
     }
@@ -1201,7 +1237,7 @@ int32_t vt_dn_denoise_frame(float* out_rgba, float* gpu_ms) {
     const size_t npx = (size_t)h * w;
     if (ensure_io(npx * 16) != 0) return -1;
     float4* out = static_cast<float4*>(net.io);
-    if (denoise_device(rad, gb, h, w, out, gpu_ms, s) != 0) {
+    if (denoise_device(rad, gb, h, w, spp, out, gpu_ms, s) != 0) {
         vt_rt_set_error("vt_dn_denoise_frame: failed");
         return -1;
     }

@@ -752,6 +752,7 @@ struct WF {
     float4* contrib[2];   // per slot: direct-light contribution if the sun ray is clear (w = 1: trace it)
     float4* dir[2];       // per slot: bounce direction (w = 1: trace it)
     uint32_t* counters;   // [0,1] queue sizes, [2..] fetch counters
+    float4* gsamp;        // optional: per path (albedo rgb, distance), (normal, hit)
 };
 
 // Shade a surface at `depth`: direct-light contribution for the sun ray and, if the path
@@ -871,10 +872,10 @@ __global__ __launch_bounds__(kBlock) void k_primary(const ReqGPU* __restrict__ r
                 rec.inst = VT_EMPTY; rec.voxel = VT_EMPTY; rec.steps = 0; rec.face = 0xFF;
                 rec.pal = 0; rec.mask = 0; rec.flags = 0; rec.pad = 0;
                 Rq.hits[pix] = rec;
-                if (Rq.gbuf) {
-                    Rq.gbuf[2 * pix] = make_float4(F.sky[0], F.sky[1], F.sky[2], 0.0f);
-                    Rq.gbuf[2 * pix + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                }
+            }
+            if (Rq.gbuf && wf.gsamp) {
+                wf.gsamp[2 * (size_t)w] = make_float4(F.sky[0], F.sky[1], F.sky[2], 0.0f);
+                wf.gsamp[2 * (size_t)w + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             }
         } else {
             ctr.hit();
@@ -895,11 +896,12 @@ __global__ __launch_bounds__(kBlock) void k_primary(const ReqGPU* __restrict__ r
             }
             surface_of(S, h, sf);
             sf.path = w;
-            if (smp == 0 && Rq.gbuf) {
+            if (Rq.gbuf && wf.gsamp) {
                 const float ex = sf.X[0] - F.O[0], ey = sf.X[1] - F.O[1], ez = sf.X[2] - F.O[2];
-                Rq.gbuf[2 * pix] = make_float4(srgb_dec[sf.rgba & 255u], srgb_dec[(sf.rgba >> 8) & 255u],
-                                               srgb_dec[(sf.rgba >> 16) & 255u], sqrtf((ex * ex + ey * ey) + ez * ez));
-                Rq.gbuf[2 * pix + 1] = make_float4(sf.n[0], sf.n[1], sf.n[2], 1.0f);
+                wf.gsamp[2 * (size_t)w] = make_float4(srgb_dec[sf.rgba & 255u], srgb_dec[(sf.rgba >> 8) & 255u],
+                                                      srgb_dec[(sf.rgba >> 16) & 255u],
+                                                      sqrtf((ex * ex + ey * ey) + ez * ez));
+                wf.gsamp[2 * (size_t)w + 1] = make_float4(sf.n[0], sf.n[1], sf.n[2], 1.0f);
             }
             float4 contrib, dirw;
             bool no_sun;
@@ -1022,6 +1024,17 @@ __device__ void accumulate_pixel(const ReqGPU* __restrict__ reqs, const BatchGPU
     if (Rq.final_chunk) {
         const float fs = (float)(Rq.F.spp ? Rq.F.spp : 1u);
         Rq.rad[p] = make_float4(acc[0] / fs, acc[1] / fs, acc[2] / fs, 1.0f);
+        if (Rq.gbuf && wf.gsamp && Rq.s0 == 0) {  // G-buffer: mean over the samples, in sample order
+            float g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (uint32_t s = 0; s < Rq.s1; ++s) {
+                const size_t q = 2 * (size_t)(wb + tm.index(px, py, s, 0u));
+                const float4 a = wf.gsamp[q], b = wf.gsamp[q + 1];
+                g[0] += a.x; g[1] += a.y; g[2] += a.z; g[3] += a.w;
+                g[4] += b.x; g[5] += b.y; g[6] += b.z; g[7] += b.w;
+            }
+            Rq.gbuf[2 * p] = make_float4(g[0] / fs, g[1] / fs, g[2] / fs, g[3] / fs);
+            Rq.gbuf[2 * p + 1] = make_float4(g[4] / fs, g[5] / fs, g[6] / fs, g[7] / fs);
+        }
     } else {
         Rq.acc[p] = make_float4(acc[0], acc[1], acc[2], 0.0f);
     }
@@ -1167,6 +1180,7 @@ static WF to_wf(const WaveBuffers& wb) {
         w.dir[q] = reinterpret_cast<float4*>(wb.dir[q]);
     }
     w.counters = wb.counters;
+    w.gsamp = reinterpret_cast<float4*>(wb.gsamp);
     return w;
 }
 

@@ -60,6 +60,8 @@ struct WaveBuffers {
     void* dir[2] = {nullptr, nullptr};      // float4 per slot
     uint32_t* counters = nullptr;           // 64 u32: queue sizes and fetch counters
     void* tables = nullptr;                 // ReqGPU[VT_MAX_BATCH] then BatchGPU
+    void* gsamp = nullptr;                  // optional, owned by the caller: 2 x float4 per path
+                                            // (G-buffer of each sample, averaged per pixel)
 };
 size_t vt_wave_bytes(uint32_t npaths);
 int vt_wave_bind(WaveBuffers& wb, void* mem, uint32_t npaths);
@@ -91,8 +93,9 @@ hipStream_t vt_rt_stream();
 const VolumeDesc* vt_rt_vol_table();
 uint32_t vt_rt_nvol();
 void vt_rt_set_error(const char* what);
-// last plain-path frame's radiance and G-buffer (device); false if there is none (caller holds the lock)
-bool vt_rt_frame(const float4** rad, const float4** gbuf, uint32_t* width, uint32_t* height);
+// last plain-path frame's radiance, G-buffer (device), size and spp; false if there is none
+// (caller holds the lock)
+bool vt_rt_frame(const float4** rad, const float4** gbuf, uint32_t* width, uint32_t* height, uint32_t* spp);
 // frame constants from the projection / camera matrices and render parameters
 bool vt_build_frame(const float* P, const float* C, const vt_render_params& p, FrameGPU& F);
 // serve.hip: frees the serving loop (caller holds the runtime lock)

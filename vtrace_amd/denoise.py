@@ -18,7 +18,7 @@ from .renderer import RendererError
 
 ENC = (32, 32, 48, 64, 96)
 DEC = (64, 48, 32, 32)
-N_FEATURES = 10   # of the 16 input channels; the rest are zero padding
+N_FEATURES = 11   # of the 16 input channels; the rest are zero padding
 
 
 class VtDnConfig(C.Structure):
@@ -42,15 +42,15 @@ def declare(lib):
     lib.vt_dn_grad.restype = C.c_int32
     lib.vt_dn_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, f]
     lib.vt_dn_features.restype = C.c_int32
-    lib.vt_dn_features.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+    lib.vt_dn_features.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
     lib.vt_dn_add_frame.restype = C.c_int32
-    lib.vt_dn_add_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]
+    lib.vt_dn_add_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]
     lib.vt_dn_clear_frames.restype = C.c_int32
     lib.vt_dn_clear_frames.argtypes = []
     lib.vt_dn_train.restype = C.c_int32
     lib.vt_dn_train.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, C.c_void_p]
     lib.vt_dn_denoise.restype = C.c_int32
-    lib.vt_dn_denoise.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, f]
+    lib.vt_dn_denoise.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, f]
     lib.vt_dn_denoise_frame.restype = C.c_int32
     lib.vt_dn_denoise_frame.argtypes = [C.c_void_p, f]
     lib.vt_dn_shutdown.restype = None
@@ -124,21 +124,21 @@ class Denoiser:
             raise RendererError(f"vt_dn_grad failed: {self.r.error()}")
         return y, g, float(loss.value)
 
-    def features(self, rad: np.ndarray, gbuf: np.ndarray) -> np.ndarray:
+    def features(self, rad: np.ndarray, gbuf: np.ndarray, spp: int = 1) -> np.ndarray:
         rad = np.ascontiguousarray(rad, dtype=np.float32)
         gbuf = np.ascontiguousarray(gbuf, dtype=np.float32)
         h, w = rad.shape[:2]
         out = np.zeros((h, w, 16), dtype=np.float32)
-        if self.lib.vt_dn_features(rad.ctypes.data, gbuf.ctypes.data, h, w, out.ctypes.data) != 0:
+        if self.lib.vt_dn_features(rad.ctypes.data, gbuf.ctypes.data, h, w, spp, out.ctypes.data) != 0:
             raise RendererError("vt_dn_features failed")
         return out
 
-    def add_frame(self, rad: np.ndarray, gbuf: np.ndarray, target: np.ndarray) -> int:
+    def add_frame(self, rad: np.ndarray, gbuf: np.ndarray, target: np.ndarray, spp: int) -> int:
         rad = np.ascontiguousarray(rad, dtype=np.float32)
         gbuf = np.ascontiguousarray(gbuf, dtype=np.float32)
         target = np.ascontiguousarray(target, dtype=np.float32)
         h, w = rad.shape[:2]
-        k = self.lib.vt_dn_add_frame(rad.ctypes.data, gbuf.ctypes.data, target.ctypes.data, h, w)
+        k = self.lib.vt_dn_add_frame(rad.ctypes.data, gbuf.ctypes.data, target.ctypes.data, h, w, spp)
         if k < 0:
             raise RendererError(f"vt_dn_add_frame failed: {self.r.error()}")
         return k
@@ -152,13 +152,13 @@ class Denoiser:
             raise RendererError(f"vt_dn_train failed: {self.r.error()}")
         return losses
 
-    def denoise(self, rad: np.ndarray, gbuf: np.ndarray):
+    def denoise(self, rad: np.ndarray, gbuf: np.ndarray, spp: int):
         rad = np.ascontiguousarray(rad, dtype=np.float32)
         gbuf = np.ascontiguousarray(gbuf, dtype=np.float32)
         h, w = rad.shape[:2]
         out = np.zeros((h, w, 4), dtype=np.float32)
         ms = C.c_float(0.0)
-        if self.lib.vt_dn_denoise(rad.ctypes.data, gbuf.ctypes.data, h, w, out.ctypes.data, C.byref(ms)) != 0:
+        if self.lib.vt_dn_denoise(rad.ctypes.data, gbuf.ctypes.data, h, w, spp, out.ctypes.data, C.byref(ms)) != 0:
             raise RendererError(f"vt_dn_denoise failed: {self.r.error()}")
         return out, float(ms.value)
 
@@ -207,3 +207,80 @@ def time_to_quality(curve: list, threshold: float):
         if e <= threshold:
             return t
     return None
+
+
+# ---- benchmark ------------------------------------------------------------------------------
+def benchmark(r, eval_scene: "scenes.Scene", train_seeds=(101, 102, 103, 104, 105, 106), train_spp=(1, 4, 16, 64),
+              target_spp: int = 1024, judge_spp: int = 4096, iters: int = 2000, batch: int = 8, crop: int = 128,
+              lr: float = 2e-3, threshold: float = 1e-3, log=None) -> dict:
+    """Trains on C2-style scenes (other seeds than the evaluation scene), then measures the GPU
+    time each method needs to reach relMSE <= threshold against a judge_spp render of
+    eval_scene: the plain path (spp doubling from 1) and render + denoise (spp doubling
+    from 1).  Times are device times (render kernels + denoiser kernels)."""
+    say = log or (lambda *a: None)
+    p = eval_scene.params
+    d = Denoiser(r, seed=0)
+    t0 = time.perf_counter()
+    for s in train_seeds:
+        sc = scenes.multi_volume_scene(16, 256, 2048, s, p.width, p.height, 1, p.max_bounces)
+        from .renderer import load_scene
+        load_scene(r, sc)
+        del sc.volumes[:]
+        target, _, _ = render_spp(r, sc, target_spp, seed=90000 + s)
+        for spp in train_spp:
+            inp, gb, _ = render_spp(r, sc, spp, seed=1000 * s + spp, gbuffer=True)
+            d.add_frame(inp, gb, target, spp)
+    t_data = time.perf_counter() - t0
+    say(f"denoiser: {len(train_seeds) * len(train_spp)} training frames in {t_data:.1f} s")
+    t0 = time.perf_counter()
+    losses = d.train(iters, batch=batch, crop=crop, lr=lr, seed=7)
+    t_train = time.perf_counter() - t0
+    d.clear_frames()
+    say(f"denoiser: {iters} iterations in {t_train:.1f} s, loss {losses[:50].mean():.4f} -> {losses[-50:].mean():.4f}")
+    from .renderer import load_scene
+    load_scene(r, eval_scene)
+    judge, _, judge_ms = render_spp(r, eval_scene, judge_spp, seed=777)
+    plain, den = [], []
+    spp = 1
+    while spp <= target_spp:
+        rad, gb, ms = render_spp(r, eval_scene, spp, seed=4242, gbuffer=True)
+        e = relmse(rad, judge)
+        plain.append({"spp": spp, "ms": round(ms, 3), "relmse": e})
+        if spp <= 256:
+            out, dms = d.denoise(rad, gb, spp)
+            den.append({"spp": spp, "ms": round(ms + dms, 3), "render_ms": round(ms, 3), "denoise_ms": round(dms, 4),
+                        "relmse": relmse(out, judge)})
+        say(f"denoiser: spp {spp}: plain {e:.3e}" + (f", denoised {den[-1]['relmse']:.3e}" if spp <= 256 else ""))
+        if e <= threshold and den and den[-1]["relmse"] <= threshold:
+            break
+        spp *= 2
+
+    def first(curve):
+        for c in curve:
+            if c["relmse"] <= threshold:
+                return c
+        return None
+
+    fp, fd = first(plain), first(den)
+    out = {"judge_spp": judge_spp, "judge_ms": round(judge_ms, 1), "threshold": threshold,
+           "params": d.real_param_count, "train": {"frames": len(train_seeds) * len(train_spp), "iters": iters,
+                                                   "batch": batch, "crop": crop, "seconds": round(t_train, 2),
+                                                   "data_seconds": round(t_data, 1),
+                                                   "loss_first50": float(losses[:50].mean()),
+                                                   "loss_last50": float(losses[-50:].mean())},
+           "plain_curve": plain, "denoised_curve": den}
+    if fp is None and plain:
+        # Monte Carlo error falls as 1/spp: extrapolate the plain path from its last point
+        last = plain[-1]
+        k = last["relmse"] / threshold
+        fp = {"spp": round(last["spp"] * k, 1), "ms": round(last["ms"] * k, 3), "relmse": threshold,
+              "extrapolated": True}
+    out["plain_time_to_threshold_ms"] = fp["ms"] if fp else None
+    out["plain_spp_at_threshold"] = fp["spp"] if fp else None
+    out["denoised_time_to_threshold_ms"] = fd["ms"] if fd else None
+    out["denoised_spp_at_threshold"] = fd["spp"] if fd else None
+    out["denoised_best_relmse"] = min(c["relmse"] for c in den) if den else None
+    if fp and fd:
+        out["speedup"] = round(fp["ms"] / fd["ms"], 2)
+    d.close()
+    return out
