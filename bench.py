@@ -27,7 +27,7 @@ serving: a seeded Poisson stream of requests (resolution from {256x256, 640x480,
 loop's measured capacity; p50/p99 latency (arrival -> radiance in host memory) for the
 continuous-batching loop and for the plain path serving the same arrivals one at a time.
 
-denoiser: a 5-level BF16 conv U-Net trained on the GPU (own kernels) on (1/4/16 spp, 1024 spp)
+denoiser: a 5-level BF16 conv U-Net trained on the GPU (own kernels) on (1..256 spp, 1024 spp)
 pairs of other seeded scenes of the same kind; device time to reach relMSE <= 1e-3 against a
 4096-spp render of the benchmark frame, denoised vs plain.
 
@@ -120,7 +120,7 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--serving-requests", type=int, default=2000,
                     help="requests in the serving stream (0 skips the serving measurement)")
-    ap.add_argument("--denoiser-iters", type=int, default=2000,
+    ap.add_argument("--denoiser-iters", type=int, default=8000,
                     help="training iterations of the denoiser (0 skips the denoiser measurement)")
     args = ap.parse_args()
     if args.gpus != 1:

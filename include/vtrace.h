@@ -140,7 +140,17 @@ typedef struct vt_serve_config {
     uint32_t tile_w;            /* pixel tile of 64 samples: tile_w x (64 / tile_w); 0 = 8 */
     uint32_t max_bounces;       /* bounce stages per iteration (largest request max_bounces) */
     uint32_t use_graph;         /* 1: replay the iteration's launch sequence from a hipGraph */
+    uint32_t policy;            /* VT_SERVE_ROUND_ROBIN, VT_SERVE_HEAD_RIDERS, VT_SERVE_SHORTEST_FIRST */
+    uint32_t pipeline;          /* iterations in flight: 1, or 2 (queue the next before waiting) */
 } vt_serve_config;
+
+/* round robin: every admitted request traces its next spp_chunk samples per iteration;
+ * head + riders: the oldest request takes the work budget, requests whose whole remainder
+ * fits a quarter of it ride along and finish in the same iteration; shortest first: the
+ * budget goes to the requests with the least remaining work. */
+#define VT_SERVE_ROUND_ROBIN 0u
+#define VT_SERVE_HEAD_RIDERS 1u
+#define VT_SERVE_SHORTEST_FIRST 2u
 
 typedef struct vt_request {
     uint32_t scene;             /* id from vt_serve_add_scene */
@@ -169,8 +179,9 @@ int32_t vt_serve_init(const vt_serve_config* config);
 int32_t vt_serve_add_scene(const float* records, uint32_t n);
 /* Queues a request; returns its id (> 0) or -1. */
 int64_t vt_serve_submit(const vt_request* request);
-/* One iteration: admit queued requests (slots, memory budget), trace the next sample chunk
- * of every admitted request in one batched launch, retire finished requests.  Writes up to
+/* One iteration: admit queued requests (slots, memory budget), launch the policy's sample
+ * chunks of the admitted requests in one batched launch, and retire the requests whose last
+ * chunk has finished (with pipeline 2 that is the previous iteration's).  Writes up to
  * max_completed finished ids; returns the number finished or -1. */
 int32_t vt_serve_step(int64_t* completed, uint32_t max_completed);
 /* Copies a finished request's RGBA32F radiance and/or hit records (16 B per pixel each)

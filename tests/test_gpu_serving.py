@@ -51,15 +51,17 @@ def _plain(r, recs, q):
     return r.radiance(), r.hits()
 
 
-def test_mixed_batch_equals_plain_render():
+@pytest.mark.parametrize("policy,pipeline", [(serving.ROUND_ROBIN, 1), (serving.HEAD_RIDERS, 2),
+                                             (serving.SHORTEST_FIRST, 2)])
+def test_mixed_batch_equals_plain_render(policy, pipeline):
     r = R.reset_renderer()
     _, recs = _setup(r)
     reqs = _requests()
-    loop = serving.ServingLoop(r, max_slots=8, spp_chunk=2, memory_budget=64 << 20, max_work=1 << 20, tile_w=8,
-                               max_bounces=3, use_graph=False)
+    loop = serving.ServingLoop(r, max_slots=8, spp_chunk=2, memory_budget=64 << 20, max_work=1 << 15, tile_w=8,
+                               max_bounces=3, use_graph=False, policy=policy, pipeline=pipeline)
     for rc in recs:
         loop.add_scene(rc)
-    _say("case: serving mixed batch of 5 vs plain render")
+    _say(f"case: serving mixed batch of 5 vs plain render, policy {policy} pipeline {pipeline}")
     got = loop.run_all(reqs)
     loop.close()
     for i, q in enumerate(reqs):
@@ -74,7 +76,7 @@ def test_alone_vs_mixed_and_graph_replays():
     results = {}
     for tag, graph, subset in (("alone", False, [1]), ("mixed", False, None), ("graph", True, None)):
         loop = serving.ServingLoop(r, max_slots=8, spp_chunk=1, memory_budget=64 << 20, max_work=1 << 20, tile_w=16,
-                                   max_bounces=3, use_graph=graph)
+                                   max_bounces=3, use_graph=graph, policy=serving.ROUND_ROBIN, pipeline=2)
         for rc in recs:
             loop.add_scene(rc)
         sel = [reqs[i] for i in subset] if subset else reqs
@@ -96,19 +98,21 @@ def test_budget_admission_queues_and_completes():
     reqs = _requests()
     budget = 80 * 80 * 48 + 4096  # room for the largest request, about one at a time
     loop = serving.ServingLoop(r, max_slots=8, spp_chunk=2, memory_budget=budget, max_work=1 << 20, tile_w=8,
-                               max_bounces=3, use_graph=False)
+                               max_bounces=3, use_graph=False, pipeline=1)
     for rc in recs:
         loop.add_scene(rc)
     ids = [loop.submit(q) for q in reqs]
-    loop.step()
+    first = loop.step()
     st = loop.status()
     assert st["queued"] >= 3 and st["active"] <= 2
     assert st["memory_in_use"] <= budget
     done = {}
+    # a finished request holds its buffers until it is read: read every completion
     for _ in range(200):
-        for rid in loop.step():
+        for rid in first + loop.step():
             q = reqs[ids.index(rid)]
             done[rid] = loop.read(rid, q.width, q.height)
+        first = []
         if len(done) == len(reqs):
             break
     loop.close()

@@ -7,12 +7,17 @@
 //   1. admission: the queue head is admitted while a slot is free and its buffers (hit
 //      records, radiance running sum, output: 48 B per pixel) fit the memory budget
 //      (first-fit arena);
-//   2. batching: every active request contributes its next chunk of samples (all pixels x
-//      up to spp_chunk samples) while the iteration's work budget allows;
+//   2. batching: admitted requests with samples left contribute a chunk of samples (all
+//      pixels x some samples) within the iteration's work budget, chosen by the policy:
+//      round robin (every request its next spp_chunk samples), head + riders (the oldest
+//      request gets the budget, requests whose whole remainder fits a quarter of it ride
+//      along and finish), or shortest remaining work first;
 //   3. one launch of the batched wavefront pipeline traces all chunks together (optionally
 //      replayed from a captured hipGraph), so requests that arrived while others were in
-//      flight share the device with them;
-//   4. requests whose last chunk ran are complete; their output waits for vt_serve_read.
+//      flight share the device with them.  With pipeline depth 2 the next iteration is
+//      queued before the previous one is waited for, so host work overlaps the GPU;
+//   4. requests whose last chunk has finished are complete; their output waits for
+//      vt_serve_read, which copies on a separate stream (overlapping the next iteration).
 // A request's pixels depend only on its own scene, camera, seed and sample indices and
 // its samples are summed in sample order, so its result is bit-identical whether it is
 // rendered alone, in any mix, or by the plain single-frame path.
@@ -84,8 +89,15 @@ struct ServeReq {
     vt_request q{};
     FrameGPU F{};
     size_t off = 0, bytes = 0;
-    uint32_t next_s = 0;
+    uint32_t next_s = 0;   // samples launched so far
     int state = kQueued;
+};
+
+// one launched iteration not yet waited for
+struct Inflight {
+    hipEvent_t done = nullptr, t0 = nullptr, t1 = nullptr;
+    std::vector<int64_t> finishing;   // requests whose last samples are in it
+    uint64_t work = 0;
 };
 
 struct Serve {
@@ -101,10 +113,11 @@ struct Serve {
     std::map<int64_t, ServeReq> reqs;
     int64_t next_id = 1;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
-    hipEvent_t done = nullptr;
+    hipStream_t copy = nullptr;
+    Inflight fl[2];
+    uint32_t fl_head = 0, fl_count = 0;
     uint64_t iterations = 0, samples = 0;
     float last_ms = 0.0f;
-    hipEvent_t t0 = nullptr, t1 = nullptr;
 };
 
 Serve sv;
@@ -126,10 +139,111 @@ void free_all() {
         if (gx) hipGraphExecDestroy(gx);
     hipFree(sv.arena_mem);
     hipFree(sv.wave_mem);
-    if (sv.done) hipEventDestroy(sv.done);
-    if (sv.t0) hipEventDestroy(sv.t0);
-    if (sv.t1) hipEventDestroy(sv.t1);
+    for (auto& f : sv.fl) {
+        if (f.done) hipEventDestroy(f.done);
+        if (f.t0) hipEventDestroy(f.t0);
+        if (f.t1) hipEventDestroy(f.t1);
+    }
+    if (sv.copy) hipStreamDestroy(sv.copy);
     sv = Serve{};
+}
+
+uint64_t per_sample(const ServeReq& r, uint32_t tile_w) {
+    return (uint64_t)tiles_of(r.q.width, r.q.height, tile_w) * 64u;
+}
+
+// The iteration's chunks: (request id, samples) by policy, within the work budget and the
+// wave buffers' capacity.
+void choose(std::vector<std::pair<int64_t, uint32_t>>& out, uint64_t& work) {
+    const vt_serve_config& c = sv.cfg;
+    out.clear();
+    work = 0;
+    std::vector<int64_t> cand;
+    for (int64_t id : sv.active) {
+        const ServeReq& r = sv.reqs[id];
+        if (r.next_s < r.q.spp) cand.push_back(id);
+    }
+    if (cand.empty()) return;
+    const uint64_t W = c.max_work, cap = sv.wave.npaths;
+    auto remaining = [&](int64_t id) {
+        const ServeReq& r = sv.reqs[id];
+        return per_sample(r, c.tile_w) * (r.q.spp - r.next_s);
+    };
+    auto add = [&](int64_t id, uint32_t ns) {
+        out.emplace_back(id, ns);
+        work += per_sample(sv.reqs[id], c.tile_w) * ns;
+    };
+    if (c.policy == VT_SERVE_ROUND_ROBIN) {
+        for (int64_t id : cand) {
+            if (out.size() >= VT_MAX_BATCH) break;
+            const ServeReq& r = sv.reqs[id];
+            const uint64_t ps = per_sample(r, c.tile_w);
+            uint32_t ns = std::min(c.spp_chunk, r.q.spp - r.next_s);
+            if (work + ps * ns > W) {
+                const uint64_t fit = work < W ? (W - work) / ps : 0;
+                if (fit == 0 && !out.empty()) continue;
+                ns = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ns, fit));
+            }
+            if (work + ps * ns > cap) continue;
+            add(id, ns);
+        }
+        return;
+    }
+    if (c.policy == VT_SERVE_SHORTEST_FIRST)
+        std::stable_sort(cand.begin(), cand.end(), [&](int64_t a, int64_t b) { return remaining(a) < remaining(b); });
+    // head: the first candidate (oldest, or shortest) takes what the budget allows; alone
+    // (nothing else admitted or queued) it takes what the wave buffers hold, so a lone
+    // request runs like a plain frame
+    {
+        const ServeReq& r = sv.reqs[cand[0]];
+        const uint64_t ps = per_sample(r, c.tile_w);
+        const bool alone = cand.size() == 1 && sv.queue.empty();
+        const uint64_t fit = std::max<uint64_t>(1, (alone ? cap : W) / ps);
+        add(cand[0], (uint32_t)std::min<uint64_t>(r.q.spp - r.next_s, fit));
+    }
+    if (c.policy == VT_SERVE_SHORTEST_FIRST) {
+        for (size_t k = 1; k < cand.size() && out.size() < VT_MAX_BATCH; ++k) {
+            const ServeReq& r = sv.reqs[cand[k]];
+            const uint64_t ps = per_sample(r, c.tile_w);
+            const uint64_t fit = work < W ? (W - work) / ps : 0;
+            const uint32_t ns = (uint32_t)std::min<uint64_t>(r.q.spp - r.next_s, fit);
+            if (ns > 0) add(cand[k], ns);
+        }
+        return;
+    }
+    // riders: requests whose whole remainder fits a quarter of the budget, smallest first
+    std::vector<int64_t> rest(cand.begin() + 1, cand.end());
+    std::stable_sort(rest.begin(), rest.end(), [&](int64_t a, int64_t b) { return remaining(a) < remaining(b); });
+    const uint64_t limit = work + std::max<uint64_t>(W > work ? W - work : 0, W / 4);
+    for (int64_t id : rest) {
+        if (out.size() >= VT_MAX_BATCH) break;
+        const uint64_t need = remaining(id);
+        if (work + need > limit || work + need > cap) break;
+        add(id, sv.reqs[id].q.spp - sv.reqs[id].next_s);
+    }
+}
+
+// waits for the oldest in-flight iteration and retires its finished requests
+uint32_t retire_oldest(int64_t* completed, uint32_t max_completed, uint32_t n_done, int* rc) {
+    Inflight& f = sv.fl[sv.fl_head];
+    if (hipEventSynchronize(f.done) != hipSuccess) {
+        *rc = -1;
+        return n_done;
+    }
+    hipEventElapsedTime(&sv.last_ms, f.t0, f.t1);
+    ++sv.iterations;
+    sv.samples += f.work;
+    for (int64_t id : f.finishing) {
+        ServeReq& r = sv.reqs[id];
+        r.state = kDone;
+        sv.active.erase(std::find(sv.active.begin(), sv.active.end(), id));
+        if (completed && n_done < max_completed) completed[n_done] = id;
+        ++n_done;
+    }
+    f.finishing.clear();
+    sv.fl_head = (sv.fl_head + 1) % 2;
+    --sv.fl_count;
+    return n_done;
 }
 
 }  // namespace
@@ -143,23 +257,29 @@ int32_t vt_serve_init(const vt_serve_config* cfg) {
     if (sv.on) free_all();
     vt_serve_config c = *cfg;
     if (c.max_slots == 0 || c.max_slots > VT_MAX_BATCH || c.memory_budget < 48 * 64 || c.max_work < 64 ||
-        c.spp_chunk == 0 || c.max_bounces > 29 || (c.tile_w && 64u % c.tile_w != 0)) {
+        c.spp_chunk == 0 || c.max_bounces > 29 || (c.tile_w && 64u % c.tile_w != 0) || c.policy > 2 ||
+        c.pipeline > 2) {
         vt_rt_set_error("vt_serve_init: invalid configuration");
         return -1;
     }
     if (c.tile_w == 0) c.tile_w = 8;
-    // path state: the work budget, or one sample of the largest frame the budget admits
+    if (c.pipeline == 0) c.pipeline = 1;
+    // path state: the work budget plus the riders' quarter, or one sample of the largest
+    // frame the budget admits
     const size_t max_px = c.memory_budget / 48;
     const size_t one_spp = max_px + 64 * 2 * 16384 + 64;
-    const size_t paths = std::max<size_t>(c.max_work, one_spp);
+    const size_t paths = std::max<size_t>((size_t)c.max_work + c.max_work / 4, one_spp);
     if (paths > 0x7FFFFFFFull) {
         vt_rt_set_error("vt_serve_init: work budget too large");
         return -1;
     }
-    if (hipMalloc(&sv.arena_mem, c.memory_budget) != hipSuccess ||
-        hipMalloc(&sv.wave_mem, vt_wave_bytes((uint32_t)paths)) != hipSuccess ||
-        hipEventCreateWithFlags(&sv.done, hipEventDisableTiming) != hipSuccess || hipEventCreate(&sv.t0) != hipSuccess ||
-        hipEventCreate(&sv.t1) != hipSuccess) {
+    bool ok = hipMalloc(&sv.arena_mem, c.memory_budget) == hipSuccess &&
+              hipMalloc(&sv.wave_mem, vt_wave_bytes((uint32_t)paths)) == hipSuccess &&
+              hipStreamCreateWithFlags(&sv.copy, hipStreamNonBlocking) == hipSuccess;
+    for (auto& f : sv.fl)
+        ok = ok && hipEventCreateWithFlags(&f.done, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreate(&f.t0) == hipSuccess && hipEventCreate(&f.t1) == hipSuccess;
+    if (!ok) {
         vt_rt_set_error("vt_serve_init: allocation failed");
         free_all();
         return -1;
@@ -243,79 +363,69 @@ int32_t vt_serve_step(int64_t* completed, uint32_t max_completed) {
         sv.active.push_back(sv.queue.front());
         sv.queue.pop_front();
     }
-    // 2. batching: next sample chunk of every active request, within the work budget
-    std::vector<ReqGPU> batch;
-    std::vector<int64_t> ids;
+    // 2. batching by policy
+    std::vector<std::pair<int64_t, uint32_t>> chunks;
     uint64_t work = 0;
-    for (int64_t id : sv.active) {
-        ServeReq& r = sv.reqs[id];
-        if (batch.size() >= VT_MAX_BATCH) break;
-        const uint64_t per_s = (uint64_t)tiles_of(r.q.width, r.q.height, c.tile_w) * 64u;
-        uint32_t ns = std::min(c.spp_chunk, r.q.spp - r.next_s);
-        if (work + per_s * ns > c.max_work) {
-            const uint64_t fit = work < c.max_work ? (c.max_work - work) / per_s : 0;
-            if (fit == 0 && !batch.empty()) continue;
-            ns = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ns, fit));
+    if (sv.fl_count < c.pipeline) choose(chunks, work);
+    int rc = 0;
+    if (!chunks.empty()) {
+        std::vector<ReqGPU> batch;
+        Inflight& f = sv.fl[(sv.fl_head + sv.fl_count) % 2];
+        f.finishing.clear();
+        for (auto& ch : chunks) {
+            ServeReq& r = sv.reqs[ch.first];
+            ReqGPU R;
+            std::memset(&R, 0, sizeof(R));
+            R.F = r.F;
+            const ServeScene& sc = sv.scenes[r.q.scene];
+            R.S.inst = sc.bvh.inst;
+            R.S.nodes = sc.bvh.nodes;
+            R.S.vols = vt_rt_vol_table();
+            R.S.ninst = sc.n;
+            R.S.nvol = vt_rt_nvol();
+            R.S.bvh_depth = sc.depth;
+            const size_t npx = (size_t)r.q.width * r.q.height;
+            char* base = sv.arena_mem + r.off;
+            R.hits = reinterpret_cast<HitRecord*>(base);
+            R.acc = reinterpret_cast<float4*>(base + npx * 16);
+            R.rad = reinterpret_cast<float4*>(base + npx * 32);
+            R.s0 = r.next_s;
+            R.s1 = r.next_s + ch.second;
+            R.final_chunk = R.s1 == r.q.spp ? 1u : 0u;
+            R.tile_w = c.tile_w;
+            batch.push_back(R);
+            r.next_s = R.s1;
+            if (R.final_chunk) f.finishing.push_back(ch.first);
         }
-        if (work + per_s * ns > sv.wave.npaths) continue;
-        ReqGPU R;
-        std::memset(&R, 0, sizeof(R));
-        R.F = r.F;
-        const ServeScene& sc = sv.scenes[r.q.scene];
-        R.S.inst = sc.bvh.inst;
-        R.S.nodes = sc.bvh.nodes;
-        R.S.vols = vt_rt_vol_table();
-        R.S.ninst = sc.n;
-        R.S.nvol = vt_rt_nvol();
-        R.S.bvh_depth = sc.depth;
-        const size_t npx = (size_t)r.q.width * r.q.height;
-        char* base = sv.arena_mem + r.off;
-        R.hits = reinterpret_cast<HitRecord*>(base);
-        R.acc = reinterpret_cast<float4*>(base + npx * 16);
-        R.rad = reinterpret_cast<float4*>(base + npx * 32);
-        R.s0 = r.next_s;
-        R.s1 = r.next_s + ns;
-        R.final_chunk = R.s1 == r.q.spp ? 1u : 0u;
-        R.tile_w = c.tile_w;
-        batch.push_back(R);
-        ids.push_back(id);
-        work += per_s * ns;
+        // 3. one launch for all chunks
+        bool deep = false;
+        for (auto& R : batch) deep = deep || R.S.bvh_depth + 1u > 16u;
+        hipEventRecord(f.t0, s);
+        if (c.use_graph) {
+            hipGraphExec_t& gx = sv.graph[deep ? 1 : 0];
+            if (!gx) rc = vt_trace_capture(c.max_bounces, deep, sv.wave, s, &gx);
+            if (rc == 0) rc = vt_trace_upload_tables(batch.data(), (uint32_t)batch.size(), sv.wave, s, nullptr);
+            if (rc == 0) rc = hipGraphLaunch(gx, s) == hipSuccess ? 0 : -1;
+        } else {
+            rc = vt_trace_batch(batch.data(), (uint32_t)batch.size(), sv.wave, nullptr, nullptr, nullptr, nullptr, s);
+        }
+        hipEventRecord(f.t1, s);
+        hipEventRecord(f.done, s);
+        if (rc != 0) {
+            vt_rt_set_error("vt_serve_step: launch failed");
+            return -1;
+        }
+        f.work = work;
+        ++sv.fl_count;
     }
-    if (batch.empty()) return 0;
-    // 3. one launch for all chunks
-    bool deep = false;
-    for (auto& R : batch) deep = deep || R.S.bvh_depth + 1u > 16u;
-    hipEventRecord(sv.t0, s);
-    int rc;
-    if (c.use_graph) {
-        hipGraphExec_t& gx = sv.graph[deep ? 1 : 0];
-        rc = 0;
-        if (!gx) rc = vt_trace_capture(c.max_bounces, deep, sv.wave, s, &gx);
-        if (rc == 0) rc = vt_trace_upload_tables(batch.data(), (uint32_t)batch.size(), sv.wave, s, nullptr);
-        if (rc == 0) rc = hipGraphLaunch(gx, s) == hipSuccess ? 0 : -1;
-    } else {
-        rc = vt_trace_batch(batch.data(), (uint32_t)batch.size(), sv.wave, nullptr, nullptr, nullptr, nullptr, s);
-    }
-    hipEventRecord(sv.t1, s);
-    if (rc != 0) {
-        vt_rt_set_error("vt_serve_step: launch failed");
-        return -1;
-    }
-    if (hipEventSynchronize(sv.t1) != hipSuccess) return -1;
-    hipEventElapsedTime(&sv.last_ms, sv.t0, sv.t1);
-    ++sv.iterations;
-    sv.samples += work;
-    // 4. retire finished requests
+    // 4. retire: keep at most pipeline - 1 iterations in flight after a launch; with nothing
+    // to launch, wait for the oldest one
     uint32_t n_done = 0;
-    for (size_t k = 0; k < batch.size(); ++k) {
-        ServeReq& r = sv.reqs[ids[k]];
-        r.next_s = batch[k].s1;
-        if (r.next_s == r.q.spp) {
-            r.state = kDone;
-            sv.active.erase(std::find(sv.active.begin(), sv.active.end(), ids[k]));
-            if (completed && n_done < max_completed) completed[n_done] = ids[k];
-            ++n_done;
-        }
+    if (sv.fl_count > 0 && (sv.fl_count >= c.pipeline || chunks.empty()))
+        n_done = retire_oldest(completed, max_completed, n_done, &rc);
+    if (rc != 0) {
+        vt_rt_set_error("vt_serve_step: iteration failed");
+        return -1;
     }
     return (int32_t)n_done;
 }
@@ -329,11 +439,12 @@ int32_t vt_serve_read(int64_t id, float* rgba, uint64_t bytes, void* hits, uint6
     const size_t npx = (size_t)r.q.width * r.q.height;
     if ((rgba && bytes < npx * 16) || (hits && hit_bytes < npx * 16)) return -1;
     hipSetDevice(vt_rt_device());
-    hipStream_t s = vt_rt_stream();
+    // the request's last iteration has completed (its event was waited for), so the copy may
+    // run beside the iteration now in flight
     const char* base = sv.arena_mem + r.off;
-    if (rgba) hipMemcpyAsync(rgba, base + npx * 32, npx * 16, hipMemcpyDeviceToHost, s);
-    if (hits) hipMemcpyAsync(hits, base, npx * 16, hipMemcpyDeviceToHost, s);
-    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    if (rgba) hipMemcpyAsync(rgba, base + npx * 32, npx * 16, hipMemcpyDeviceToHost, sv.copy);
+    if (hits) hipMemcpyAsync(hits, base, npx * 16, hipMemcpyDeviceToHost, sv.copy);
+    if (hipStreamSynchronize(sv.copy) != hipSuccess) return -1;
     sv.arena.release(r.off, r.bytes);
     sv.reqs.erase(it);
     return 0;

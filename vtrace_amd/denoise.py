@@ -201,17 +201,24 @@ def render_spp(r, scene: "scenes.Scene", spp: int, seed: int, chunk: int = 16, g
     return (acc / nfr).astype(np.float32), gb, ms
 
 
-def time_to_quality(curve: list, threshold: float):
-    """First (time, relMSE) point of a (time, relMSE) curve at or below threshold, else None."""
-    for t, e in curve:
-        if e <= threshold:
-            return t
+def crossing_at(curve: list, threshold: float):
+    """Time (ms) at which a curve of {"ms", "relmse"} points, relMSE falling, reaches threshold
+    (log-log interpolation; None if it never does)."""
+    prev = None
+    for c in curve:
+        if c["relmse"] <= threshold:
+            if prev is None:
+                return c["ms"]
+            f = (math.log(prev["relmse"]) - math.log(threshold)) / (math.log(prev["relmse"]) - math.log(c["relmse"]))
+            return math.exp(math.log(prev["ms"]) + f * (math.log(c["ms"]) - math.log(prev["ms"])))
+        prev = c
     return None
 
 
 # ---- benchmark ------------------------------------------------------------------------------
-def benchmark(r, eval_scene: "scenes.Scene", train_seeds=(101, 102, 103, 104, 105, 106), train_spp=(1, 4, 16, 64),
-              target_spp: int = 1024, judge_spp: int = 4096, iters: int = 2000, batch: int = 8, crop: int = 128,
+def benchmark(r, eval_scene: "scenes.Scene", train_seeds=(101, 102, 103, 104, 105, 106),
+              train_spp=(1, 4, 16, 64, 256), target_spp: int = 1024, judge_spp: int = 4096, iters: int = 8000,
+              batch: int = 8, crop: int = 128,
               lr: float = 2e-3, threshold: float = 1e-3, log=None) -> dict:
     """Trains on C2-style scenes (other seeds than the evaluation scene), then measures the GPU
     time each method needs to reach relMSE <= threshold against a judge_spp render of
@@ -255,32 +262,48 @@ def benchmark(r, eval_scene: "scenes.Scene", train_seeds=(101, 102, 103, 104, 10
             break
         spp *= 2
 
-    def first(curve):
+    def crossing(curve):
+        """(ms, spp) where relMSE reaches threshold: log-log interpolation between the last
+        point above and the first point at or below it (the first point if it is already below)."""
+        prev = None
         for c in curve:
             if c["relmse"] <= threshold:
-                return c
+                if prev is None:
+                    return c["ms"], c["spp"], False
+                f = (math.log(prev["relmse"]) - math.log(threshold)) / (math.log(prev["relmse"]) - math.log(c["relmse"]))
+                ms = math.exp(math.log(prev["ms"]) + f * (math.log(c["ms"]) - math.log(prev["ms"])))
+                spp = math.exp(math.log(prev["spp"]) + f * (math.log(c["spp"]) - math.log(prev["spp"])))
+                return ms, spp, False
+            prev = c
+        if len(curve) >= 2:  # Monte Carlo error falls as 1/spp: extrapolate from the last point
+            last = curve[-1]
+            k = last["relmse"] / threshold
+            return last["ms"] * k, last["spp"] * k, True
         return None
 
-    fp, fd = first(plain), first(den)
     out = {"judge_spp": judge_spp, "judge_ms": round(judge_ms, 1), "threshold": threshold,
            "params": d.real_param_count, "train": {"frames": len(train_seeds) * len(train_spp), "iters": iters,
                                                    "batch": batch, "crop": crop, "seconds": round(t_train, 2),
                                                    "data_seconds": round(t_data, 1),
                                                    "loss_first50": float(losses[:50].mean()),
-                                                   "loss_last50": float(losses[-50:].mean())},
+                                                   "loss_last200": float(losses[-200:].mean())},
            "plain_curve": plain, "denoised_curve": den}
-    if fp is None and plain:
-        # Monte Carlo error falls as 1/spp: extrapolate the plain path from its last point
-        last = plain[-1]
-        k = last["relmse"] / threshold
-        fp = {"spp": round(last["spp"] * k, 1), "ms": round(last["ms"] * k, 3), "relmse": threshold,
-              "extrapolated": True}
-    out["plain_time_to_threshold_ms"] = fp["ms"] if fp else None
-    out["plain_spp_at_threshold"] = fp["spp"] if fp else None
-    out["denoised_time_to_threshold_ms"] = fd["ms"] if fd else None
-    out["denoised_spp_at_threshold"] = fd["spp"] if fd else None
+    for name, curve in (("plain", plain), ("denoised", den)):
+        c = crossing(curve)
+        out[f"{name}_time_to_threshold_ms"] = round(c[0], 3) if c else None
+        out[f"{name}_spp_at_threshold"] = round(c[1], 1) if c else None
+        out[f"{name}_extrapolated"] = bool(c[2]) if c else None
     out["denoised_best_relmse"] = min(c["relmse"] for c in den) if den else None
-    if fp and fd:
-        out["speedup"] = round(fp["ms"] / fd["ms"], 2)
+    if out["plain_time_to_threshold_ms"] and out["denoised_time_to_threshold_ms"]:
+        out["speedup"] = round(out["plain_time_to_threshold_ms"] / out["denoised_time_to_threshold_ms"], 3)
+    # equal-quality speedups at low spp: plain time to reach the denoised error of each input spp
+    eq = []
+    for c in den:
+        sub = [q for q in plain if q["relmse"] > 0]
+        tgt = c["relmse"]
+        t_plain = crossing_at(sub, tgt)
+        if t_plain:
+            eq.append({"spp": c["spp"], "denoised_ms": c["ms"], "plain_ms_same_relmse": round(t_plain, 3)})
+    out["equal_quality"] = eq
     d.close()
     return out

@@ -1,9 +1,10 @@
 """Serving loop driver: resident scenes, a Poisson request stream, continuous batching.
 
 The scheduler itself is native (vt_serve_* in librender_hip, serve.hip): FIFO admission by
-slots and HBM budget, one batched launch per iteration holding the next sample chunk of
-every admitted request, optional hipGraph replay.  This module feeds it requests at their
-arrival times, measures per-request latency (arrival -> result available) on the wall
+slots and HBM budget, one batched launch per iteration holding sample chunks of the admitted
+requests chosen by a policy (round robin, head + riders, shortest remaining work first),
+optional hipGraph replay, two iterations in flight.  This module feeds it requests at their
+arrival times, measures per-request latency (arrival -> result in host memory) on the wall
 clock, and compares against the plain path serving the same stream one request at a time.
 
 The request stream (seeded): resolution uniform over {256x256, 640x480, 1280x720,
@@ -28,10 +29,14 @@ from .renderer import HIT_DTYPE, RendererError
 RESOLUTIONS = ((256, 256), (640, 480), (1280, 720), (1920, 1080))
 
 
+ROUND_ROBIN, HEAD_RIDERS, SHORTEST_FIRST = 0, 1, 2
+POLICY_NAMES = {ROUND_ROBIN: "round_robin", HEAD_RIDERS: "head_riders", SHORTEST_FIRST: "shortest_first"}
+
+
 class VtServeConfig(C.Structure):
     _fields_ = [("max_slots", C.c_uint32), ("spp_chunk", C.c_uint32), ("memory_budget", C.c_uint64),
                 ("max_work", C.c_uint32), ("tile_w", C.c_uint32), ("max_bounces", C.c_uint32),
-                ("use_graph", C.c_uint32)]
+                ("use_graph", C.c_uint32), ("policy", C.c_uint32), ("pipeline", C.c_uint32)]
 
 
 class VtRequest(C.Structure):
@@ -102,12 +107,13 @@ class Request:
 
 
 class ServingLoop:
-    def __init__(self, renderer, max_slots=32, spp_chunk=4, memory_budget=8 << 30, max_work=8 << 20, tile_w=8,
-                 max_bounces=4, use_graph=True):
+    def __init__(self, renderer, max_slots=32, spp_chunk=4, memory_budget=2 << 30, max_work=16 << 20, tile_w=8,
+                 max_bounces=4, use_graph=True, policy=HEAD_RIDERS, pipeline=2):
         self.r = renderer
         self.lib = renderer.lib
         declare(self.lib)
-        cfg = VtServeConfig(max_slots, spp_chunk, memory_budget, max_work, tile_w, max_bounces, 1 if use_graph else 0)
+        cfg = VtServeConfig(max_slots, spp_chunk, memory_budget, max_work, tile_w, max_bounces, 1 if use_graph else 0,
+                            policy, pipeline)
         if self.lib.vt_serve_init(C.byref(cfg)) != 0:
             raise RendererError(f"vt_serve_init failed: {renderer.error()}")
         self.cfg = cfg
@@ -304,13 +310,15 @@ def default_tune_cache() -> str:
     return os.path.join(base, "vtrace_amd", "autotune.json")
 
 
-def benchmark(renderer, nreq: int = 2000, util: float = 0.75, seed: int = 11, probe: int = 24, max_slots: int = 32,
-              spp_chunk: int = 4, memory_budget: int = 8 << 30, max_work: int = 8 << 20, max_bounces: int = 4,
-              use_graph: bool = True, tune_cache: str | None = None, plain: bool = True, log=None) -> dict:
-    """The serving workload: 32 resident noise volumes (64^3..512^3, fill 5-40%), a seeded Poisson
-    stream of nreq requests at `util` of the measured batched capacity, served by the batching loop
-    and then by the plain path one request at a time (same arrivals).  Latency = arrival -> result
-    read back to host memory, wall clock."""
+def benchmark(renderer, nreq: int = 2000, util: float = 0.75, seed: int = 11, probe: int = 48, max_slots: int = 32,
+              spp_chunk: int = 4, memory_budget: int = 2 << 30, max_work: int = 16 << 20, max_bounces: int = 4,
+              use_graph: bool = True, policy: int = HEAD_RIDERS, pipeline: int = 2, tune_cache: str | None = None,
+              plain: bool = True, log=None) -> dict:
+    """The serving workload: 32 resident noise volumes (64^3..512^3, fill 5-40%) and seeded Poisson
+    streams of nreq requests.  Capacities (requests/s with a probe of requests all queued at
+    once) are measured for the batching loop and for the plain path; each stream is then served
+    by both, at `util` of the loop's capacity and at `util` of the plain path's capacity.
+    Latency = arrival -> radiance in host memory, wall clock."""
     say = log or (lambda *a: None)
     t0 = time.perf_counter()
     vols = resident_volumes()
@@ -325,10 +333,10 @@ def benchmark(renderer, nreq: int = 2000, util: float = 0.75, seed: int = 11, pr
     recs = scene_records(ids, dims)
     say(f"serving: 32 volumes generated in {t_gen:.1f} s, bricked in {t_up:.1f} s, {resident / 2**20:.0f} MiB resident")
     cfg = dict(max_slots=max_slots, spp_chunk=spp_chunk, memory_budget=memory_budget, max_work=max_work,
-               max_bounces=max_bounces)
+               max_bounces=max_bounces, policy=policy, pipeline=pipeline)
     probe_reqs = request_stream(probe, len(recs), dims, rate_hz=1.0, seed=seed + 1, max_bounces=max_bounces)
-    tune = autotune_tile(renderer, probe_reqs[: max(4, probe // 3)], recs, cache_path=tune_cache or default_tune_cache(),
-                         key=f"serve32:{max_bounces}b:{spp_chunk}c", **cfg)
+    tune = autotune_tile(renderer, probe_reqs[:12], recs, cache_path=tune_cache or default_tune_cache(),
+                         key=f"serve32:{max_bounces}b:{POLICY_NAMES[policy]}:{max_work}", **cfg)
     say(f"serving: tile {tune['tile']} ({tune['seconds']})")
     loop = ServingLoop(renderer, tile_w=tune["tile_w"], use_graph=use_graph, **cfg)
     for rc in recs:
@@ -336,31 +344,38 @@ def benchmark(renderer, nreq: int = 2000, util: float = 0.75, seed: int = 11, pr
     loop.run_all(probe_reqs[:4])  # warm-up (graph capture, first-touch of buffers)
     t0 = time.perf_counter()
     loop.run_all(probe_reqs)
-    t_probe = time.perf_counter() - t0
-    service = t_probe / len(probe_reqs)
-    rate = util / service
-    reqs = request_stream(nreq, len(recs), dims, rate_hz=rate, seed=seed, max_bounces=max_bounces)
-    say(f"serving: capacity {1 / service:.1f} req/s (probe of {probe}), stream of {nreq} at {rate:.1f} req/s")
-    st0 = loop.status()
-    res = loop.run_stream(reqs)
-    st1 = loop.status()
+    cap_loop = len(probe_reqs) / (time.perf_counter() - t0)
+    at_once = [Request(**{**q.__dict__, "arrival": 0.0}) for q in probe_reqs]
+    plain_one_at_a_time(renderer, recs, at_once[:4])
+    t0 = time.perf_counter()
+    plain_one_at_a_time(renderer, recs, at_once)
+    cap_plain = len(probe_reqs) / (time.perf_counter() - t0)
+    say(f"serving: capacity {cap_loop:.1f} req/s batched, {cap_plain:.1f} req/s plain (probe of {probe})")
+    out = {"requests": nreq, "target_utilisation": util, "capacity_req_s": {"batched": round(cap_loop, 2),
+                                                                           "plain": round(cap_plain, 2)},
+           "tile": tune["tile"], "resident_volume_mib": round(resident / 2**20, 1),
+           "config": {"max_slots": max_slots, "memory_budget_gib": memory_budget / 2**30, "max_work": max_work,
+                      "graph": bool(use_graph), "max_bounces": max_bounces, "policy": POLICY_NAMES[policy],
+                      "pipeline": pipeline}, "streams": {}}
+    for tag, cap in (("at_batched_capacity", cap_loop), ("at_plain_capacity", cap_plain)):
+        rate = util * cap
+        reqs = request_stream(nreq, len(recs), dims, rate_hz=rate, seed=seed, max_bounces=max_bounces)
+        st0 = loop.status()
+        res = loop.run_stream(reqs)
+        st1 = loop.status()
+        p = percentiles(res["latency_s"])
+        iters = max(1, st1["iterations"] - st0["iterations"])
+        e = {"arrival_rate_hz": round(rate, 3),
+             "batched": dict(p, wall_s=round(res["wall_s"], 3), throughput_req_s=round(nreq / res["wall_s"], 3),
+                             msamples_s=round(res["samples"] / res["wall_s"] / 1e6, 2), iterations=iters,
+                             mean_batch_samples=int((st1["samples"] - st0["samples"]) / iters))}
+        say(f"serving [{tag}, {rate:.1f} req/s]: batched p50 {p['p50_ms']} ms p99 {p['p99_ms']} ms")
+        if plain:
+            pr = plain_one_at_a_time(renderer, recs, reqs)
+            pp = percentiles(pr["latency_s"])
+            e["plain"] = dict(pp, wall_s=round(pr["wall_s"], 3), throughput_req_s=round(nreq / pr["wall_s"], 3),
+                              msamples_s=round(res["samples"] / pr["wall_s"] / 1e6, 2))
+            say(f"serving [{tag}]: plain p50 {pp['p50_ms']} ms p99 {pp['p99_ms']} ms")
+        out["streams"][tag] = e
     loop.close()
-    p = percentiles(res["latency_s"])
-    out = {"requests": nreq, "arrival_rate_hz": round(rate, 3), "target_utilisation": util,
-           "capacity_req_s": round(1 / service, 3), "tile": tune["tile"],
-           "batched": dict(p, wall_s=round(res["wall_s"], 3), throughput_req_s=round(nreq / res["wall_s"], 3),
-                           msamples_s=round(res["samples"] / res["wall_s"] / 1e6, 2),
-                           iterations=int(st1["iterations"] - st0["iterations"]),
-                           mean_batch_samples=int((st1["samples"] - st0["samples"]) /
-                                                  max(1, st1["iterations"] - st0["iterations"]))),
-           "resident_volume_mib": round(resident / 2**20, 1),
-           "config": {"max_slots": max_slots, "spp_chunk": spp_chunk, "memory_budget_gib": memory_budget / 2**30,
-                      "max_work": max_work, "graph": bool(use_graph), "max_bounces": max_bounces}}
-    say(f"serving: batched p50 {p['p50_ms']} ms p99 {p['p99_ms']} ms over {res['wall_s']:.1f} s")
-    if plain:
-        pr = plain_one_at_a_time(renderer, recs, reqs)
-        pp = percentiles(pr["latency_s"])
-        out["plain"] = dict(pp, wall_s=round(pr["wall_s"], 3), throughput_req_s=round(nreq / pr["wall_s"], 3),
-                            msamples_s=round(res["samples"] / pr["wall_s"] / 1e6, 2))
-        say(f"serving: plain p50 {pp['p50_ms']} ms p99 {pp['p99_ms']} ms over {pr['wall_s']:.1f} s")
     return out
