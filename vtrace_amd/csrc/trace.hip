@@ -189,23 +189,6 @@ struct Dda {
     bool hit;
 };
 
-// One DDA step (trace.frag:83-86): axes whose side distance is the minimum step together.
-#define VT_DDA_STEP()                                         \
-    {                                                         \
-        const bool mx = sx <= gmin(sy, sz);                   \
-        const bool my = sy <= gmin(sz, sx);                   \
-        const bool mz = sz <= gmin(sx, sy);                   \
-        t_last = mx ? sx : (my ? sy : sz);                    \
-        sx += (mx ? 1.0f : 0.0f) * dx;                        \
-        sy += (my ? 1.0f : 0.0f) * dy;                        \
-        sz += (mz ? 1.0f : 0.0f) * dz;                        \
-        vx += (mx ? 1 : 0) * stx;                             \
-        vy += (my ? 1 : 0) * sty;                             \
-        vz += (mz ? 1 : 0) * stz;                             \
-        mask = (mx ? 1u : 0u) | (my ? 2u : 0u) | (mz ? 4u : 0u); \
-        ++steps;                                              \
-    }
-
 template <bool ST>
 __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3], const int v0[3], V3 mdir, Dda& r,
                                         Ctr<ST>& ctr, const Lds& mslot) {
@@ -222,9 +205,12 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
     const int nx = (int)V.dim[0], ny = (int)V.dim[1], nz = (int)V.dim[2];
     const uint32_t max_steps = V.dim[0] + V.dim[1] + V.dim[2];
     const uint32_t bdx = V.bdim[0], bdy = V.bdim[1];
-    uint32_t steps = 0, mask = 0, entry = VT_EMPTY, li = 0;
+    uint32_t steps = 0, entry = VT_EMPTY, li = 0;
     float t_last = 0.0f;
     bool hit = false;
+    bool lmx = false, lmy = false, lmz = false;  // axes of the last step (the mask, built once at the end)
+// One DDA step (trace.frag:83-86): axes whose side distance is the minimum step together;
+// float(mask) * delta is the select between delta and 0 * delta.
 #define VT_DDA_STEP()                                             \
     {                                                             \
         const bool mx = sx <= gmin(sy, sz);                       \
@@ -237,7 +223,9 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
         vx += mx ? stx : 0;                                       \
         vy += my ? sty : 0;                                       \
         vz += mz ? stz : 0;                                       \
-        mask = (mx ? 1u : 0u) | (my ? 2u : 0u) | (mz ? 4u : 0u);  \
+        lmx = mx;                                                 \
+        lmy = my;                                                 \
+        lmz = mz;                                                 \
         ++steps;                                                  \
     }
     while (steps < max_steps && (uint32_t)vx < (uint32_t)nx && (uint32_t)vy < (uint32_t)ny &&
@@ -279,6 +267,7 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
     }
 #undef VT_DDA_STEP
     ctr.steps(steps);
+    const uint32_t mask = (lmx ? 1u : 0u) | (lmy ? 2u : 0u) | (lmz ? 4u : 0u);
     r.v[0] = vx; r.v[1] = vy; r.v[2] = vz;
     r.steps = steps;
     r.mask = hit && steps > 0 ? mask : 0u;
@@ -833,7 +822,7 @@ struct TileMap {
 };
 
 template <bool ST, bool DEEP>
-__global__ __launch_bounds__(kBlock) void k_primary(const ReqGPU* __restrict__ reqs, const BatchGPU* __restrict__ B,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_primary(const ReqGPU* __restrict__ reqs, const BatchGPU* __restrict__ B,
                                                     const float* __restrict__ srgb_dec, WF wf,
                                                     unsigned long long* stats) {
     __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
@@ -951,7 +940,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(const ReqGPU* __restrict__ re
 }
 
 template <bool ST, bool DEEP>
-__global__ __launch_bounds__(kBlock) void k_bounce(const ReqGPU* __restrict__ reqs, const BatchGPU* __restrict__ B,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_bounce(const ReqGPU* __restrict__ reqs, const BatchGPU* __restrict__ B,
                                                    const float* __restrict__ srgb_dec, WF wf, uint32_t q,
                                                    uint32_t depth, uint32_t fetch_idx, unsigned long long* stats) {
     __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
