@@ -124,6 +124,10 @@ int32_t vt_bench_frames(uint32_t warmup, uint32_t iters, float* kernel_ms, float
  * trips; info8 = {w, h, d, fmt, non-empty bricks, bytes lo, bytes hi, 0}. */
 int32_t vt_read_volume(int32_t id, void* dst, uint64_t bytes);
 int32_t vt_volume_info(int32_t id, uint32_t* info8);
+/* The volume's brick map (one u32 per 8^3 brick, x-fastest brick order): a non-empty brick's
+ * id, or 0xFFFFFF00 | r for an empty brick, r = radius of the cube of empty bricks around it
+ * (L-inf brick distance to the nearest non-empty brick - 1, at most 31). */
+int32_t vt_read_brick_map(int32_t id, uint32_t* dst, uint64_t bytes);
 /* LBVH of the current instances: (2n-1) x 32 B nodes and n x 128 B instance records. */
 int32_t vt_read_lbvh(void* nodes, uint64_t node_bytes, void* instances, uint64_t inst_bytes);
 int32_t vt_instance_count(void);

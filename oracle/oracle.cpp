@@ -29,7 +29,8 @@
 //   orc_instance_prep       pinned: AABB contains all transformed corners; texture-id bitcast
 //   orc_lbvh_build          pinned: sort == std::stable_sort, topology invariants, AABB
 //                           containment, traversal result == brute force
-//   orc_brick               pinned: dense <-> brick round trip, popcount == dense count
+//   orc_brick               pinned: dense <-> brick round trip, popcount == dense count,
+//                           empty radius == brute-force L-inf distance (tests/test_oracle.py)
 //   orc_render (primary)    pinned: brute-force nearest-voxel test on tiny volumes,
 //                           axis-aligned closed forms, mirror symmetry, max_steps cap
 //   orc_render (path)       pinned: white-furnace / single-plane closed forms, FP32-vs-FP64
@@ -95,6 +96,8 @@ typedef struct orc_stats {
 namespace {
 
 constexpr uint32_t kMiss = 0xFFFFFFFFu;
+// brick map: non-empty bricks hold their id, empty ones kEmptyBrick | radius
+constexpr uint32_t kEmptyBrick = 0xFFFFFF00u;
 constexpr float kInf = std::numeric_limits<float>::infinity();
 
 // ---- GLSL built-ins with the semantics this restatement fixes ---------------------
@@ -223,6 +226,7 @@ struct Vol {
     float size[3];
     std::vector<uint16_t> remap[3];  // texel index for DDA voxel v per axis
     std::vector<uint8_t> brick_nonempty;
+    std::vector<uint8_t> brick_radius;  // empty bricks: radius of the cube of empty bricks around them
     uint32_t bdim[3];
     const SrgbTables* s;
 
@@ -259,6 +263,37 @@ uint16_t nearest_texel(int v, float size, uint32_t dim) {
     return (uint16_t)t;
 }
 
+// Empty-space radius of every brick (this project's rule, no reference counterpart): the
+// L-infinity distance d, in bricks, from an empty brick to the nearest non-empty brick of the
+// volume, capped at kRadiusCap + 1; radius = d - 1, so the cube of bricks within `radius` of it
+// is empty (bricks outside the volume count as empty).  Computed as breadth-first layers of
+// 26-neighbour dilation.  Non-empty bricks get 0.
+constexpr uint32_t kRadiusCap = 31;
+void brick_radius(Vol& V) {
+    const int bx = (int)V.bdim[0], by = (int)V.bdim[1], bz = (int)V.bdim[2];
+    std::vector<uint8_t> d(V.brick_nonempty.size());
+    for (size_t b = 0; b < d.size(); ++b) d[b] = V.brick_nonempty[b] ? 0 : 255;
+    for (uint32_t k = 1; k <= kRadiusCap + 1; ++k)
+        for (int z = 0; z < bz; ++z)
+            for (int y = 0; y < by; ++y)
+                for (int x = 0; x < bx; ++x) {
+                    uint8_t& c = d[x + bx * (y + by * z)];
+                    if (c != 255) continue;
+                    bool near = false;
+                    for (int dz = -1; dz <= 1 && !near; ++dz)
+                        for (int dy = -1; dy <= 1 && !near; ++dy)
+                            for (int dx = -1; dx <= 1 && !near; ++dx) {
+                                const int X = x + dx, Y = y + dy, Z = z + dz;
+                                if (X < 0 || Y < 0 || Z < 0 || X >= bx || Y >= by || Z >= bz) continue;
+                                near = d[X + bx * (Y + by * Z)] == k - 1;
+                            }
+                    if (near) c = (uint8_t)k;
+                }
+    V.brick_radius.assign(d.size(), 0);
+    for (size_t b = 0; b < d.size(); ++b)
+        V.brick_radius[b] = V.brick_nonempty[b] ? 0 : (uint8_t)(std::min<uint32_t>(d[b], kRadiusCap + 1) - 1);
+}
+
 void init_vol(Vol& V, const orc_volume* v) {
     V.v = v;
     V.s = &srgb();
@@ -275,6 +310,7 @@ void init_vol(Vol& V, const orc_volume* v) {
             for (uint32_t x = 0; x < V.dim[0]; ++x)
                 if (V.occupied_texel(V.texel_index(x, y, z)))
                     V.brick_nonempty[(x >> 3) + V.bdim[0] * ((y >> 3) + V.bdim[1] * (z >> 3))] = 1;
+    brick_radius(V);
 }
 
 // world AABB padding: keeps the LBVH conservative with respect to the model-space slab
@@ -556,9 +592,10 @@ DdaResult dda(const Vol& V, const float mpos[3], const int v0[3], const float md
 }
 
 // Two-level DDA for secondary rays (no reference counterpart; this project's rule).
-// Empty 8^3 bricks are crossed in one step: the ray moves from its current position q to
-// the nearest exit plane of the brick (ties: every axis at that distance), the exit axes
-// step into the neighbouring brick and the other axes take floor(q) clamped to the brick.
+// An empty 8^3 brick of radius r (brick_radius) is crossed together with the cube of
+// (2r+1)^3 empty bricks around it in one step: the ray moves from its current position q to
+// the nearest exit plane of that cube (ties: every axis at that distance), the exit axes
+// step into the voxel beyond it and the other axes take floor(q) clamped to the cube.
 // Inside a non-empty brick a voxel walk from q (trace.frag's side/delta formulas, steps
 // with selects so zero direction components never produce NaN) tests occupancy until it
 // hits or leaves the brick; the position where it leaves is q + t * unit dir with the
@@ -583,17 +620,19 @@ DdaResult dda2(const Vol& V, const float p[3], const int v0[3], const float m[3]
     for (uint32_t it = 0; it < guard; ++it) {
         if (!(v[0] >= 0 && v[1] >= 0 && v[2] >= 0 && v[0] < n[0] && v[1] < n[1] && v[2] < n[2])) break;
         const int b8[3] = {v[0] & ~7, v[1] & ~7, v[2] & ~7};
-        const bool nonempty =
-            V.brick_nonempty[(uint32_t)(b8[0] >> 3) + V.bdim[0] * ((uint32_t)(b8[1] >> 3) + V.bdim[1] * (uint32_t)(b8[2] >> 3))] != 0;
+        const uint32_t bi = (uint32_t)(b8[0] >> 3) + V.bdim[0] * ((uint32_t)(b8[1] >> 3) + V.bdim[1] * (uint32_t)(b8[2] >> 3));
+        const bool nonempty = V.brick_nonempty[bi] != 0;
         C.brick(V, v[0], v[1], v[2]);
         if (!nonempty) {
+            // the cube of empty bricks of the brick's radius is crossed in one step
+            const int rad = 8 * (int)V.brick_radius[bi];
             float t[3];
             float tmin = kInf;
             for (int a = 0; a < 3; ++a) {
                 if (st[a] == 0) {
                     t[a] = kInf;
                 } else {
-                    const float plane = (float)(b8[a] + (st[a] > 0 ? 8 : 0));
+                    const float plane = (float)(st[a] > 0 ? b8[a] + 8 + rad : b8[a] - rad);
                     t[a] = (plane - q[a]) * rinv[a];
                 }
                 tmin = gmin(tmin, t[a]);
@@ -602,15 +641,16 @@ DdaResult dda2(const Vol& V, const float p[3], const int v0[3], const float m[3]
             float nq[3];
             int nv[3];
             for (int a = 0; a < 3; ++a) {
+                const int lo = b8[a] - rad, hi = b8[a] + 8 + rad;
                 if (st[a] != 0 && t[a] <= tmin) {
                     mk |= 1u << a;
-                    nq[a] = (float)(b8[a] + (st[a] > 0 ? 8 : 0));
-                    nv[a] = st[a] > 0 ? b8[a] + 8 : b8[a] - 1;
+                    nq[a] = (float)(st[a] > 0 ? hi : lo);
+                    nv[a] = st[a] > 0 ? hi : lo - 1;
                 } else {
                     const float x = q[a] + tmin * u[a];
                     nq[a] = x;
                     int f = (int)std::floor(x);
-                    f = f < b8[a] ? b8[a] : (f > b8[a] + 7 ? b8[a] + 7 : f);
+                    f = f < lo ? lo : (f > hi - 1 ? hi - 1 : f);
                     nv[a] = f;
                 }
             }
@@ -1073,7 +1113,7 @@ int orc_lbvh_build(uint32_t n, const float* records, uint32_t nvol, void* nodes_
 }
 
 // Dense -> 8^3 bricks in the product's layout: brick map (u32 per brick, x-fastest brick
-// order, 0xFFFFFFFF = empty, others numbered in brick order), occupancy masks (64 B per
+// order, 0xFFFFFF00 | radius = empty, others numbered in brick order), occupancy masks (64 B per
 // non-empty brick, bit x + 8y + 64z of the brick), payload (u8 index or RGBA8 per voxel,
 // voxel x + 8y + 64z).  Voxel v of the brick volume holds the texel the reference samples
 // for v (nearest_texel remap).  Returns the non-empty brick count.
@@ -1086,7 +1126,7 @@ uint32_t orc_brick(const orc_volume* v, uint32_t* map_out, uint64_t* masks_out, 
     for (uint32_t b = 0; b < nb; ++b) {
         uint32_t bx = b % V.bdim[0], by = (b / V.bdim[0]) % V.bdim[1], bz = b / (V.bdim[0] * V.bdim[1]);
         if (!V.brick_nonempty[b]) {
-            if (map_out) map_out[b] = kMiss;
+            if (map_out) map_out[b] = kEmptyBrick | V.brick_radius[b];
             continue;
         }
         uint32_t id = next++;

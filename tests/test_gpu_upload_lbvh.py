@@ -17,7 +17,7 @@ def _say(*a):
 
 
 @pytest.mark.parametrize("dims,fmt,fill", [((64, 64, 64), 0, 0.3), ((50, 40, 30), 0, 0.1), ((33, 17, 9), 1, 0.4),
-                                            ((8, 8, 8), 0, 0.0), ((24, 24, 24), 1, 1.0)])
+                                            ((8, 8, 8), 0, 0.0), ((24, 24, 24), 1, 1.0), ((200, 96, 128), 0, 0.02)])
 def test_brick_roundtrip(dims, fmt, fill):
     _say(f"case: brick dims={dims} fmt={fmt} fill={fill}")
     r = R.reset_renderer()
@@ -28,7 +28,9 @@ def test_brick_roundtrip(dims, fmt, fill):
     info = r.volume_info(tid)
     dense = r.volume_dense(tid)
     bmap, masks, payload = O.brick(vol)
-    assert info["nbricks"] == int((bmap != 0xFFFFFFFF).sum())
+    assert info["nbricks"] == int((bmap < 0xFFFFFF00).sum())
+    # brick map bit-exact: ids of non-empty bricks and the empty-space radius of empty ones
+    assert np.array_equal(r.brick_map(tid), bmap)
     # voxel order after the nearest-texel remap; for these sizes the remap is the identity
     # except when size is not a power of two; compare against the oracle's remapped volume
     dz, dy, dx = vol.data.shape

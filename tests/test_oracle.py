@@ -176,7 +176,7 @@ def test_brick_roundtrip_and_popcount(dims, fmt):
     recon = np.zeros((bdz * 8, bdy * 8, bdx * 8), dtype=vol.data.dtype)
     for b, e in enumerate(bmap):
         bx, by, bz = b % bdx, (b // bdx) % bdy, b // (bdx * bdy)
-        if e == 0xFFFFFFFF:
+        if e >= 0xFFFFFF00:
             continue
         m = masks[e]
         for z in range(8):
@@ -193,8 +193,36 @@ def test_brick_roundtrip_and_popcount(dims, fmt):
     full = np.zeros((bdz * 8, bdy * 8, bdx * 8), dtype=bool)
     full[:dz, :dy, :dx] = occ
     blocks = full.reshape(bdz, 8, bdy, 8, bdx, 8).any(axis=(1, 3, 5)).reshape(-1)
-    assert np.array_equal(bmap != 0xFFFFFFFF, blocks)
+    assert np.array_equal(bmap < 0xFFFFFF00, blocks)
     _ = bpv
+
+
+def _radius_brute(nonempty):
+    """Empty-space radius by brute force: L-inf brick distance to the nearest non-empty brick
+    minus 1, capped at 31 (32 - 1); 0 for non-empty bricks."""
+    nz = np.argwhere(nonempty)
+    out = np.zeros(nonempty.shape, dtype=np.int64)
+    for b in np.argwhere(~nonempty):
+        d = 32 if len(nz) == 0 else min(32, int(np.abs(nz - b).max(axis=1).min()))
+        out[tuple(b)] = d - 1
+    return out
+
+
+@pytest.mark.parametrize("dims,fill,boxes", [((64, 64, 64), 0.03, None), ((96, 40, 72), 0.10, None),
+                                             ((48, 48, 48), 0.0, [[20, 21, 22, 21, 22, 23]]),
+                                             ((40, 24, 16), 0.0, None)])
+def test_brick_radius_matches_brute_force(dims, fill, boxes):
+    idx = scenes.gen_volume(dims, 77, fill, boxes=boxes) if fill > 0 or boxes else np.zeros(dims[::-1], np.uint8)
+    vol = scenes.Volume(idx, scenes.gen_palette(77))
+    bmap, _, _ = O.brick(vol)
+    bd = [(d + 7) // 8 for d in dims]
+    m = bmap.reshape(bd[2], bd[1], bd[0])
+    nonempty = m < 0xFFFFFF00
+    rad = np.where(nonempty, 0, (m & 0xFF).astype(np.int64))
+    want = _radius_brute(nonempty)
+    assert np.array_equal(rad, want)
+    if fill == 0.03:
+        assert rad.max() >= 2  # sparse volumes exercise cubes of more than one brick
 
 
 # ---- DDA against brute force -----------------------------------------------------------------------
@@ -258,6 +286,48 @@ def test_dda_first_hit_matches_brute_force(level):
                 assert hit and v == tuple(int(c) for c in want), (n, p, d, v, want)
             checked += 1
     assert checked > 500
+
+
+def test_secondary_dda_crosses_empty_cubes_like_brute_force():
+    """Sparse multi-brick volumes (a few small boxes): the secondary walk crosses cubes of
+    empty bricks (radius >= 1) in one step and must still find the first occupied voxel."""
+    rng = np.random.default_rng(91)
+    checked = 0
+    big = 0
+    for trial in range(12):
+        n = int(rng.choice([40, 64, 96]))
+        occ = np.zeros((n, n, n), dtype=bool)
+        for _ in range(int(rng.integers(2, 7))):
+            o = rng.integers(0, n - 4, size=3)
+            sz = rng.integers(1, 5, size=3)
+            occ[o[2]:o[2] + sz[2], o[1]:o[1] + sz[1], o[0]:o[0] + sz[0]] = True
+        idx = (occ * rng.integers(1, 256, size=occ.shape)).astype(np.uint8)
+        vol = scenes.Volume(idx, scenes.gen_palette(2))
+        bmap, _, _ = O.brick(vol)
+        big += int(((bmap >= 0xFFFFFF00) & ((bmap & 0xFF) >= 2)).any())
+        targets = np.argwhere(occ)
+        for _ in range(40):
+            p = rng.uniform(0.0, n, size=3)
+            if rng.random() < 0.7:  # aim near an occupied voxel so that most rays hit
+                t = targets[rng.integers(len(targets))][::-1] + 0.5
+                d = t - p + rng.normal(scale=1.5, size=3)
+            else:
+                d = rng.normal(size=3)
+            d /= np.linalg.norm(d)
+            v0 = np.floor(p).astype(int)
+            if occ[tuple(v0[::-1])]:
+                continue
+            p32 = p.astype(np.float32)
+            hit, v, _, _, _, _ = O.dda(vol, 1, p32, v0, d.astype(np.float32))
+            want, gap = _first_voxel_brute(occ, p32.astype(np.float64), d.astype(np.float32).astype(np.float64))
+            if gap < 1e-4:
+                continue
+            if want is None:
+                assert not hit
+            else:
+                assert hit and v == tuple(int(c) for c in want), (n, p, d, v, want)
+            checked += 1
+    assert checked > 300 and big >= 6
 
 
 def test_dda_axis_aligned_reference_quirk():

@@ -12,7 +12,7 @@ REFERENCE_EXPORTS = ("entry", "render_tick", "get_input_data_pointer", "add_text
                      "end_update_instances", "cleanup")
 EXTENSION_EXPORTS = ("vt_set_params", "vt_get_params", "vt_add_volume_indexed", "vt_read_hits", "vt_read_radiance",
                      "vt_read_bgra8", "vt_read_stats", "vt_read_stage_stats", "vt_last_frame_ms", "vt_bench_frames",
-                     "vt_bench_stages", "vt_read_volume",
+                     "vt_bench_stages", "vt_read_volume", "vt_read_brick_map",
                      "vt_volume_info", "vt_read_lbvh", "vt_instance_count", "vt_synchronize", "vt_last_error",
                      "vt_serve_init", "vt_serve_add_scene", "vt_serve_submit", "vt_serve_step", "vt_serve_read",
                      "vt_serve_status", "vt_serve_shutdown", "vt_enable_gbuffer", "vt_read_gbuffer",
@@ -81,6 +81,8 @@ def declare(lib: C.CDLL) -> None:
     lib.vt_bench_frames.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     lib.vt_read_volume.restype = C.c_int32
     lib.vt_read_volume.argtypes = [C.c_int32, C.c_void_p, C.c_uint64]
+    lib.vt_read_brick_map.restype = C.c_int32
+    lib.vt_read_brick_map.argtypes = [C.c_int32, C.c_void_p, C.c_uint64]
     lib.vt_volume_info.restype = C.c_int32
     lib.vt_volume_info.argtypes = [C.c_int32, C.c_void_p]
     lib.vt_read_lbvh.restype = C.c_int32

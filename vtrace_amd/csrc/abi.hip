@@ -749,6 +749,16 @@ int32_t vt_read_volume(int32_t id, void* dst, uint64_t bytes) {
     return rc;
 }
 
+int32_t vt_read_brick_map(int32_t id, uint32_t* dst, uint64_t bytes) {
+    Guard G;
+    if (!g.inited || id < 0 || (size_t)id >= g.vols.size() || !dst) return -1;
+    const VolumeStore& v = g.vols[id];
+    const size_t need = (size_t)v.bdim[0] * v.bdim[1] * v.bdim[2] * 4;
+    if (bytes < need) return -1;
+    hipMemcpyAsync(dst, v.brick_map, need, hipMemcpyDeviceToHost, g.stream);
+    return hipStreamSynchronize(g.stream) == hipSuccess ? 0 : -1;
+}
+
 int32_t vt_read_lbvh(void* nodes, uint64_t node_bytes, void* instances, uint64_t inst_bytes) {
     Guard G;
     if (!g.inited || g.inst_count == 0) return -1;

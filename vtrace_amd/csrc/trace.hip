@@ -23,7 +23,7 @@
 // an atomic counter, so terminated paths never occupy lanes.  Each path's arithmetic is
 // independent of queue order, so results are deterministic.
 //
-// DDA: empty 8^3 bricks (brick map entry 0xFFFFFFFF) are crossed by a tight loop without
+// DDA: empty 8^3 bricks (brick map entry VT_EMPTY_BRICK | radius) are crossed by a tight loop without
 // memory access; in non-empty bricks the 64 B occupancy mask is staged once into the
 // thread's LDS slot and tested per step from LDS.  The LBVH stack lives in LDS as well.
 //
@@ -235,8 +235,8 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
         // extent of this brick inside the volume, per axis (the loop stays in brick and volume)
         const uint32_t ex = (uint32_t)min(8, nx - bx8), ey = (uint32_t)min(8, ny - by8), ez = (uint32_t)min(8, nz - bz8);
         entry = gld(V.brick_map + (uint32_t)(bx8 >> 3) + bdx * ((uint32_t)(by8 >> 3) + bdy * (uint32_t)(bz8 >> 3)));
-        ctr.brick(entry != VT_EMPTY);
-        if (entry == VT_EMPTY) {
+        ctr.brick(!vt_brick_empty(entry));
+        if (vt_brick_empty(entry)) {
             do {
                 VT_DDA_STEP();
             } while (steps < max_steps && (uint32_t)(vx - bx8) < ex && (uint32_t)(vy - by8) < ey &&
@@ -279,9 +279,10 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
 }
 
 // Two-level DDA for secondary rays (no reference counterpart; this project's rule).
-// Empty 8^3 bricks are crossed in one step: the ray moves from its current position q to
-// the nearest exit plane of the brick (ties: every axis at that distance), the exit axes
-// step into the neighbouring brick and the other axes take floor(q) clamped to the brick.
+// An empty 8^3 brick of radius r is crossed together with the cube of (2r+1)^3 empty bricks
+// around it in one step: the ray moves from its current position q to the nearest exit plane
+// of that cube (ties: every axis at that distance), the exit axes step into the voxel beyond
+// it and the other axes take floor(q) clamped to the cube.
 // Inside a non-empty brick a voxel walk from q (trace.frag's side/delta formulas, steps
 // with selects so zero direction components never produce NaN) tests occupancy from the
 // LDS-staged mask until it hits or leaves the brick; the position where it leaves is
@@ -313,8 +314,10 @@ __device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], 
         if (!((uint32_t)v[0] < (uint32_t)n[0] && (uint32_t)v[1] < (uint32_t)n[1] && (uint32_t)v[2] < (uint32_t)n[2])) break;
         const int b8[3] = {v[0] & ~7, v[1] & ~7, v[2] & ~7};
         entry = gld(V.brick_map + (uint32_t)(b8[0] >> 3) + bdx * ((uint32_t)(b8[1] >> 3) + bdy * (uint32_t)(b8[2] >> 3)));
-        ctr.brick(entry != VT_EMPTY);
-        if (entry == VT_EMPTY) {
+        ctr.brick(!vt_brick_empty(entry));
+        if (vt_brick_empty(entry)) {
+            // the cube of empty bricks of this brick's radius is crossed in one step
+            const int rad = 8 * (int)(entry & 0xFFu);
             float t[3];
             float tmin = INFINITY;
 #pragma unroll
@@ -322,7 +325,7 @@ __device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], 
                 if (st[a] == 0) {
                     t[a] = INFINITY;
                 } else {
-                    const float plane = (float)(b8[a] + (st[a] > 0 ? 8 : 0));
+                    const float plane = (float)(st[a] > 0 ? b8[a] + 8 + rad : b8[a] - rad);
                     t[a] = (plane - q[a]) * rinv[a];
                 }
                 tmin = gmin(tmin, t[a]);
@@ -330,15 +333,16 @@ __device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], 
             uint32_t mk = 0;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
+                const int lo = b8[a] - rad, hi = b8[a] + 8 + rad;
                 if (st[a] != 0 && t[a] <= tmin) {
                     mk |= 1u << a;
-                    q[a] = (float)(b8[a] + (st[a] > 0 ? 8 : 0));
-                    v[a] = st[a] > 0 ? b8[a] + 8 : b8[a] - 1;
+                    q[a] = (float)(st[a] > 0 ? hi : lo);
+                    v[a] = st[a] > 0 ? hi : lo - 1;
                 } else {
                     const float x = q[a] + tmin * u[a];
                     q[a] = x;
                     int f = (int)floorf(x);
-                    f = f < b8[a] ? b8[a] : (f > b8[a] + 7 ? b8[a] + 7 : f);
+                    f = f < lo ? lo : (f > hi - 1 ? hi - 1 : f);
                     v[a] = f;
                 }
             }

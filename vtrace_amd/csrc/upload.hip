@@ -10,8 +10,14 @@
 //   k_brick_classify  one 512-thread block per brick: occupancy bit per voxel
 //                     (alpha > 0, trace.frag:78), one ballot per z-slice -> 8 x u64 mask;
 //   k_scan_*          exclusive scan of the non-empty flags -> brick ids in brick order;
-//   k_brick_scatter   masks and payload of non-empty bricks into the pools.
+//   k_radius_*        empty-space radius of every empty brick: breadth-first layers of
+//                     26-neighbour dilation from the non-empty bricks (in place, layer k marks
+//                     the still-unreached bricks next to layer k-1);
+//   k_brick_scatter   brick map (id, or VT_EMPTY_BRICK | radius), masks and payload of
+//                     non-empty bricks into the pools.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "vt_internal.h"
 
@@ -63,14 +69,36 @@ __global__ __launch_bounds__(512) void k_brick_classify(BrickArgs a, uint64_t* m
     if (t == 0) flags[b] = any;
 }
 
+__global__ void k_radius_init(const uint32_t* flags, uint32_t* dist, uint32_t nb) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb) dist[b] = flags[b] ? 0u : 255u;
+}
+
+__global__ void k_radius_step(uint32_t* dist, uint32_t bx, uint32_t by, uint32_t bz, uint32_t k) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= bx * by * bz || dist[b] != 255u) return;
+    const int x = (int)(b % bx), y = (int)((b / bx) % by), z = (int)(b / (bx * by));
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int X = x + dx, Y = y + dy, Z = z + dz;
+                if (X < 0 || Y < 0 || Z < 0 || X >= (int)bx || Y >= (int)by || Z >= (int)bz) continue;
+                if (dist[(uint32_t)X + bx * ((uint32_t)Y + by * (uint32_t)Z)] == k - 1u) {
+                    dist[b] = k;
+                    return;
+                }
+            }
+}
+
 __global__ __launch_bounds__(512) void k_brick_scatter(BrickArgs a, const uint64_t* masks_all, const uint32_t* ids,
-                                                       const uint32_t* flags, uint32_t* brick_map, uint64_t* masks,
-                                                       uint8_t* payload) {
+                                                       const uint32_t* flags, const uint32_t* dist,
+                                                       uint32_t* brick_map, uint64_t* masks, uint8_t* payload) {
     const uint32_t b = blockIdx.x;
     const uint32_t t = threadIdx.x;
     const bool nonempty = flags[b] != 0;
     const uint32_t id = ids[b];
-    if (t == 0) brick_map[b] = nonempty ? id : VT_EMPTY;
+    if (t == 0)
+        brick_map[b] = nonempty ? id : (VT_EMPTY_BRICK | (min(dist[b], VT_BRICK_RADIUS_CAP + 1u) - 1u));
     if (!nonempty) return;
     if (t < 8) masks[(size_t)id * 8 + t] = masks_all[(size_t)b * 8 + t];
     const uint32_t bx = b % a.bdim[0];
@@ -176,16 +204,23 @@ int vt_brick_volume(const uint8_t* dense_dev, uint32_t fmt, const uint32_t* pale
     a.fmt = fmt;
     const uint32_t nb = a.bdim[0] * a.bdim[1] * a.bdim[2];
     uint64_t* masks_all = nullptr;
-    uint32_t *flags = nullptr, *ids = nullptr, *blocks = nullptr, *total = nullptr;
+    uint32_t *flags = nullptr, *ids = nullptr, *blocks = nullptr, *total = nullptr, *dist = nullptr;
     const uint32_t nblk = (nb + kScanBlock - 1) / kScanBlock + 1;
     if (hipMalloc(&masks_all, (size_t)nb * 64) != hipSuccess) return -1;
     if (hipMalloc(&flags, (size_t)nb * 4) != hipSuccess || hipMalloc(&ids, (size_t)nb * 4) != hipSuccess ||
-        hipMalloc(&blocks, (size_t)nblk * 4) != hipSuccess || hipMalloc(&total, 4) != hipSuccess) {
-        hipFree(masks_all); hipFree(flags); hipFree(ids); hipFree(blocks); hipFree(total);
+        hipMalloc(&blocks, (size_t)nblk * 4) != hipSuccess || hipMalloc(&total, 4) != hipSuccess ||
+        hipMalloc(&dist, (size_t)nb * 4) != hipSuccess) {
+        hipFree(masks_all); hipFree(flags); hipFree(ids); hipFree(blocks); hipFree(total); hipFree(dist);
         return -1;
     }
     hipLaunchKernelGGL(k_brick_classify, dim3(nb), dim3(512), 0, s, a, masks_all, flags);
     vt_scan_flags(flags, nb, ids, blocks, total, s);
+    hipLaunchKernelGGL(k_radius_init, dim3((nb + 255) / 256), dim3(256), 0, s, flags, dist, nb);
+    const uint32_t layers = std::min<uint32_t>(VT_BRICK_RADIUS_CAP + 1u,
+                                               std::max(a.bdim[0], std::max(a.bdim[1], a.bdim[2])));
+    for (uint32_t k = 1; k <= layers; ++k)
+        hipLaunchKernelGGL(k_radius_step, dim3((nb + 255) / 256), dim3(256), 0, s, dist, a.bdim[0], a.bdim[1],
+                           a.bdim[2], k);
     uint32_t count = 0;
     hipMemcpyAsync(&count, total, 4, hipMemcpyDeviceToHost, s);
     hipError_t e = hipStreamSynchronize(s);
@@ -203,13 +238,13 @@ int vt_brick_volume(const uint8_t* dense_dev, uint32_t fmt, const uint32_t* pale
             rc = -1;
     }
     if (rc == 0) {
-        hipLaunchKernelGGL(k_brick_scatter, dim3(nb), dim3(512), 0, s, a, masks_all, ids, flags, out->brick_map,
+        hipLaunchKernelGGL(k_brick_scatter, dim3(nb), dim3(512), 0, s, a, masks_all, ids, flags, dist, out->brick_map,
                            reinterpret_cast<uint64_t*>(out->masks), out->payload);
         if (hipGetLastError() != hipSuccess) rc = -1;
         if (hipStreamSynchronize(s) != hipSuccess) rc = -1;
         out->bytes = (size_t)nb * 4 + (size_t)(count ? count : 1) * (64 + 512 * bpv);
     }
-    hipFree(masks_all); hipFree(flags); hipFree(ids); hipFree(blocks); hipFree(total);
+    hipFree(masks_all); hipFree(flags); hipFree(ids); hipFree(blocks); hipFree(total); hipFree(dist);
     return rc;
 }
 
@@ -224,8 +259,8 @@ __global__ void k_debrick(VolumeDesc v, uint8_t* dense) {
     uint32_t b = (x >> 3) + v.bdim[0] * ((y >> 3) + v.bdim[1] * (z >> 3));
     uint32_t e = v.brick_map[b];
     uint32_t li = (x & 7) + 8 * (y & 7) + 64 * (z & 7);
-    if (v.fmt == 0) dense[i] = e == VT_EMPTY ? 0 : v.payload[(size_t)e * 512 + li];
-    else reinterpret_cast<uint32_t*>(dense)[i] = e == VT_EMPTY ? 0u : reinterpret_cast<const uint32_t*>(v.payload)[(size_t)e * 512 + li];
+    if (v.fmt == 0) dense[i] = vt_brick_empty(e) ? 0 : v.payload[(size_t)e * 512 + li];
+    else reinterpret_cast<uint32_t*>(dense)[i] = vt_brick_empty(e) ? 0u : reinterpret_cast<const uint32_t*>(v.payload)[(size_t)e * 512 + li];
 }
 }  // namespace
 

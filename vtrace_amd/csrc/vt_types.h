@@ -2,7 +2,7 @@
 //
 // HBM layout (one GPU):
 //   * volumes: per volume a brick map (u32 per 8^3 brick, x-fastest brick order,
-//     0xFFFFFFFF = empty), an occupancy-mask pool (64 B = 16 u32 per non-empty brick,
+//     VT_EMPTY_BRICK | radius = empty), an occupancy-mask pool (64 B = 16 u32 per non-empty brick,
 //     bit x + 8y + 64z of the brick) and a payload pool (512 B u8 palette indices or
 //     2048 B RGBA8 per non-empty brick, voxel x + 8y + 64z), plus a 256-entry palette.
 //     Replaces the reference's R8G8B8A8_SRGB 3-D images (lib/memory.c:286-385).
@@ -18,6 +18,15 @@
 #include <stdint.h>
 
 #define VT_EMPTY 0xFFFFFFFFu
+// Brick map entries: a non-empty brick holds its id; an empty brick holds VT_EMPTY_BRICK | r,
+// r = radius of the cube of empty bricks around it (L-inf brick distance to the nearest
+// non-empty brick - 1, at most VT_BRICK_RADIUS_CAP; bricks outside the volume count as empty).
+#define VT_EMPTY_BRICK 0xFFFFFF00u
+#define VT_BRICK_RADIUS_CAP 31u
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline bool vt_brick_empty(uint32_t e) { return e >= VT_EMPTY_BRICK; }
 
 struct VolumeDesc {             // 64 B
     uint32_t dim[3];

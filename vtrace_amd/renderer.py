@@ -287,6 +287,14 @@ class Renderer:
         return {"dims": tuple(int(x) for x in info[:3]), "fmt": int(info[3]), "nbricks": int(info[4]),
                 "bytes": int(info[5]) | (int(info[6]) << 32)}
 
+    def brick_map(self, tid: int) -> np.ndarray:
+        info = self.volume_info(tid)
+        bd = [(d + 7) // 8 for d in info["dims"]]
+        out = np.zeros(bd[0] * bd[1] * bd[2], dtype=np.uint32)
+        if self.lib.vt_read_brick_map(tid, out.ctypes.data, out.nbytes) != 0:
+            raise RendererError("vt_read_brick_map failed")
+        return out
+
     def lbvh(self):
         n = self.lib.vt_instance_count()
         nodes = np.zeros(max(1, 2 * n - 1), dtype=NODE_DTYPE)
