@@ -11,10 +11,11 @@ from vtrace_amd import _native, abi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "vtrace.h")
+SCENE_HEADER = os.path.join(ROOT, "include", "vtscene.h")
 
 
-def header_functions():
-    text = open(HEADER).read()
+def header_functions(path=HEADER):
+    text = open(path).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text, flags=re.M)
     return sorted(set(n for n in names if n not in ("if", "while")))
@@ -49,3 +50,11 @@ def test_python_binding_declares_all():
     lib = _native.render_lib()
     for n in abi.REFERENCE_EXPORTS + abi.EXTENSION_EXPORTS:
         assert getattr(lib, n).restype is not None or n in ("cleanup", "vt_serve_shutdown", "vt_dn_shutdown")
+
+
+def test_scene_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_native.SCENE_LIB)
+    names = header_functions(SCENE_HEADER)
+    assert "vt_vox_model" in names and "vt_gen_volume" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing

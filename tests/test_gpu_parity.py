@@ -177,3 +177,29 @@ def test_gbuffer_consistent_with_hits():
         assert abs(v[a] - want) < 0.05, (y, x, v, vox, face)
         others = [k for k in range(3) if k != a]
         assert np.all(v[others] > vox[others] - 0.05) and np.all(v[others] < vox[others] + 1.05)
+
+
+def test_vox_model_renders_like_oracle():
+    """A model read from .vox bytes (reference layout and palette) through the plain path."""
+    import struct
+    from vtrace_amd import vox
+    rng = np.random.default_rng(8)
+    D = 24
+    cells = {tuple(int(c) for c in rng.integers(0, D, 3)) for _ in range(2500)}
+    voxels = b"".join(struct.pack("<BBBB", x, y, z, int(rng.integers(1, 256))) for (x, y, z) in cells)
+    pal = b"".join(struct.pack("<I", int(c) | (255 << 24)) for c in rng.integers(0, 2**24, 256))
+    def chunk(cid, content, kids=b""):
+        return cid + struct.pack("<II", len(content), len(kids)) + content + kids
+    data = b"VOX " + struct.pack("<I", 150) + chunk(b"MAIN", b"", chunk(b"SIZE", struct.pack("<iii", D, D, D)) +
+                                                   chunk(b"XYZI", struct.pack("<I", len(cells)) + voxels) +
+                                                   chunk(b"RGBA", pal))
+    (vol,) = vox.parse(data)
+    sc = scenes.single_volume_scene(D, 0.3, 5, 160, 120, mode=1, name="vox24")
+    sc.volumes = [vol]
+    _say("case: .vox model 24^3 mode1 160x120")
+    g_hits, g_rad, _ = gpu_render(sc)
+    o_hits, o_rad, _ = O.render(sc)
+    assert_hits_equal(g_hits, o_hits, "vox")
+    mx, _ = radiance_error(g_rad, o_rad)
+    assert mx <= REL_TOL
+    assert (g_hits["inst"] != 0xFFFFFFFF).mean() > 0.05

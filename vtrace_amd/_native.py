@@ -50,6 +50,10 @@ def scene_lib() -> ctypes.CDLL:
         lib.vt_gen_palette.argtypes = [c.c_uint32, c.c_int, c.c_void_p]
         lib.vt_expand_palette.restype = None
         lib.vt_expand_palette.argtypes = [c.c_void_p, c.c_size_t, c.c_void_p, c.c_void_p]
+        lib.vt_vox_models.restype = c.c_int32
+        lib.vt_vox_models.argtypes = [c.c_void_p, c.c_uint64]
+        lib.vt_vox_model.restype = c.c_int32
+        lib.vt_vox_model.argtypes = [c.c_void_p, c.c_uint64, c.c_uint32, c.c_void_p, c.c_void_p, c.c_void_p]
         lib._vt_typed = True
     return lib
 
