@@ -203,3 +203,24 @@ def test_vox_model_renders_like_oracle():
     mx, _ = radiance_error(g_rad, o_rad)
     assert mx <= REL_TOL
     assert (g_hits["inst"] != 0xFFFFFFFF).mean() > 0.05
+
+
+def test_bgra8_encode_matches_oracle():
+    """The swapchain format (B8G8R8A8_SRGB, lib/swapchain.c:88): every pixel's bytes from the
+    GPU encode of the frame equal the oracle's encode of its own (bit-identical) colour."""
+    from vtrace_amd import renderer as R
+    sc = scenes.config1(320, 240)
+    p = copy.copy(sc.params)
+    p.mode = 0
+    sc.params = p
+    r = R.reset_renderer()
+    R.load_scene(r, sc)
+    _say("case: bgra8 encode 320x240 mode0")
+    r.render(sc.P, sc.C)
+    rad, bgra = r.radiance(), r.bgra8()
+    enc = np.vectorize(lambda v: O.lib().orc_srgb_encode(float(v)), otypes=[np.uint32])
+    b, g, rr = enc(rad[..., 2]), enc(rad[..., 1]), enc(rad[..., 0])
+    a = np.clip(np.floor(rad[..., 3].astype(np.float32) * np.float32(255.0) + np.float32(0.5)), 0, 255).astype(np.uint32)
+    want = b | (g << 8) | (rr << 16) | (a << 24)
+    assert np.array_equal(bgra, want)
+    assert len(np.unique(bgra)) > 50

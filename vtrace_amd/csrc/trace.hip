@@ -32,6 +32,8 @@
 // (instance, voxel, face, palette index, steps) bit-exact with the CPU oracle.
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include <cstring>
 
 #include "vt_internal.h"
@@ -189,58 +191,75 @@ struct Dda {
     bool hit;
 };
 
-template <bool ST>
-__device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3], const int v0[3], V3 mdir, Dda& r,
-                                        Ctr<ST>& ctr, const Lds& mslot) {
-    int vx = v0[0], vy = v0[1], vz = v0[2];
-    const float len = sqrtf((mdir.x * mdir.x + mdir.y * mdir.y) + mdir.z * mdir.z);
-    const float sgx = gsign(mdir.x), sgy = gsign(mdir.y), sgz = gsign(mdir.z);
-    const int stx = (int)sgx, sty = (int)sgy, stz = (int)sgz;
-    const float dx = fabsf(len / mdir.x), dy = fabsf(len / mdir.y), dz = fabsf(len / mdir.z);
-    // float(mask) * delta of trace.frag:84 as a select: delta, or 0 * delta (0, or NaN for inf)
-    const float zx = 0.0f * dx, zy = 0.0f * dy, zz = 0.0f * dz;
-    float sx = ((sgx * ((float)vx - mpos[0]) + sgx * 0.5f) + 0.5f) * dx;
-    float sy = ((sgy * ((float)vy - mpos[1]) + sgy * 0.5f) + 0.5f) * dy;
-    float sz = ((sgz * ((float)vz - mpos[2]) + sgz * 0.5f) + 0.5f) * dz;
+// State of one reference DDA walk (trace.frag:66-87).
+struct DdaWalk {
+    int vx, vy, vz;
+    float sx, sy, sz;            // side distances
+    float dx, dy, dz;            // deltas
+    float zx, zy, zz;            // 0 * delta: what float(mask) * delta adds for an axis not stepped
+    int stx, sty, stz;
+    uint32_t steps, entry, li;
+    float t_last;
+    bool hit, lmx, lmy, lmz;     // axes of the last step (the mask, built once at the end)
+};
+
+// The walk over the bricked volume.  FAST: every delta is finite, so 0 * delta is 0, no side
+// distance can become NaN and GLSL's min(y, x) = (y < x ? y : x) in "side <= min(a, b)" is two
+// comparisons; otherwise the selects reproduce min's NaN behaviour exactly.  A brick is left
+// through one voxel per stepping axis (the first voxel outside it on that side), which is
+// what the loops test.
+template <bool ST, bool FAST>
+__device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST>& ctr, const Lds& mslot) {
     const int nx = (int)V.dim[0], ny = (int)V.dim[1], nz = (int)V.dim[2];
     const uint32_t max_steps = V.dim[0] + V.dim[1] + V.dim[2];
     const uint32_t bdx = V.bdim[0], bdy = V.bdim[1];
+    int vx = w.vx, vy = w.vy, vz = w.vz;
+    float sx = w.sx, sy = w.sy, sz = w.sz;
+    const float dx = w.dx, dy = w.dy, dz = w.dz, zx = w.zx, zy = w.zy, zz = w.zz;
+    const int stx = w.stx, sty = w.sty, stz = w.stz;
     uint32_t steps = 0, entry = VT_EMPTY, li = 0;
     float t_last = 0.0f;
-    bool hit = false;
-    bool lmx = false, lmy = false, lmz = false;  // axes of the last step (the mask, built once at the end)
+    bool hit = false, lmx = false, lmy = false, lmz = false;
 // One DDA step (trace.frag:83-86): axes whose side distance is the minimum step together;
 // float(mask) * delta is the select between delta and 0 * delta.
-#define VT_DDA_STEP()                                             \
-    {                                                             \
-        const bool mx = sx <= gmin(sy, sz);                       \
-        const bool my = sy <= gmin(sz, sx);                       \
-        const bool mz = sz <= gmin(sx, sy);                       \
-        t_last = mx ? sx : (my ? sy : sz);                        \
-        sx += mx ? dx : zx;                                       \
-        sy += my ? dy : zy;                                       \
-        sz += mz ? dz : zz;                                       \
-        vx += mx ? stx : 0;                                       \
-        vy += my ? sty : 0;                                       \
-        vz += mz ? stz : 0;                                       \
-        lmx = mx;                                                 \
-        lmy = my;                                                 \
-        lmz = mz;                                                 \
-        ++steps;                                                  \
+#define VT_DDA_STEP()                                                                   \
+    {                                                                                   \
+        bool mx, my, mz;                                                                \
+        if (FAST) {                                                                     \
+            mx = sx <= sy && sx <= sz;                                                  \
+            my = sy <= sz && sy <= sx;                                                  \
+            mz = sz <= sx && sz <= sy;                                                  \
+        } else {                                                                        \
+            mx = sx <= gmin(sy, sz);                                                    \
+            my = sy <= gmin(sz, sx);                                                    \
+            mz = sz <= gmin(sx, sy);                                                    \
+        }                                                                               \
+        t_last = mx ? sx : (my ? sy : sz);                                              \
+        sx += mx ? dx : zx;                                                             \
+        sy += my ? dy : zy;                                                             \
+        sz += mz ? dz : zz;                                                             \
+        vx += mx ? stx : 0;                                                             \
+        vy += my ? sty : 0;                                                             \
+        vz += mz ? stz : 0;                                                             \
+        lmx = mx;                                                                       \
+        lmy = my;                                                                       \
+        lmz = mz;                                                                       \
+        ++steps;                                                                        \
     }
     while (steps < max_steps && (uint32_t)vx < (uint32_t)nx && (uint32_t)vy < (uint32_t)ny &&
            (uint32_t)vz < (uint32_t)nz) {
         // entering a brick: the current voxel is inside the volume
         const int bx8 = vx & ~7, by8 = vy & ~7, bz8 = vz & ~7;
-        // extent of this brick inside the volume, per axis (the loop stays in brick and volume)
-        const uint32_t ex = (uint32_t)min(8, nx - bx8), ey = (uint32_t)min(8, ny - by8), ez = (uint32_t)min(8, nz - bz8);
+        // the voxel through which each stepping axis leaves this brick (or the volume)
+        const int ox = stx > 0 ? min(bx8 + 8, nx) : (stx < 0 ? bx8 - 1 : INT_MIN);
+        const int oy = sty > 0 ? min(by8 + 8, ny) : (sty < 0 ? by8 - 1 : INT_MIN);
+        const int oz = stz > 0 ? min(bz8 + 8, nz) : (stz < 0 ? bz8 - 1 : INT_MIN);
         entry = gld(V.brick_map + (uint32_t)(bx8 >> 3) + bdx * ((uint32_t)(by8 >> 3) + bdy * (uint32_t)(bz8 >> 3)));
         ctr.brick(!vt_brick_empty(entry));
         if (vt_brick_empty(entry)) {
             do {
                 VT_DDA_STEP();
-            } while (steps < max_steps && (uint32_t)(vx - bx8) < ex && (uint32_t)(vy - by8) < ey &&
-                     (uint32_t)(vz - bz8) < ez);
+            } while (steps < max_steps && vx != ox && vy != oy && vz != oz);
             continue;
         }
         stage_mask(V.masks, entry, mslot);
@@ -259,23 +278,44 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
                 break;
             }
             VT_DDA_STEP();
-            if (!(steps < max_steps && (uint32_t)(vx - bx8) < ex && (uint32_t)(vy - by8) < ey &&
-                  (uint32_t)(vz - bz8) < ez))
-                break;
+            if (!(steps < max_steps && vx != ox && vy != oy && vz != oz)) break;
         }
         if (hit) break;
     }
 #undef VT_DDA_STEP
-    ctr.steps(steps);
-    const uint32_t mask = (lmx ? 1u : 0u) | (lmy ? 2u : 0u) | (lmz ? 4u : 0u);
-    r.v[0] = vx; r.v[1] = vy; r.v[2] = vz;
-    r.steps = steps;
-    r.mask = hit && steps > 0 ? mask : 0u;
-    r.t_last = t_last;
+    w.vx = vx; w.vy = vy; w.vz = vz;
+    w.steps = steps; w.entry = entry; w.li = li; w.t_last = t_last;
+    w.hit = hit; w.lmx = lmx; w.lmy = lmy; w.lmz = lmz;
+}
+
+template <bool ST>
+__device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3], const int v0[3], V3 mdir, Dda& r,
+                                        Ctr<ST>& ctr, const Lds& mslot) {
+    DdaWalk w;
+    w.vx = v0[0]; w.vy = v0[1]; w.vz = v0[2];
+    const float len = sqrtf((mdir.x * mdir.x + mdir.y * mdir.y) + mdir.z * mdir.z);
+    const float sgx = gsign(mdir.x), sgy = gsign(mdir.y), sgz = gsign(mdir.z);
+    w.stx = (int)sgx; w.sty = (int)sgy; w.stz = (int)sgz;
+    w.dx = fabsf(len / mdir.x); w.dy = fabsf(len / mdir.y); w.dz = fabsf(len / mdir.z);
+    // float(mask) * delta of trace.frag:84 as a select: delta, or 0 * delta (0, or NaN for inf)
+    w.zx = 0.0f * w.dx; w.zy = 0.0f * w.dy; w.zz = 0.0f * w.dz;
+    w.sx = ((sgx * ((float)w.vx - mpos[0]) + sgx * 0.5f) + 0.5f) * w.dx;
+    w.sy = ((sgy * ((float)w.vy - mpos[1]) + sgy * 0.5f) + 0.5f) * w.dy;
+    w.sz = ((sgz * ((float)w.vz - mpos[2]) + sgz * 0.5f) + 0.5f) * w.dz;
+    if (isfinite(w.dx) && isfinite(w.dy) && isfinite(w.dz) && isfinite(w.sx) && isfinite(w.sy) && isfinite(w.sz))
+        dda_walk<ST, true>(V, w, ctr, mslot);
+    else
+        dda_walk<ST, false>(V, w, ctr, mslot);
+    ctr.steps(w.steps);
+    const uint32_t mask = (w.lmx ? 1u : 0u) | (w.lmy ? 2u : 0u) | (w.lmz ? 4u : 0u);
+    r.v[0] = w.vx; r.v[1] = w.vy; r.v[2] = w.vz;
+    r.steps = w.steps;
+    r.mask = w.hit && w.steps > 0 ? mask : 0u;
+    r.t_last = w.t_last;
     r.q[0] = mpos[0]; r.q[1] = mpos[1]; r.q[2] = mpos[2];
-    r.entry = entry;
-    r.li = li;
-    r.hit = hit;
+    r.entry = w.entry;
+    r.li = w.li;
+    r.hit = w.hit;
 }
 
 // Two-level DDA for secondary rays (no reference counterpart; this project's rule).
@@ -287,9 +327,11 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
 // with selects so zero direction components never produce NaN) tests occupancy from the
 // LDS-staged mask until it hits or leaves the brick; the position where it leaves is
 // q + t * unit dir with the exit axes snapped to the voxel boundary.
-template <bool ST>
-__device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], const int v0[3], V3 m, Dda& r,
-                                         Ctr<ST>& ctr, const Lds& mslot) {
+// FAST: every delta is finite, so no side distance is NaN and "side <= gmin(a, b)" is two
+// comparisons (otherwise the selects keep GLSL's min semantics for NaN sides).
+template <bool ST, bool FAST>
+__device__ __forceinline__ void dda2_walk(const VolumeDesc& V, const float p[3], const int v0[3], V3 m, Dda& r,
+                                          Ctr<ST>& ctr, const Lds& mslot) {
     const float len = sqrtf((m.x * m.x + m.y * m.y) + m.z * m.z);
     const float mm[3] = {m.x, m.y, m.z};
     float sg[3], u[3], dl[3], rinv[3];
@@ -353,6 +395,10 @@ __device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], 
         stage_mask(V.masks, entry, mslot);
         int cur_z = -1;
         uint64_t slice = 0;
+        // the voxel through which each stepping axis leaves this brick (or the volume)
+        int out[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) out[a] = st[a] > 0 ? min(b8[a] + 8, n[a]) : (st[a] < 0 ? b8[a] - 1 : INT_MIN);
         float sx = ((sg[0] * ((float)v[0] - q[0]) + sg[0] * 0.5f) + 0.5f) * dl[0];
         float sy = ((sg[1] * ((float)v[1] - q[1]) + sg[1] * 0.5f) + 0.5f) * dl[1];
         float sz = ((sg[2] * ((float)v[2] - q[2]) + sg[2] * 0.5f) + 0.5f) * dl[2];
@@ -370,9 +416,16 @@ __device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], 
                 t_last = tl;
                 break;
             }
-            const bool mx = sx <= gmin(sy, sz);
-            const bool my = sy <= gmin(sz, sx);
-            const bool mz = sz <= gmin(sx, sy);
+            bool mx, my, mz;
+            if (FAST) {
+                mx = sx <= sy && sx <= sz;
+                my = sy <= sz && sy <= sx;
+                mz = sz <= sx && sz <= sy;
+            } else {
+                mx = sx <= gmin(sy, sz);
+                my = sy <= gmin(sz, sx);
+                mz = sz <= gmin(sx, sy);
+            }
             tl = mx ? sx : (my ? sy : sz);
             sx = mx ? sx + dl[0] : sx;
             sy = my ? sy + dl[1] : sy;
@@ -382,9 +435,7 @@ __device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], 
             v[2] += mz ? st[2] : 0;
             mask = (mx ? 1u : 0u) | (my ? 2u : 0u) | (mz ? 4u : 0u);
             ++steps;
-            if ((uint32_t)(v[0] - b8[0]) >= 8u || (uint32_t)(v[1] - b8[1]) >= 8u || (uint32_t)(v[2] - b8[2]) >= 8u ||
-                v[0] >= n[0] || v[1] >= n[1] || v[2] >= n[2])
-                break;
+            if (v[0] == out[0] || v[1] == out[1] || v[2] == out[2]) break;
         }
         if (hit) break;
 #pragma unroll
@@ -407,6 +458,16 @@ __device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], 
     r.entry = entry;
     r.li = li;
     r.hit = hit;
+}
+
+template <bool ST>
+__device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], const int v0[3], V3 m, Dda& r,
+                                         Ctr<ST>& ctr, const Lds& mslot) {
+    const float len = sqrtf((m.x * m.x + m.y * m.y) + m.z * m.z);
+    if (isfinite(fabsf(len / m.x)) && isfinite(fabsf(len / m.y)) && isfinite(fabsf(len / m.z)))
+        dda2_walk<ST, true>(V, p, v0, m, r, ctr, mslot);
+    else
+        dda2_walk<ST, false>(V, p, v0, m, r, ctr, mslot);
 }
 
 struct HitG {
