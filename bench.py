@@ -31,6 +31,11 @@ denoiser: a 5-level BF16 conv U-Net trained on the GPU (own kernels) on (1..256 
 pairs of other seeded scenes of the same kind; device time to reach relMSE <= 1e-3 against a
 4096-spp render of the benchmark frame, denoised vs plain.
 
+streaming: the terrain world of src/world.rs (16^3 chunks paged around the camera): chunk
+upload throughput one add_texture per chunk vs one vt_add_textures batch, a 61-tick camera
+flight at the reference's one upload per tick vs draining the queue each tick, and
+end_update_instances for an unchanged vs a changed list.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
@@ -122,6 +127,7 @@ def main():
                     help="requests in the serving stream (0 skips the serving measurement)")
     ap.add_argument("--denoiser-iters", type=int, default=8000,
                     help="training iterations of the denoiser (0 skips the denoiser measurement)")
+    ap.add_argument("--streaming", type=int, default=1, help="1: streamed terrain measurement, 0: skip it")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the renderer has no multi-GPU path); got --gpus {args.gpus}",
@@ -229,6 +235,10 @@ def main():
         from vtrace_amd import denoise
         out["denoiser"] = denoise.benchmark(r, sc, iters=args.denoiser_iters,
                                             log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+    if args.streaming:
+        from vtrace_amd import world
+        out["streaming"] = world.benchmark(r, args.width, args.height,
+                                           log=lambda m: print("# " + m, file=sys.stderr, flush=True))
     print(json.dumps(out), flush=True)
 
 

@@ -64,7 +64,8 @@ int32_t add_texture(const uint8_t* data, uint32_t width, uint32_t height, uint32
 float* start_update_instances(uint32_t instance_count);
 
 /* Replaces lib/memory.c:250-267.  Commits instance_count records (0 -> 1) and rebuilds the
- * LBVH on the device.  0 or -1. */
+ * LBVH on the device; when the records equal those of the last build and no texture was
+ * added since, the resident tables and tree are kept (no copy, no rebuild).  0 or -1. */
 int32_t end_update_instances(uint32_t instance_count);
 
 /* ---------------- headless extensions (no reference counterpart) ---------------------- */
@@ -86,6 +87,18 @@ typedef struct vt_render_params {
 
 int32_t vt_set_params(const vt_render_params* params);
 int32_t vt_get_params(vt_render_params* params);
+
+/* Batched add_texture for streamed chunks (src/render.rs:246-268 uploads one texture per
+ * render_tick; src/gen/pager.rs:52-78 queues a 16^3 chunk per newly paged position): count
+ * RGBA8 volumes of one size width x height x depth, packed one after another (each x fastest,
+ * width*height*depth*4 bytes), copied and bricked together (one copy, one set of launches,
+ * shared pools).  Each volume is laid out exactly as add_texture lays it out alone.
+ * ids_out[k] (may be NULL) = texture id of volume k; ids are consecutive.  0, or -1 with
+ * nothing added. */
+int32_t vt_add_textures(const uint8_t* rgba8, uint32_t count, uint32_t width, uint32_t height, uint32_t depth,
+                        int32_t* ids_out);
+/* Number of LBVH builds so far (end_update_instances calls that rebuilt the tree). */
+uint64_t vt_lbvh_builds(void);
 
 /* u8 palette-index volume + 256 x RGBA8 palette (alpha > 0 = occupied).  Same id space as
  * add_texture. */

@@ -31,6 +31,16 @@ int32_t vt_vox_models(const uint8_t* buf, uint64_t n);
 int32_t vt_vox_model(const uint8_t* buf, uint64_t n, uint32_t k, uint32_t* dims3, uint8_t* indices,
                      uint32_t* palette256);
 
+/* Streamed terrain (src/gen/terrain.rs:37-94 structure; its noise-rs OpenSimplex/Billow are
+ * replaced by seeded FP64 value noise).  Colour of world voxel (x, y, z) as RGBA8, r in the low
+ * byte, 0 = empty. */
+uint32_t vt_terrain_voxel(int32_t x, int32_t y, int32_t z, uint32_t seed);
+/* Chunk (cx, cy, cz) = voxels 16c .. 16c+15 per axis into rgba (4096 u32, x fastest).  Returns 1
+ * if a voxel is occupied, 0 if the chunk is empty (terrain.rs returns None), -1 on error. */
+int32_t vt_terrain_chunk(int32_t cx, int32_t cy, int32_t cz, uint32_t seed, uint32_t* rgba);
+/* n chunks (coords = 3n ints) on host threads: rgba = n x 4096 u32, filled[i] = chunk i occupied. */
+int32_t vt_terrain_chunks(const int32_t* coords, uint32_t n, uint32_t seed, uint32_t* rgba, uint8_t* filled);
+
 #ifdef __cplusplus
 }
 #endif

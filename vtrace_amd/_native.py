@@ -54,6 +54,12 @@ def scene_lib() -> ctypes.CDLL:
         lib.vt_vox_models.argtypes = [c.c_void_p, c.c_uint64]
         lib.vt_vox_model.restype = c.c_int32
         lib.vt_vox_model.argtypes = [c.c_void_p, c.c_uint64, c.c_uint32, c.c_void_p, c.c_void_p, c.c_void_p]
+        lib.vt_terrain_voxel.restype = c.c_uint32
+        lib.vt_terrain_voxel.argtypes = [c.c_int32, c.c_int32, c.c_int32, c.c_uint32]
+        lib.vt_terrain_chunk.restype = c.c_int32
+        lib.vt_terrain_chunk.argtypes = [c.c_int32, c.c_int32, c.c_int32, c.c_uint32, c.c_void_p]
+        lib.vt_terrain_chunks.restype = c.c_int32
+        lib.vt_terrain_chunks.argtypes = [c.c_void_p, c.c_uint32, c.c_uint32, c.c_void_p, c.c_void_p]
         lib._vt_typed = True
     return lib
 

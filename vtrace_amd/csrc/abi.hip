@@ -35,6 +35,9 @@ struct State {
     uint32_t inst_count = 0;
     uint32_t bvh_depth = 0;
     hipEvent_t inst_copied = nullptr;
+    std::vector<float> inst_prev;     // records of the last build
+    size_t inst_prev_vols = 0;        // textures known at the last build (validity of the records)
+    uint64_t lbvh_builds = 0;
     LbvhWork bvh;
     // volumes (lib/memory.c:286-385)
     std::vector<VolumeStore> vols;
@@ -215,8 +218,15 @@ uint16_t nearest_texel(int v, uint32_t dim) {
     return (uint16_t)t;
 }
 
-int upload_volume(const void* data, size_t bytes, uint32_t fmt, const uint32_t dim[3], const uint32_t* palette) {
-    if (g.vols.size() >= kMaxTextures) {
+// count volumes of one size, packed one after another (bytes each), bricked in one pass;
+// ids are consecutive from the returned first id.
+int upload_volume(const void* data, size_t bytes, uint32_t fmt, const uint32_t dim[3], const uint32_t* palette,
+                  uint32_t count = 1) {
+    if (count == 0 || bytes > ((size_t)1 << 40) / count) {
+        set_err("invalid volume batch");
+        return -1;
+    }
+    if (g.vols.size() + count > kMaxTextures) {
         std::fprintf(stderr, "ERROR: Tried allocating too many textures\n");  // lib/memory.c:288
         set_err("MAX_TEXTURES reached");
         return -1;
@@ -226,9 +236,10 @@ int upload_volume(const void* data, size_t bytes, uint32_t fmt, const uint32_t d
         return -1;
     }
     hipError_t e;
-    if (bytes > g.staging_size) {  // grow to a power of two (lib/memory.c:297-302)
+    const size_t total = bytes * count;
+    if (total > g.staging_size) {  // grow to a power of two (lib/memory.c:297-302)
         size_t sz = 1;
-        while (sz < bytes) sz *= 2;
+        while (sz < total) sz *= 2;
         hipFree(g.staging);
         g.staging = nullptr;
         g.staging_size = 0;
@@ -238,7 +249,7 @@ int upload_volume(const void* data, size_t bytes, uint32_t fmt, const uint32_t d
         }
         g.staging_size = sz;
     }
-    if ((e = hipMemcpyAsync(g.staging, data, bytes, hipMemcpyHostToDevice, g.stream)) != hipSuccess) {
+    if ((e = hipMemcpyAsync(g.staging, data, total, hipMemcpyHostToDevice, g.stream)) != hipSuccess) {
         set_err("volume copy", e);
         return -1;
     }
@@ -266,26 +277,32 @@ int upload_volume(const void* data, size_t bytes, uint32_t fmt, const uint32_t d
         hipMemcpyAsync(vs.palette, palette, 1024, hipMemcpyHostToDevice, g.stream);
     }
     const uint16_t* remaps[3] = {g.remap, g.remap + dim[0], g.remap + dim[0] + dim[1]};
-    if (vt_brick_volume(g.staging, fmt, vs.palette, remaps, dim, &vs, g.stream) != 0) {
+    std::vector<VolumeStore> out(count);
+    if (vt_brick_volume(g.staging, fmt, vs.palette, remaps, dim, count, out.data(), g.stream) != 0) {
         hipFree(vs.palette);
         set_err("bricking failed");
         return -1;
     }
-    VolumeDesc d{};
-    for (int a = 0; a < 3; ++a) { d.dim[a] = vs.dim[a]; d.bdim[a] = vs.bdim[a]; }
-    d.fmt = vs.fmt;
-    d.nbricks = vs.nbricks;
-    d.brick_map = vs.brick_map;
-    d.masks = vs.masks;
-    d.payload = vs.payload;
-    d.palette = vs.palette;
-    int32_t id = (int32_t)g.vols.size();
-    hipMemcpyAsync(&g.vol_table[id], &d, sizeof(d), hipMemcpyHostToDevice, g.stream);
+    std::vector<VolumeDesc> descs(count);
+    for (uint32_t k = 0; k < count; ++k) {
+        const VolumeStore& v = out[k];
+        VolumeDesc& d = descs[k];
+        d = VolumeDesc{};
+        for (int a = 0; a < 3; ++a) { d.dim[a] = v.dim[a]; d.bdim[a] = v.bdim[a]; }
+        d.fmt = v.fmt;
+        d.nbricks = v.nbricks;
+        d.brick_map = v.brick_map;
+        d.masks = v.masks;
+        d.payload = v.payload;
+        d.palette = v.palette;
+    }
+    const int32_t id = (int32_t)g.vols.size();
+    hipMemcpyAsync(&g.vol_table[id], descs.data(), sizeof(VolumeDesc) * count, hipMemcpyHostToDevice, g.stream);
     if ((e = hipStreamSynchronize(g.stream)) != hipSuccess) {
         set_err("upload", e);
         return -1;
     }
-    g.vols.push_back(vs);
+    g.vols.insert(g.vols.end(), out.begin(), out.end());
     return id;
 }
 
@@ -500,6 +517,7 @@ void cleanup(void) {
     vt_serve_release();
     vt_dn_release();
     for (auto& v : g.vols) {
+        if (!v.owner) continue;  // shares the pools of the first volume of its batch
         hipFree(v.brick_map); hipFree(v.masks); hipFree(v.payload); hipFree(v.palette);
     }
     g.vols.clear();
@@ -522,6 +540,18 @@ int32_t add_texture(const uint8_t* data, uint32_t width, uint32_t height, uint32
     return upload_volume(data, (size_t)4 * width * height * depth, 1, dim, nullptr);
 }
 
+int32_t vt_add_textures(const uint8_t* rgba8, uint32_t count, uint32_t width, uint32_t height, uint32_t depth,
+                        int32_t* ids_out) {
+    Guard G;
+    if (!g.inited || count == 0) return -1;
+    const uint32_t dim[3] = {width, height, depth};
+    const int32_t first = upload_volume(rgba8, (size_t)4 * width * height * depth, 1, dim, nullptr, count);
+    if (first < 0) return -1;
+    if (ids_out)
+        for (uint32_t k = 0; k < count; ++k) ids_out[k] = first + (int32_t)k;
+    return 0;
+}
+
 float* start_update_instances(uint32_t instance_count) {
     Guard G;
     if (!g.inited) return nullptr;
@@ -539,6 +569,13 @@ int32_t end_update_instances(uint32_t instance_count) {
         set_err("end_update_instances: more instances than started");
         return -1;
     }
+    // the same records over the same textures as the last build: the device tables and the
+    // tree are current (a streamed world re-sends an unchanged list on most updates)
+    const size_t nrec = (size_t)instance_count * 16;
+    if (g.inst_count == instance_count && g.inst_prev.size() == nrec && g.inst_prev_vols == g.vols.size() &&
+        std::memcmp(g.inst_prev.data(), g.inst_host, nrec * 4) == 0)
+        return 0;
+    g.inst_prev.clear();
     hipError_t e = hipMemcpyAsync(g.inst_dev, g.inst_host, (size_t)instance_count * 64, hipMemcpyHostToDevice, g.stream);
     if (e != hipSuccess) {
         set_err("instance copy", e);
@@ -559,7 +596,15 @@ int32_t end_update_instances(uint32_t instance_count) {
     }
     g.bvh_depth = depth;
     g.inst_count = instance_count;
+    g.inst_prev.assign(g.inst_host, g.inst_host + nrec);
+    g.inst_prev_vols = g.vols.size();
+    ++g.lbvh_builds;
     return 0;
+}
+
+uint64_t vt_lbvh_builds(void) {
+    Guard G;
+    return g.lbvh_builds;
 }
 
 int32_t vt_set_params(const vt_render_params* p) {

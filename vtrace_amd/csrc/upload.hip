@@ -6,7 +6,12 @@
 // NEAREST/CLAMP sampling at v/size selects (shaders/trace.frag:77, lib/descriptor.c:100-105);
 // the per-axis remap tables carry that choice (identity for power-of-two sizes).
 //
-// Three kernels:
+// Several volumes of one size can be bricked together (the streamed 16^3 terrain chunks):
+// their bricks are numbered volume after volume, one scan numbers the non-empty bricks of all
+// of them into shared pools, and each volume's map holds ids relative to its first brick in
+// the pools, so every volume is laid out exactly as if it had been uploaded alone.
+//
+// Kernels:
 //   k_brick_classify  one 512-thread block per brick: occupancy bit per voxel
 //                     (alpha > 0, trace.frag:78), one ballot per z-slice -> 8 x u64 mask;
 //   k_scan_*          exclusive scan of the non-empty flags -> brick ids in brick order;
@@ -18,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "vt_internal.h"
 
@@ -36,24 +42,35 @@ __device__ inline uint32_t texel_rgba(const uint8_t* dense, uint32_t fmt, const 
 }
 
 struct BrickArgs {
-    const uint8_t* dense;
+    const uint8_t* dense;      // nvol volumes, one after another
     const uint32_t* palette;
     const uint16_t* remap[3];
     uint32_t dim[3];
     uint32_t bdim[3];
     uint32_t fmt;
+    uint32_t nbv;              // bricks per volume
+    size_t vol_texels;         // texels per volume
 };
+
+// brick b of the batch -> (texel offset of its volume, brick coordinates inside the volume)
+__device__ inline size_t brick_coords(const BrickArgs& a, uint32_t b, uint32_t* bx, uint32_t* by, uint32_t* bz) {
+    const uint32_t k = b / a.nbv, lb = b - k * a.nbv;
+    *bx = lb % a.bdim[0];
+    *by = (lb / a.bdim[0]) % a.bdim[1];
+    *bz = lb / (a.bdim[0] * a.bdim[1]);
+    return (size_t)k * a.vol_texels;
+}
 
 __global__ __launch_bounds__(512) void k_brick_classify(BrickArgs a, uint64_t* masks_all, uint32_t* flags) {
     const uint32_t b = blockIdx.x;
-    const uint32_t bx = b % a.bdim[0];
-    const uint32_t by = (b / a.bdim[0]) % a.bdim[1];
-    const uint32_t bz = b / (a.bdim[0] * a.bdim[1]);
+    uint32_t bx, by, bz;
+    const size_t vbase = brick_coords(a, b, &bx, &by, &bz);
     const uint32_t t = threadIdx.x;  // x + 8y + 64z inside the brick
     const uint32_t gx = bx * 8 + (t & 7), gy = by * 8 + ((t >> 3) & 7), gz = bz * 8 + (t >> 6);
     bool occ = false;
     if (gx < a.dim[0] && gy < a.dim[1] && gz < a.dim[2]) {
-        size_t ti = (size_t)a.remap[0][gx] + (size_t)a.dim[0] * ((size_t)a.remap[1][gy] + (size_t)a.dim[1] * a.remap[2][gz]);
+        size_t ti = vbase + (size_t)a.remap[0][gx] +
+                    (size_t)a.dim[0] * ((size_t)a.remap[1][gy] + (size_t)a.dim[1] * a.remap[2][gz]);
         uint32_t raw;
         occ = (texel_rgba(a.dense, a.fmt, a.palette, ti, &raw) >> 24) != 0u;
     }
@@ -74,17 +91,20 @@ __global__ void k_radius_init(const uint32_t* flags, uint32_t* dist, uint32_t nb
     if (b < nb) dist[b] = flags[b] ? 0u : 255u;
 }
 
-__global__ void k_radius_step(uint32_t* dist, uint32_t bx, uint32_t by, uint32_t bz, uint32_t k) {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= bx * by * bz || dist[b] != 255u) return;
+// layer k over nvol volumes of bx*by*bz bricks each (neighbours within the same volume)
+__global__ void k_radius_step(uint32_t* dist, uint32_t bx, uint32_t by, uint32_t bz, uint32_t nvol, uint32_t k) {
+    const uint32_t gb = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nbv = bx * by * bz;
+    if (gb >= nbv * nvol || dist[gb] != 255u) return;
+    const uint32_t base = (gb / nbv) * nbv, b = gb - base;
     const int x = (int)(b % bx), y = (int)((b / bx) % by), z = (int)(b / (bx * by));
     for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dx = -1; dx <= 1; ++dx) {
                 const int X = x + dx, Y = y + dy, Z = z + dz;
                 if (X < 0 || Y < 0 || Z < 0 || X >= (int)bx || Y >= (int)by || Z >= (int)bz) continue;
-                if (dist[(uint32_t)X + bx * ((uint32_t)Y + by * (uint32_t)Z)] == k - 1u) {
-                    dist[b] = k;
+                if (dist[base + (uint32_t)X + bx * ((uint32_t)Y + by * (uint32_t)Z)] == k - 1u) {
+                    dist[gb] = k;
                     return;
                 }
             }
@@ -96,18 +116,19 @@ __global__ __launch_bounds__(512) void k_brick_scatter(BrickArgs a, const uint64
     const uint32_t b = blockIdx.x;
     const uint32_t t = threadIdx.x;
     const bool nonempty = flags[b] != 0;
-    const uint32_t id = ids[b];
+    const uint32_t id = ids[b];                            // pool slot
+    const uint32_t first = ids[(b / a.nbv) * a.nbv];        // pool slot of the volume's first brick
     if (t == 0)
-        brick_map[b] = nonempty ? id : (VT_EMPTY_BRICK | (min(dist[b], VT_BRICK_RADIUS_CAP + 1u) - 1u));
+        brick_map[b] = nonempty ? id - first : (VT_EMPTY_BRICK | (min(dist[b], VT_BRICK_RADIUS_CAP + 1u) - 1u));
     if (!nonempty) return;
     if (t < 8) masks[(size_t)id * 8 + t] = masks_all[(size_t)b * 8 + t];
-    const uint32_t bx = b % a.bdim[0];
-    const uint32_t by = (b / a.bdim[0]) % a.bdim[1];
-    const uint32_t bz = b / (a.bdim[0] * a.bdim[1]);
+    uint32_t bx, by, bz;
+    const size_t vbase = brick_coords(a, b, &bx, &by, &bz);
     const uint32_t gx = bx * 8 + (t & 7), gy = by * 8 + ((t >> 3) & 7), gz = bz * 8 + (t >> 6);
     uint32_t raw = 0;
     if (gx < a.dim[0] && gy < a.dim[1] && gz < a.dim[2]) {
-        size_t ti = (size_t)a.remap[0][gx] + (size_t)a.dim[0] * ((size_t)a.remap[1][gy] + (size_t)a.dim[1] * a.remap[2][gz]);
+        size_t ti = vbase + (size_t)a.remap[0][gx] +
+                    (size_t)a.dim[0] * ((size_t)a.remap[1][gy] + (size_t)a.dim[1] * a.remap[2][gz]);
         (void)texel_rgba(a.dense, a.fmt, a.palette, ti, &raw);  // payload keeps the texel as stored
     }
     if (a.fmt == 0) payload[(size_t)id * 512 + t] = (uint8_t)raw;
@@ -190,9 +211,11 @@ hipError_t vt_scan_flags(const uint32_t* flags, uint32_t n, uint32_t* out, uint3
     return hipGetLastError();
 }
 
-// Brick a dense device volume.  Allocates the pools (after one sync to learn the count).
+// Brick nvol dense device volumes of one size (packed one after another).  Allocates one set
+// of pools for all of them (after one sync to learn the count); out[0] owns the allocations.
 int vt_brick_volume(const uint8_t* dense_dev, uint32_t fmt, const uint32_t* palette_dev, const uint16_t* remap_dev[3],
-                    const uint32_t dim[3], VolumeStore* out, hipStream_t s) {
+                    const uint32_t dim[3], uint32_t nvol, VolumeStore* out, hipStream_t s) {
+    if (nvol == 0) return -1;
     BrickArgs a{};
     a.dense = dense_dev;
     a.palette = palette_dev;
@@ -202,7 +225,11 @@ int vt_brick_volume(const uint8_t* dense_dev, uint32_t fmt, const uint32_t* pale
         a.bdim[k] = (dim[k] + 7) / 8;
     }
     a.fmt = fmt;
-    const uint32_t nb = a.bdim[0] * a.bdim[1] * a.bdim[2];
+    a.nbv = a.bdim[0] * a.bdim[1] * a.bdim[2];
+    a.vol_texels = (size_t)dim[0] * dim[1] * dim[2];
+    const uint64_t nb64 = (uint64_t)a.nbv * nvol;
+    if (nb64 >= 0x7FFFFFFFull) return -1;
+    const uint32_t nb = (uint32_t)nb64;
     uint64_t* masks_all = nullptr;
     uint32_t *flags = nullptr, *ids = nullptr, *blocks = nullptr, *total = nullptr, *dist = nullptr;
     const uint32_t nblk = (nb + kScanBlock - 1) / kScanBlock + 1;
@@ -220,29 +247,47 @@ int vt_brick_volume(const uint8_t* dense_dev, uint32_t fmt, const uint32_t* pale
                                                std::max(a.bdim[0], std::max(a.bdim[1], a.bdim[2])));
     for (uint32_t k = 1; k <= layers; ++k)
         hipLaunchKernelGGL(k_radius_step, dim3((nb + 255) / 256), dim3(256), 0, s, dist, a.bdim[0], a.bdim[1],
-                           a.bdim[2], k);
-    uint32_t count = 0;
-    hipMemcpyAsync(&count, total, 4, hipMemcpyDeviceToHost, s);
+                           a.bdim[2], nvol, k);
+    // pool slot of each volume's first brick (ids[k * nbv]) and the total
+    std::vector<uint32_t> first(nvol + 1, 0);
+    hipMemcpy2DAsync(first.data(), 4, ids, (size_t)a.nbv * 4, 4, nvol, hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(&first[nvol], total, 4, hipMemcpyDeviceToHost, s);
     hipError_t e = hipStreamSynchronize(s);
     int rc = 0;
     if (e != hipSuccess) rc = -1;
+    const uint32_t count = first[nvol];
     const size_t bpv = fmt == 0 ? 1 : 4;
+    uint32_t* pool_map = nullptr;
+    uint32_t* pool_masks = nullptr;
+    uint8_t* pool_payload = nullptr;
     if (rc == 0) {
-        out->nbricks = count;
-        out->fmt = fmt;
-        for (int k = 0; k < 3; ++k) { out->dim[k] = dim[k]; out->bdim[k] = a.bdim[k]; }
-        size_t cnt = count ? count : 1;
-        if (hipMalloc(&out->brick_map, (size_t)nb * 4) != hipSuccess ||
-            hipMalloc(&out->masks, cnt * 64) != hipSuccess ||
-            hipMalloc(&out->payload, cnt * 512 * bpv) != hipSuccess)
+        const size_t cnt = count ? count : 1;
+        if (hipMalloc(&pool_map, (size_t)nb * 4) != hipSuccess || hipMalloc(&pool_masks, cnt * 64) != hipSuccess ||
+            hipMalloc(&pool_payload, cnt * 512 * bpv) != hipSuccess) {
+            hipFree(pool_map); hipFree(pool_masks); hipFree(pool_payload);
             rc = -1;
+        }
     }
     if (rc == 0) {
-        hipLaunchKernelGGL(k_brick_scatter, dim3(nb), dim3(512), 0, s, a, masks_all, ids, flags, dist, out->brick_map,
-                           reinterpret_cast<uint64_t*>(out->masks), out->payload);
+        hipLaunchKernelGGL(k_brick_scatter, dim3(nb), dim3(512), 0, s, a, masks_all, ids, flags, dist, pool_map,
+                           reinterpret_cast<uint64_t*>(pool_masks), pool_payload);
         if (hipGetLastError() != hipSuccess) rc = -1;
         if (hipStreamSynchronize(s) != hipSuccess) rc = -1;
-        out->bytes = (size_t)nb * 4 + (size_t)(count ? count : 1) * (64 + 512 * bpv);
+        if (rc != 0) { hipFree(pool_map); hipFree(pool_masks); hipFree(pool_payload); }
+    }
+    if (rc == 0) {
+        for (uint32_t k = 0; k < nvol; ++k) {
+            VolumeStore& o = out[k];
+            o.owner = k == 0;
+            o.nbricks = first[k + 1] - first[k];
+            o.fmt = fmt;
+            for (int c = 0; c < 3; ++c) { o.dim[c] = dim[c]; o.bdim[c] = a.bdim[c]; }
+            o.brick_map = pool_map + (size_t)k * a.nbv;
+            o.masks = pool_masks + (size_t)first[k] * 16;
+            o.payload = pool_payload + (size_t)first[k] * 512 * bpv;
+            o.palette = const_cast<uint32_t*>(palette_dev);
+            o.bytes = (size_t)a.nbv * 4 + (size_t)o.nbricks * (64 + 512 * bpv);
+        }
     }
     hipFree(masks_all); hipFree(flags); hipFree(ids); hipFree(blocks); hipFree(total); hipFree(dist);
     return rc;

@@ -21,13 +21,14 @@ struct VolumeStore {
     uint8_t* payload = nullptr;
     uint32_t* palette = nullptr;
     size_t bytes = 0;
+    bool owner = true;          // frees brick_map/masks/payload/palette (volumes bricked together share them)
 };
 
 // upload.hip
 hipError_t vt_scan_flags(const uint32_t* flags, uint32_t n, uint32_t* out, uint32_t* scratch_blocks, uint32_t* total_dev,
                          hipStream_t s);
 int vt_brick_volume(const uint8_t* dense_dev, uint32_t fmt, const uint32_t* palette_dev, const uint16_t* remap_dev[3],
-                    const uint32_t dim[3], VolumeStore* out, hipStream_t s);
+                    const uint32_t dim[3], uint32_t nvol, VolumeStore* out, hipStream_t s);
 int vt_debrick(const VolumeDesc& v, uint8_t* dense_dev, hipStream_t s);
 
 // lbvh.hip: builds InstanceGPU records and the LBVH from n ABI records on the device.

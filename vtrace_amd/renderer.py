@@ -4,7 +4,8 @@
 the Vulkan library: ``entry`` at construction, ``start_update_instances`` /
 ``end_update_instances`` with one record per uploaded child (un-uploaded textures are
 skipped, render.rs:220-238), at most one ``add_texture`` per ``render_tick``
-(render.rs:246-268), then ``render_tick`` with the perspective and camera matrices.
+(render.rs:246-268; ``uploads_per_tick=None`` drains the queue with batched uploads instead),
+then ``render_tick`` with the perspective and camera matrices.
 ``SceneGraph`` flattens parents and children depth-first in child order; instance order
 is the draw order and therefore the instance index of the hit buffers (scene.rs:146-181).
 
@@ -132,12 +133,24 @@ class Renderer:
         if self.lib.end_update_instances(n) != 0:
             raise RendererError("ERROR: Updating instances failed")
 
-    def render_tick(self, perspective: np.ndarray, camera: np.ndarray, queue: TextureUploadQueue | None = None) -> bool:
+    def render_tick(self, perspective: np.ndarray, camera: np.ndarray, queue: TextureUploadQueue | None = None,
+                    uploads_per_tick: int | None = 1) -> bool:
+        """uploads_per_tick = 1 is the reference (render.rs:246); None drains the queue, k takes
+        up to k, uploading runs of same-size RGBA8 volumes together (vt_add_textures)."""
         if queue is not None:
-            item = queue.pop()
-            if item is not None:
-                volume, handle = item
+            items = []
+            while uploads_per_tick is None or len(items) < uploads_per_tick:
+                item = queue.pop()
+                if item is None:
+                    break
+                items.append(item)
+            if len(items) == 1:
+                volume, handle = items[0]
                 self.texture_handle_lookup[handle] = self.upload(volume)
+            elif items:
+                ids = self.upload_many([v for v, _ in items])
+                for (_, handle), tid in zip(items, ids):
+                    self.texture_handle_lookup[handle] = tid
         P = np.ascontiguousarray(perspective, dtype=np.float32)
         Cm = np.ascontiguousarray(camera, dtype=np.float32)
         info = RenderTickInfo(P.ctypes.data, Cm.ctypes.data)
@@ -172,6 +185,38 @@ class Renderer:
         if volume.fmt == 0:
             return self.add_volume_indexed(volume.data, volume.palette)
         return self.add_texture_rgba(volume.data)
+
+    def add_textures_rgba(self, volumes: list) -> list[int]:
+        """vt_add_textures: RGBA8 volumes of one shape (d, h, w) uint32, uploaded together."""
+        a = np.ascontiguousarray(np.stack([np.asarray(v, dtype=np.uint32) for v in volumes]))
+        n, d, h, w = a.shape
+        ids = np.zeros(n, np.int32)
+        if self.lib.vt_add_textures(a.ctypes.data, n, w, h, d, ids.ctypes.data) != 0:
+            raise RendererError(f"ERROR: Adding textures failed: {self.error()}")
+        return [int(i) for i in ids]
+
+    def upload_many(self, volumes: list) -> list[int]:
+        """Uploads in order; consecutive RGBA8 volumes of one shape go up together."""
+        ids: list[int] = []
+        k = 0
+        while k < len(volumes):
+            v = volumes[k]
+            if v.fmt == 0:
+                ids.append(self.upload(v))
+                k += 1
+                continue
+            j = k + 1
+            while j < len(volumes) and volumes[j].fmt == 1 and volumes[j].data.shape == v.data.shape:
+                j += 1
+            if j - k == 1:
+                ids.append(self.add_texture_rgba(v.data))
+            else:
+                ids.extend(self.add_textures_rgba([u.data for u in volumes[k:j]]))
+            k = j
+        return ids
+
+    def lbvh_builds(self) -> int:
+        return int(self.lib.vt_lbvh_builds())
 
     def set_instances(self, records: np.ndarray):
         recs = np.ascontiguousarray(records, dtype=np.float32).reshape(-1, 16)
