@@ -97,6 +97,10 @@ int32_t vt_get_params(vt_render_params* params);
  * nothing added. */
 int32_t vt_add_textures(const uint8_t* rgba8, uint32_t count, uint32_t width, uint32_t height, uint32_t depth,
                         int32_t* ids_out);
+/* Headless cursor: the position render_tick reads into user_input.mouse_x/y (after moving the
+ * previous one to last_mouse_x/y, lib/entry.c:157-163); keys are written into the struct from
+ * get_input_data_pointer directly, as the reference's key callback does (lib/entry.c:31-52). */
+void vt_set_cursor(double x, double y);
 /* Number of LBVH builds so far (end_update_instances calls that rebuilt the tree). */
 uint64_t vt_lbvh_builds(void);
 

@@ -49,7 +49,7 @@ def test_struct_layouts_match_reference():
 def test_python_binding_declares_all():
     lib = _native.render_lib()
     for n in abi.REFERENCE_EXPORTS + abi.EXTENSION_EXPORTS:
-        assert getattr(lib, n).restype is not None or n in ("cleanup", "vt_serve_shutdown", "vt_dn_shutdown")
+        assert getattr(lib, n).restype is not None or n in ("cleanup", "vt_serve_shutdown", "vt_dn_shutdown", "vt_set_cursor")
 
 
 def test_scene_library_exports_every_declared_symbol():

@@ -28,6 +28,7 @@ struct State {
     int device = 0;
     hipStream_t stream = nullptr;
     user_input input{};
+    double cursor[2] = {0.0, 0.0};    // headless cursor position (vt_set_cursor)
     // instances (lib/memory.c:221-267)
     float* inst_host = nullptr;    // pinned staging
     float* inst_dev = nullptr;
@@ -493,6 +494,16 @@ int32_t render_tick(int32_t* window_width, int32_t* window_height, const render_
         set_err("render_tick: not initialised or null info");
         return -1;
     }
+    // lib/entry.c:157-163: the previous cursor becomes last_mouse, the current cursor is read
+    // (here: the position the host set with vt_set_cursor); on the first frame both are equal
+    g.input.last_mouse_x = g.input.mouse_x;
+    g.input.last_mouse_y = g.input.mouse_y;
+    g.input.mouse_x = g.cursor[0];
+    g.input.mouse_y = g.cursor[1];
+    if (g.frames == 0) {
+        g.input.last_mouse_x = g.input.mouse_x;
+        g.input.last_mouse_y = g.input.mouse_y;
+    }
     FrameGPU F;
     if (!build_frame(static_cast<const float*>(info->perspective), static_cast<const float*>(info->camera), F)) {
         set_err("render_tick: singular projection or camera");
@@ -508,6 +519,12 @@ int32_t render_tick(int32_t* window_width, int32_t* window_height, const render_
 }
 
 user_input* get_input_data_pointer(void) { return &g.input; }
+
+void vt_set_cursor(double x, double y) {
+    Guard G;
+    g.cursor[0] = x;
+    g.cursor[1] = y;
+}
 
 void cleanup(void) {
     std::lock_guard<std::mutex> lk(g_lock);

@@ -123,13 +123,17 @@ class Renderer:
         if not ptr:
             raise RendererError("ERROR: Updating instances failed")
         buf = np.ctypeslib.as_array(ptr, shape=(max(count, 1) * 16,))
-        n = 0
+        models, tex = [], []
         for model, handle in scene:
-            tex = self.texture_handle_lookup.get(handle)
-            if tex is None:
-                continue
-            buf[16 * n:16 * (n + 1)] = scenes.instance_record(model, tex)
-            n += 1
+            t = self.texture_handle_lookup.get(handle)
+            if t is not None:
+                models.append(model)
+                tex.append(t)
+        n = len(models)
+        if n:
+            recs = np.asarray(models, dtype=np.float32).reshape(n, 16).copy()
+            recs.view(np.uint32)[:, 15] = np.asarray(tex, dtype=np.uint32)  # GPUInstance::from_model
+            buf[:16 * n] = recs.reshape(-1)
         if self.lib.end_update_instances(n) != 0:
             raise RendererError("ERROR: Updating instances failed")
 

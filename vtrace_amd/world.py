@@ -68,6 +68,7 @@ class WorldPager:
     def __init__(self, seed: int = 0):
         self.chunks: dict[tuple[int, int, int], tuple[scenes.Volume, int] | None] = {}
         self.terrain_generator = TerrainGenerator(seed)
+        self.view = None  # (camera chunk, its terrain scene): paging is cached, so it cannot change
 
     def page(self, cx: int, cy: int, cz: int, queue: TextureUploadQueue) -> int | None:
         key = (cx, cy, cz)
@@ -123,13 +124,18 @@ def load_positions(camera_position) -> list[tuple[int, int, int]]:
 
 
 def terrain_scene(camera_position, pager: WorldPager, queue: TextureUploadQueue) -> SceneGraph:
-    """world.rs:163-198: scene_terrain for a camera position (chunks paged on the way)."""
+    """world.rs:163-198: scene_terrain for a camera position (chunks paged on the way).  While the
+    camera stays in one chunk the visited positions, and so the scene, are those of the last call."""
+    cpos = get_chunk_pos(camera_position)
+    if pager.view is not None and pager.view[0] == cpos:
+        return pager.view[1]
     scene = SceneGraph()
     positions = load_positions(camera_position)
     handles = pager.page_many(positions, queue)
     for p, h in zip(positions, handles):
         if h is not None:
             scene.add_child(SceneGraph.child(chunk_model(*p), h))
+    pager.view = (cpos, scene)
     return scene
 
 
