@@ -50,7 +50,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-KERNEL_NAMES = {"render": "k_render0", "primary": "k_primary", "shadow": "k_shadow", "bounce": "k_bounce",
+KERNEL_NAMES = {"render": "k_render0", "primary": "k_primary", "secondary": "k_secondary",
                 "accumulate": "k_accumulate"}
 
 

@@ -42,6 +42,10 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kStackLds = 16;   // LBVH stack entries per thread held in LDS
+// wavefront counters: [0] queue of the primary vertices, [2] primary fetch, [3, 64) fetch
+// counters of the secondary launches, [64 + d + 1] the queue built at depth d
+constexpr uint32_t kCounters = 128;
+constexpr uint32_t kQueueCounters = 64;
 
 // GLSL min/max as "y if y < x else x" (also fixes the NaN cases the DDA can produce)
 __device__ __forceinline__ float gmin(float x, float y) { return (y < x) ? y : x; }
@@ -609,9 +613,9 @@ __device__ __forceinline__ bool better(const HitG& c, const HitG& b) {
 }
 __device__ __forceinline__ float prune_bound(float t) { return t + fabsf(t) * 0.0009765625f + 0.0009765625f; }
 
-// Closest (or any) hit over all instances except `skip`, through the LBVH.
-template <bool PRIMARY, bool ANY, bool ST, bool DEEP>
-__device__ HitG query(const SceneGPU& S, const FrameGPU& F, V3 O, V3 D, uint32_t skip, Ctr<ST>& ctr,
+// Closest (any_hit: any) hit over all instances except `skip`, through the LBVH.
+template <bool PRIMARY, bool ST, bool DEEP>
+__device__ HitG query(const SceneGPU& S, const FrameGPU& F, V3 O, V3 D, uint32_t skip, bool any_hit, Ctr<ST>& ctr,
                       const Lds& stk_lds, const Lds& mslot) {
     HitG best;
     best.hit = false;
@@ -636,7 +640,7 @@ __device__ HitG query(const SceneGPU& S, const FrameGPU& F, V3 O, V3 D, uint32_t
             HitG h;
             if (enter_instance<PRIMARY, ST>(S, F, i, O, D, h, ctr, mslot) && better(h, best)) {
                 best = h;
-                if (ANY) return best;
+                if (any_hit) return best;
             }
         } else {
             stk.push(N.right);
@@ -689,9 +693,9 @@ __device__ void surface_of(const SceneGPU& S, const HitG& h, SurfaceRec& sf) {
     sf.face = h.face;
 }
 
-template <bool ANY, bool ST, bool DEEP>
-__device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const SurfaceRec& sf, V3 w, Ctr<ST>& ctr,
-                          const Lds& stk_lds, const Lds& mslot) {
+template <bool ST, bool DEEP>
+__device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const SurfaceRec& sf, V3 w, bool any_hit,
+                          Ctr<ST>& ctr, const Lds& stk_lds, const Lds& mslot) {
     ctr.ray();
     HitG h;
     h.hit = false;
@@ -721,7 +725,7 @@ __device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const SurfaceRec
         ctr.hit();
         return h;
     }
-    h = query<false, ANY, ST, DEEP>(S, F, V3{sf.X[0], sf.X[1], sf.X[2]}, w, origin, ctr, stk_lds, mslot);
+    h = query<false, ST, DEEP>(S, F, V3{sf.X[0], sf.X[1], sf.X[2]}, w, origin, any_hit, ctr, stk_lds, mslot);
     if (h.hit) ctr.hit();
     return h;
 }
@@ -798,7 +802,8 @@ __device__ __forceinline__ uint32_t append_slot(uint32_t* counter) {
 
 // Wavefront buffers (device pointers), see WaveBuffers in vt_internal.h.
 struct WF {
-    float4* L;            // per path: accumulated radiance
+    float4* L;            // per path: accumulated radiance (sky and emission terms)
+    float4* Ls;           // per path: accumulated sun terms (written by sun rays only)
     float4* beta;         // per path: throughput
     uint32_t* rng;        // per path: RNG state
     uint8_t* shadowed;    // per path: primary vertex shadowed
@@ -915,7 +920,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         }
         const V3 D = camera_ray(F, (float)px + jx, (float)py + jy);
         ctr.ray();
-        HitG h = query<true, false, ST, DEEP>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, VT_EMPTY, ctr, stk, mslot);
+        HitG h = query<true, ST, DEEP>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, VT_EMPTY, false, ctr, stk, mslot);
         float4 L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float beta[3] = {1.0f, 1.0f, 1.0f};
         uint8_t shadowed = 0;
@@ -967,6 +972,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
             wf.dir[0][slot] = dirw;
         }
         wf.L[w] = L;
+        wf.Ls[w] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         wf.beta[w] = make_float4(beta[0], beta[1], beta[2], 0.0f);
         wf.rng[w] = rng.s;
         wf.shadowed[w] = shadowed;
@@ -974,55 +980,47 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     if (ST) ctr.flush(stats);
 }
 
+// The secondary rays of one depth in one launch: items [0, nb) are the bounce rays of queue
+// slots 0..nb-1 (nb = the queue size when paths continue, else 0), items [nb, nb + count) the
+// sun rays of slots 0..count-1.  The two kinds write different per-path state (bounce rays: L,
+// beta, rng and the next queue; sun rays: Ls and the shadow flag), so they run side by side and
+// fill each other's tail.  in_idx / out_idx: counters of this depth's queue and the next one.
 template <bool ST, bool DEEP>
-__global__ __launch_bounds__(kBlock) void k_shadow(const ReqGPU* __restrict__ reqs, const BatchGPU* __restrict__ B,
-                                                   WF wf, uint32_t q, uint32_t depth, uint32_t fetch_idx,
-                                                   unsigned long long* stats) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_secondary(
+    const ReqGPU* __restrict__ reqs, const BatchGPU* __restrict__ B, const float* __restrict__ srgb_dec, WF wf,
+    uint32_t q, uint32_t depth, uint32_t in_idx, uint32_t out_idx, uint32_t fetch_idx, uint32_t bounce,
+    unsigned long long* stats) {
     __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
     const Lds stk{&smem[threadIdx.x]}, mslot{&smem[kStackLds * kBlock + threadIdx.x]};
     Ctr<ST> ctr;
-    const uint32_t count = wf.counters[q];
-    const uint32_t nreq = B->nreq;
-    if (blockIdx.x == 0 && threadIdx.x == 0) wf.counters[1u - q] = 0u;  // next queue starts empty
-    for_each_item(&wf.counters[fetch_idx], count, [&](uint32_t slot) {
-        const float4 c = wf.contrib[q][slot];
-        if (c.w == 0.0f) return;
-        const SurfaceRec sf = wf.surf[q][slot];
-        const ReqGPU& Rq = reqs[find_req(B->work_base, nreq, sf.path)];
-        const SceneGPU S = Rq.S;
-        HitG h = secondary<true, ST, DEEP>(S, Rq.F, sf, V3{Rq.F.sun[0], Rq.F.sun[1], Rq.F.sun[2]}, ctr, stk, mslot);
-        if (!h.hit) {
-            float4 L = wf.L[sf.path];
-            L.x = L.x + c.x;
-            L.y = L.y + c.y;
-            L.z = L.z + c.z;
-            wf.L[sf.path] = L;
-        } else if (depth == 0) {
-            wf.shadowed[sf.path] = 1;
-        }
-    });
-    if (ST) ctr.flush(stats);
-}
-
-template <bool ST, bool DEEP>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_bounce(const ReqGPU* __restrict__ reqs, const BatchGPU* __restrict__ B,
-                                                   const float* __restrict__ srgb_dec, WF wf, uint32_t q,
-                                                   uint32_t depth, uint32_t fetch_idx, unsigned long long* stats) {
-    __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
-    const Lds stk{&smem[threadIdx.x]}, mslot{&smem[kStackLds * kBlock + threadIdx.x]};
-    Ctr<ST> ctr;
-    const uint32_t count = wf.counters[q];
+    const uint32_t count = wf.counters[in_idx];
+    const uint32_t nb = bounce ? count : 0u;
     const uint32_t nq = 1u - q;
     const uint32_t nreq = B->nreq;
-    for_each_item(&wf.counters[fetch_idx], count, [&](uint32_t slot) {
-        const float4 dw = wf.dir[q][slot];
+    for_each_item(&wf.counters[fetch_idx], nb + count, [&](uint32_t item) {
+        const bool is_bounce = item < nb;
+        const uint32_t slot = is_bounce ? item : item - nb;
+        const float4 dw = is_bounce ? wf.dir[q][slot] : wf.contrib[q][slot];  // w = 1: trace it
         if (dw.w == 0.0f) return;
         const SurfaceRec sf = wf.surf[q][slot];
         const uint32_t path = sf.path;
         const ReqGPU& Rq = reqs[find_req(B->work_base, nreq, path)];
         const SceneGPU S = Rq.S;
         const FrameGPU& F = Rq.F;
-        HitG h = secondary<false, ST, DEEP>(S, F, sf, V3{dw.x, dw.y, dw.z}, ctr, stk, mslot);
+        const V3 w = is_bounce ? V3{dw.x, dw.y, dw.z} : V3{F.sun[0], F.sun[1], F.sun[2]};
+        HitG h = secondary<ST, DEEP>(S, F, sf, w, !is_bounce, ctr, stk, mslot);
+        if (!is_bounce) {
+            if (!h.hit) {
+                float4 L = wf.Ls[path];
+                L.x = L.x + dw.x;
+                L.y = L.y + dw.y;
+                L.z = L.z + dw.z;
+                wf.Ls[path] = L;
+            } else if (depth == 0) {
+                wf.shadowed[path] = 1;
+            }
+            return;
+        }
         const float4 b4 = wf.beta[path];
         if (!h.hit) {
             float4 L = wf.L[path];
@@ -1043,7 +1041,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         bool no_sun;
         const uint32_t max_b = F.mode == 2 ? F.max_bounces : 0u;
         shade(F, srgb_dec, ns, depth + 1u, max_b, beta, rng, contrib, dirw, no_sun);
-        const uint32_t ns_slot = append_slot(&wf.counters[nq]);
+        const uint32_t ns_slot = append_slot(&wf.counters[out_idx]);
         wf.surf[nq][ns_slot] = ns;
         wf.contrib[nq][ns_slot] = contrib;
         wf.dir[nq][ns_slot] = dirw;
@@ -1070,10 +1068,11 @@ __device__ void accumulate_pixel(const ReqGPU* __restrict__ reqs, const BatchGPU
         acc[0] = a.x; acc[1] = a.y; acc[2] = a.z;
     }
     for (uint32_t s = Rq.s0; s < Rq.s1; ++s) {
-        const float4 L = wf.L[wb + tm.index(px, py, s, Rq.s0)];
-        acc[0] += L.x;
-        acc[1] += L.y;
-        acc[2] += L.z;
+        const size_t k = wb + tm.index(px, py, s, Rq.s0);
+        const float4 L = wf.L[k], Ls = wf.Ls[k];
+        acc[0] += L.x + Ls.x;
+        acc[1] += L.y + Ls.y;
+        acc[2] += L.z + Ls.z;
     }
     if (Rq.final_chunk) {
         const float fs = (float)(Rq.F.spp ? Rq.F.spp : 1u);
@@ -1134,7 +1133,7 @@ __global__ __launch_bounds__(kBlock) void k_render0(const FrameGPU F, const Scen
             }
             const V3 D = camera_ray(F, (float)px + jx, (float)py + jy);
             ctr.ray();
-            HitG h = query<true, false, ST, DEEP>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, VT_EMPTY, ctr, stk, mslot);
+            HitG h = query<true, ST, DEEP>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, VT_EMPTY, false, ctr, stk, mslot);
             if (h.hit) {
                 ctr.hit();
                 uint8_t pal = 0;
@@ -1196,7 +1195,7 @@ void vt_trace_set_tables(const float* dec_dev, const float* thr_dev) {
 void vt_trace_set_persistent_blocks(uint32_t blocks) { g_persistent_blocks = blocks ? blocks : 2048; }
 
 size_t vt_wave_bytes(uint32_t npaths) {
-    return (size_t)npaths * (16 + 16 + 4 + 1) + 2 * (size_t)npaths * (64 + 16 + 16) + 64 * 4 + 16 * 256 +
+    return (size_t)npaths * (16 + 16 + 16 + 4 + 1) + 2 * (size_t)npaths * (64 + 16 + 16) + kCounters * 4 + 16 * 256 +
            sizeof(ReqGPU) * VT_MAX_BATCH + sizeof(BatchGPU);
 }
 
@@ -1209,6 +1208,7 @@ int vt_wave_bind(WaveBuffers& wb, void* mem, uint32_t npaths) {
     };
     wb.npaths = npaths;
     wb.L = take((size_t)npaths * 16);
+    wb.Ls = take((size_t)npaths * 16);
     wb.beta = take((size_t)npaths * 16);
     wb.rng = take((size_t)npaths * 4);
     wb.shadowed = take((size_t)npaths);
@@ -1217,7 +1217,7 @@ int vt_wave_bind(WaveBuffers& wb, void* mem, uint32_t npaths) {
         wb.contrib[q] = take((size_t)npaths * 16);
         wb.dir[q] = take((size_t)npaths * 16);
     }
-    wb.counters = reinterpret_cast<uint32_t*>(take(64 * 4));
+    wb.counters = reinterpret_cast<uint32_t*>(take(kCounters * 4));
     wb.tables = take(sizeof(ReqGPU) * VT_MAX_BATCH + sizeof(BatchGPU));
     return 0;
 }
@@ -1225,6 +1225,7 @@ int vt_wave_bind(WaveBuffers& wb, void* mem, uint32_t npaths) {
 static WF to_wf(const WaveBuffers& wb) {
     WF w;
     w.L = reinterpret_cast<float4*>(wb.L);
+    w.Ls = reinterpret_cast<float4*>(wb.Ls);
     w.beta = reinterpret_cast<float4*>(wb.beta);
     w.rng = reinterpret_cast<uint32_t*>(wb.rng);
     w.shadowed = reinterpret_cast<uint8_t*>(wb.shadowed);
@@ -1308,22 +1309,18 @@ static int launch_batch(const BatchGPU& Bh, const WaveBuffers& wb, unsigned long
         ++stage;
     };
     auto st_at = [&](uint32_t k) { return stats ? stats + (size_t)k * VT_STAT_COUNT : nullptr; };
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, wb.counters, 64u);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kCounters), 0, s, wb.counters, kCounters);
     begin(VT_STAGE_PRIMARY);
     hipLaunchKernelGGL((k_primary<ST, DEEP>), grid, block, 0, s, reqs, B, g_srgb_dec, wf, st_at(stage));
     end();
-    uint32_t fetch = 3;
-    for (uint32_t d = 0; d <= Bh.max_bounces && fetch + 2 < 64; ++d) {
-        const uint32_t q = d & 1u;
-        begin(VT_STAGE_SHADOW);
-        hipLaunchKernelGGL((k_shadow<ST, DEEP>), grid, block, 0, s, reqs, B, wf, q, d, fetch++, st_at(stage));
+    uint32_t fetch = 3, in_idx = 0;
+    for (uint32_t d = 0; d <= Bh.max_bounces && fetch < kQueueCounters && kQueueCounters + d + 1 < kCounters; ++d) {
+        const uint32_t out_idx = kQueueCounters + d + 1;
+        begin(VT_STAGE_SECONDARY);
+        hipLaunchKernelGGL((k_secondary<ST, DEEP>), grid, block, 0, s, reqs, B, g_srgb_dec, wf, d & 1u, d, in_idx,
+                           out_idx, fetch++, d < Bh.max_bounces ? 1u : 0u, st_at(stage));
         end();
-        if (d < Bh.max_bounces) {
-            begin(VT_STAGE_BOUNCE);
-            hipLaunchKernelGGL((k_bounce<ST, DEEP>), grid, block, 0, s, reqs, B, g_srgb_dec, wf, q, d, fetch++,
-                               st_at(stage));
-            end();
-        }
+        in_idx = out_idx;
     }
     begin(VT_STAGE_ACCUMULATE);
     hipLaunchKernelGGL(k_accumulate, dim3(g_persistent_blocks), dim3(256), 0, s, reqs, B, wf);

@@ -53,13 +53,14 @@ int vt_lbvh_build(LbvhWork& w, const float* records_dev, uint32_t n, const Volum
 struct WaveBuffers {
     uint32_t npaths = 0;        // capacity in pixel samples
     void* L = nullptr;          // float4 per path
+    void* Ls = nullptr;         // float4 per path (sun terms)
     void* beta = nullptr;       // float4 per path
     void* rng = nullptr;        // u32 per path
     void* shadowed = nullptr;   // u8 per path
     void* surf[2] = {nullptr, nullptr};     // 64 B surface records per queue slot
     void* contrib[2] = {nullptr, nullptr};  // float4 per slot
     void* dir[2] = {nullptr, nullptr};      // float4 per slot
-    uint32_t* counters = nullptr;           // 64 u32: queue sizes and fetch counters
+    uint32_t* counters = nullptr;           // 128 u32: queue sizes and fetch counters
     void* tables = nullptr;                 // ReqGPU[VT_MAX_BATCH] then BatchGPU
     void* gsamp = nullptr;                  // optional, owned by the caller: 2 x float4 per path
                                             // (G-buffer of each sample, averaged per pixel)

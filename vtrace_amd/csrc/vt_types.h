@@ -119,7 +119,14 @@ struct BatchGPU {
 // Launch stages of one frame (stats and timings are kept per stage):
 // mode 0: [render0]; modes 1/2: [primary, shadow d0, bounce d0, ..., shadow dB, accumulate].
 #define VT_MAX_STAGES 64
-enum VtStageKind { VT_STAGE_RENDER0 = 0, VT_STAGE_PRIMARY, VT_STAGE_SHADOW, VT_STAGE_BOUNCE, VT_STAGE_ACCUMULATE };
+enum VtStageKind {
+    VT_STAGE_RENDER0 = 0,
+    VT_STAGE_PRIMARY,
+    VT_STAGE_SHADOW,      // (unused: sun rays run in the secondary stage)
+    VT_STAGE_BOUNCE,      // (unused: bounce rays run in the secondary stage)
+    VT_STAGE_ACCUMULATE,
+    VT_STAGE_SECONDARY    // sun and bounce rays of one depth
+};
 
 // stats buffer layout (u64 each, VT_STAT_COUNT per stage)
 enum VtStat {

@@ -289,7 +289,7 @@ class Renderer:
             raise RendererError(self.error())
         return {k: int(v) for k, v in zip(STAT_NAMES, out)}
 
-    STAGE_KINDS = ("render", "primary", "shadow", "bounce", "accumulate")
+    STAGE_KINDS = ("render", "primary", "shadow", "bounce", "accumulate", "secondary")
 
     def stage_stats(self) -> list:
         """Per-launch-stage counters of the last counted frame: [(kind, {counter: value})]."""
