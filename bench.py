@@ -7,8 +7,10 @@ sun shadow ray per vertex, hash-RNG seeded per pixel/sample, FP32 radiance.
 A step is one frame: LBVH traversal + brick DDA + shading for every pixel sample, with the
 volumes, LBVH and instances already resident in HBM.  Rays counted = traversal queries
 (primary + shadow + bounce) from the library's own counters on an untimed counting frame.
-`value` = rays per frame / mean frame time (HIP events on the library stream around the K
-timed frames).
+`value` = rays per frame / mean host time per frame over the K timed frames, rendered as
+render_tick renders them (two sample chunks on two streams, each frame finished before the
+next).  The launch-stage table and the roofline come from the same frames on one stream with
+HIP events around every launch (`single_stream_ms_per_frame`).
 
 Roofline (HBM-bound gather work, no MFMA): algorithmic bytes = 32 B per LBVH node visited
 + 64 B per instance entered + (4 B + 64 B if non-empty) per 8^3 brick entered + 5 B per hit,
@@ -128,6 +130,8 @@ def main():
     ap.add_argument("--denoiser-iters", type=int, default=8000,
                     help="training iterations of the denoiser (0 skips the denoiser measurement)")
     ap.add_argument("--streaming", type=int, default=1, help="1: streamed terrain measurement, 0: skip it")
+    ap.add_argument("--single-stream", action="store_true",
+                    help="render every frame on one stream (kernel traces then match the stage table)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the renderer has no multi-GPU path); got --gpus {args.gpus}",
@@ -158,12 +162,20 @@ def main():
     hit_frac = float((hits["inst"] != 0xFFFFFFFF).mean())
     r.set_params(sc.params, stats=False)
     r.render(sc.P, sc.C)
-    # timed: K frames, every launch bracketed by HIP events on the library stream
+    # timed: W untimed frames, then exactly K frames as render_tick renders them (each complete
+    # before the next); wall_ms = host time per frame, kernel_ms = GPU time per frame
+    r.lib.vt_set_frame_split(0 if args.single_stream else 1)
+    r.bench(args.warmup, 1) if args.warmup > 0 else None
     r.lib.vt_synchronize()
     t0 = time.perf_counter()
-    kernel_ms, wall_ms, stage_ms = r.bench_stages(args.warmup, args.steps)
+    kernel_ms, wall_ms = r.bench(0, args.steps)
     r.lib.vt_synchronize()
     t_all = time.perf_counter() - t0
+    # launch-stage breakdown (untimed for the metric): the same frames on one stream, HIP events
+    # around every launch; the roofline of the dominant kernel comes from these
+    r.lib.vt_set_frame_split(0)
+    single_kernel_ms, single_wall_ms, stage_ms = r.bench_stages(args.warmup, args.steps)
+    r.lib.vt_set_frame_split(0 if args.single_stream else 1)
     # per-frame latency distribution (render_tick round trips, GPU time)
     lat = []
     for _ in range(max(5, min(args.steps, 20))):
@@ -222,6 +234,7 @@ def main():
                        "GBs": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in per_kind.items()},
         "latency_ms": {"p50": round(p50, 4), "p99": round(p99, 4)},
         "kernel_ms_per_frame": round(kernel_ms, 4),
+        "single_stream_ms_per_frame": round(single_wall_ms, 4),
         "counters": st,
         "timed_region_s": round(t_all, 3),
     }

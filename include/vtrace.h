@@ -101,6 +101,9 @@ int32_t vt_add_textures(const uint8_t* rgba8, uint32_t count, uint32_t width, ui
  * previous one to last_mouse_x/y, lib/entry.c:157-163); keys are written into the struct from
  * get_input_data_pointer directly, as the reference's key callback does (lib/entry.c:31-52). */
 void vt_set_cursor(double x, double y);
+/* 1 (default): a path-traced frame of 2+ samples without G-buffer runs as two sample chunks on
+ * two streams (results identical, the per-pixel sum stays in sample order); 0: one stream. */
+int32_t vt_set_frame_split(uint32_t on);
 /* Number of LBVH builds so far (end_update_instances calls that rebuilt the tree). */
 uint64_t vt_lbvh_builds(void);
 
@@ -124,17 +127,18 @@ int32_t vt_read_gbuffer(float* dst, uint64_t bytes);
  * bricks, non-empty bricks, hits, DDA steps. */
 int32_t vt_read_stats(uint64_t* dst7);
 /* Per-launch-stage counters of the last counted frame: dst holds max_stages x 7 u64, kinds
- * receives the stage kinds (0 render, 1 primary, 2 shadow, 3 bounce, 4 accumulate).
- * Returns the number of stages. */
+ * receives the stage kinds (0 render, 1 primary, 4 accumulate, 5 secondary: the sun and bounce
+ * rays of one depth).  Returns the number of stages. */
 int32_t vt_read_stage_stats(uint64_t* dst, uint32_t max_stages, uint8_t* kinds);
-/* Like vt_bench_frames, plus the mean GPU time of every launch stage (HIP events around
- * each launch on the library stream).  Returns the number of stages or -1. */
+/* Frames on the library stream alone (one sample chunk), with the mean GPU time of every
+ * launch stage (HIP events around each launch).  Returns the number of stages or -1. */
 int32_t vt_bench_stages(uint32_t warmup, uint32_t iters, float* kernel_ms, float* wall_ms, float* stage_ms,
                         uint32_t max_stages, uint8_t* kinds);
 /* GPU time of the last render_tick's trace kernel (HIP events on the library stream). */
 float vt_last_frame_ms(void);
-/* Renders `iters` frames after `warmup` with the last camera; per-frame kernel time from
- * HIP events on the library stream.  Writes the mean kernel ms and the wall ms per frame. */
+/* Renders `iters` frames after `warmup` with the last camera, as render_tick renders them
+ * (vt_set_frame_split; each frame finished before the next).  kernel_ms = mean GPU time per
+ * frame (HIP events), wall_ms = mean host time per frame including the wait. */
 int32_t vt_bench_frames(uint32_t warmup, uint32_t iters, float* kernel_ms, float* wall_ms);
 
 /* Dense read-back of a bricked volume in voxel order (u8 indices or RGBA8), for round

@@ -9,6 +9,7 @@
 // (lib/command.c:111-251).
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -61,6 +62,15 @@ struct State {
     void* wave_mem = nullptr;   // wavefront path state + queues
     size_t wave_cap = 0;        // pixel samples it holds
     WaveBuffers wave;
+    // two-stream frames: samples [spp/2, spp) on stream2 with their own wavefront block
+    bool split = true;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    void* wave2_mem = nullptr;
+    size_t wave2_cap = 0;
+    WaveBuffers wave2;
+    float4* acc = nullptr;      // per pixel running sum between the two sample chunks
+    size_t acc_cap = 0;
     unsigned long long* stats = nullptr;  // VT_MAX_STAGES x VT_STAT_COUNT
     uint8_t stage_kinds[VT_MAX_STAGES] = {};
     uint32_t nstages = 0;
@@ -139,6 +149,37 @@ int ensure_wave_buffers() {
     vt_wave_bind(g.wave, g.wave_mem, (uint32_t)paths);
     g.wave_cap = paths;
     g.wave.gsamp = g.gbuf_on ? g.gsamp : nullptr;
+    return 0;
+}
+
+// second wavefront block and the per-pixel running sum for two-stream frames
+int ensure_split_buffers(uint32_t spp) {
+    const size_t paths = padded_paths(g.params.width, g.params.height, spp - spp / 2);
+    if (paths > g.wave2_cap || !g.wave2_mem) {
+        hipFree(g.wave2_mem);
+        g.wave2_mem = nullptr;
+        g.wave2_cap = 0;
+        hipError_t e = hipMalloc(&g.wave2_mem, vt_wave_bytes((uint32_t)paths));
+        if (e != hipSuccess) {
+            set_err("wavefront buffer allocation", e);
+            return -1;
+        }
+        vt_wave_bind(g.wave2, g.wave2_mem, (uint32_t)paths);
+        g.wave2_cap = paths;
+    }
+    g.wave2.gsamp = nullptr;
+    const size_t n = (size_t)g.params.width * g.params.height;
+    if (n > g.acc_cap || !g.acc) {
+        hipFree(g.acc);
+        g.acc = nullptr;
+        g.acc_cap = 0;
+        hipError_t e = hipMalloc(&g.acc, n * sizeof(float4));
+        if (e != hipSuccess) {
+            set_err("accumulation buffer allocation", e);
+            return -1;
+        }
+        g.acc_cap = n;
+    }
     return 0;
 }
 
@@ -357,13 +398,50 @@ SceneGPU scene_gpu() {
     return S;
 }
 
+// A path-traced frame of two or more samples runs as two sample chunks on two streams
+// (samples [0, spp/2) on the library stream, [spp/2, spp) on stream2, each with its own
+// wavefront block), so each chunk's launches fill the GPU while the other's drain; the second
+// chunk's accumulation waits for the first's, keeping the per-pixel sum in sample order.
+int render_split(const FrameGPU& F, const SceneGPU& S) {
+    const uint32_t spp = F.spp;
+    if (ensure_split_buffers(spp) != 0) return -1;
+    ReqGPU A;
+    std::memset(&A, 0, sizeof(A));
+    A.F = F;
+    A.S = S;
+    A.hits = g.hits;
+    A.acc = g.acc;
+    A.rad = reinterpret_cast<float4*>(g.rad);
+    A.gbuf = nullptr;
+    A.s0 = 0;
+    A.s1 = spp / 2;
+    A.final_chunk = 0;
+    A.tile_w = 8;
+    ReqGPU B = A;
+    B.s0 = spp / 2;
+    B.s1 = spp;
+    B.final_chunk = 1;
+    uint8_t kinds_b[VT_MAX_STAGES];
+    uint32_t ns_b = 0;
+    hipStreamWaitEvent(g.stream2, g.ev0, 0);
+    int rc = vt_trace_batch(&A, 1, g.wave, g.stats, nullptr, g.stage_kinds, &g.nstages, g.stream);
+    hipEventRecord(g.ev_a, g.stream);
+    if (rc == 0) rc = vt_trace_batch(&B, 1, g.wave2, g.stats, nullptr, kinds_b, &ns_b, g.stream2, g.ev_a);
+    hipEventRecord(g.ev_b, g.stream2);
+    hipStreamWaitEvent(g.stream, g.ev_b, 0);
+    return rc;
+}
+
 int render_frame(const FrameGPU& F) {
     if (ensure_frame_buffers() != 0) return -1;
     if (F.stats) hipMemsetAsync(g.stats, 0, (size_t)VT_MAX_STAGES * VT_STAT_COUNT * 8, g.stream);
     hipEventRecord(g.ev0, g.stream);
-    int rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.gbuf_on ? g.gbuf : nullptr,
-                             g.wave, g.stats, nullptr,
-                             g.stage_kinds, &g.nstages, g.stream);
+    int rc;
+    if (g.split && F.mode >= 1 && F.spp >= 2 && !g.gbuf_on)
+        rc = render_split(F, scene_gpu());
+    else
+        rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.gbuf_on ? g.gbuf : nullptr,
+                             g.wave, g.stats, nullptr, g.stage_kinds, &g.nstages, g.stream);
     hipEventRecord(g.ev1, g.stream);
     if (rc != 0) {
         set_err("trace launch failed", hipGetLastError());
@@ -462,6 +540,9 @@ uint64_t entry(void) {
     g.device = 0;
     if ((e = hipSetDevice(g.device)) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&g.stream2, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&g.ev_a, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&g.ev_b, hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreate(&g.ev0)) != hipSuccess || (e = hipEventCreate(&g.ev1)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&g.inst_copied, hipEventDisableTiming)) != hipSuccess ||
         (e = hipMalloc(&g.vol_table, (size_t)kMaxTextures * sizeof(VolumeDesc))) != hipSuccess ||
@@ -543,8 +624,12 @@ void cleanup(void) {
     hipFree(g.inst_dev); hipFree(g.vol_table); hipFree(g.staging); hipFree(g.remap);
     hipFree(g.hits); hipFree(g.rad); hipFree(g.bgra); hipFree(g.gbuf); hipFree(g.gsamp); hipFree(g.stats); hipFree(g.srgb_dec); hipFree(g.srgb_thr);
     hipFree(g.wave_mem);
+    hipStreamSynchronize(g.stream2);
+    hipFree(g.wave2_mem); hipFree(g.acc);
     hipEventDestroy(g.ev0); hipEventDestroy(g.ev1); hipEventDestroy(g.inst_copied);
+    hipEventDestroy(g.ev_a); hipEventDestroy(g.ev_b);
     hipStreamDestroy(g.stream);
+    hipStreamDestroy(g.stream2);
     vt_trace_set_tables(nullptr, nullptr);
     vt_trace_release();
     g = State{};
@@ -724,8 +809,39 @@ int32_t vt_read_stage_stats(uint64_t* dst, uint32_t max_stages, uint8_t* kinds) 
 
 float vt_last_frame_ms(void) { return g.last_ms; }
 
+// Whole frames as render_tick renders them (two streams for path-traced frames of 2+ samples,
+// each frame complete before the next starts): kernel_ms = mean GPU time per frame (HIP events
+// around each frame), wall_ms = mean host time per frame including the per-frame wait.
 int32_t vt_bench_frames(uint32_t warmup, uint32_t iters, float* kernel_ms, float* wall_ms) {
-    return vt_bench_stages(warmup, iters, kernel_ms, wall_ms, nullptr, 0, nullptr) < 0 ? -1 : 0;
+    Guard G;
+    if (!g.inited || !g.have_frame || iters == 0) return -1;
+    FrameGPU F = g.last_frame;
+    F.stats = 0;
+    for (uint32_t i = 0; i < warmup; ++i) {
+        if (render_frame(F) != 0) return -1;
+        hipEventSynchronize(g.ev1);
+    }
+    double gpu = 0.0;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t i = 0; i < iters; ++i) {
+        if (render_frame(F) != 0) return -1;
+        hipEventSynchronize(g.ev1);
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, g.ev0, g.ev1);
+        gpu += ms;
+    }
+    const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (kernel_ms) *kernel_ms = (float)(gpu / iters);
+    if (wall_ms) *wall_ms = (float)(wall / iters);
+    return 0;
+}
+
+int32_t vt_set_frame_split(uint32_t on) {
+    Guard G;
+    if (!g.inited) return -1;
+    hipStreamSynchronize(g.stream);
+    g.split = on != 0;
+    return 0;
 }
 
 int32_t vt_bench_stages(uint32_t warmup, uint32_t iters, float* kernel_ms, float* wall_ms, float* stage_ms,

@@ -78,8 +78,10 @@ int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float
 int vt_encode_srgb8(const float4* rad, uint32_t* bgra, uint32_t n, hipStream_t s);
 // Batched path tracing (modes 1/2): n requests, their sample chunks traced in one pipeline.
 bool vt_batch_layout(const ReqGPU* reqs, uint32_t n, uint32_t npaths, BatchGPU& B);
+// acc_after (optional): the accumulation launch waits for this event (a previous sample chunk
+// of the same pixels accumulating on another stream).
 int vt_trace_batch(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, unsigned long long* stats, hipEvent_t* ev,
-                   uint8_t* kinds, uint32_t* nstages, hipStream_t s);
+                   uint8_t* kinds, uint32_t* nstages, hipStream_t s, hipEvent_t acc_after = nullptr);
 int vt_trace_capture(uint32_t max_bounces, bool deep, const WaveBuffers& wb, hipStream_t s, hipGraphExec_t* out);
 int vt_trace_upload_tables(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, hipStream_t s, BatchGPU* layout);
 void vt_trace_release();
