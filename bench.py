@@ -8,7 +8,7 @@ A step is one frame: LBVH traversal + brick DDA + shading for every pixel sample
 volumes, LBVH and instances already resident in HBM.  Rays counted = traversal queries
 (primary + shadow + bounce) from the library's own counters on an untimed counting frame.
 `value` = rays per frame / mean host time per frame over the K timed frames, rendered as
-render_tick renders them (two sample chunks on two streams, each frame finished before the
+render_tick renders them (four sample chunks on four streams, each frame finished before the
 next).  The launch-stage table and the roofline come from the same frames on one stream with
 HIP events around every launch (`single_stream_ms_per_frame`).
 
@@ -164,7 +164,7 @@ def main():
     r.render(sc.P, sc.C)
     # timed: W untimed frames, then exactly K frames as render_tick renders them (each complete
     # before the next); wall_ms = host time per frame, kernel_ms = GPU time per frame
-    r.lib.vt_set_frame_split(0 if args.single_stream else 1)
+    r.lib.vt_set_frame_split(1 if args.single_stream else 4)
     r.bench(args.warmup, 1) if args.warmup > 0 else None
     r.lib.vt_synchronize()
     t0 = time.perf_counter()
@@ -173,9 +173,9 @@ def main():
     t_all = time.perf_counter() - t0
     # launch-stage breakdown (untimed for the metric): the same frames on one stream, HIP events
     # around every launch; the roofline of the dominant kernel comes from these
-    r.lib.vt_set_frame_split(0)
+    r.lib.vt_set_frame_split(1)
     single_kernel_ms, single_wall_ms, stage_ms = r.bench_stages(args.warmup, args.steps)
-    r.lib.vt_set_frame_split(0 if args.single_stream else 1)
+    r.lib.vt_set_frame_split(1 if args.single_stream else 4)
     # per-frame latency distribution (render_tick round trips, GPU time)
     lat = []
     for _ in range(max(5, min(args.steps, 20))):

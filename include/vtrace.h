@@ -101,9 +101,10 @@ int32_t vt_add_textures(const uint8_t* rgba8, uint32_t count, uint32_t width, ui
  * previous one to last_mouse_x/y, lib/entry.c:157-163); keys are written into the struct from
  * get_input_data_pointer directly, as the reference's key callback does (lib/entry.c:31-52). */
 void vt_set_cursor(double x, double y);
-/* 1 (default): a path-traced frame of 2+ samples without G-buffer runs as two sample chunks on
- * two streams (results identical, the per-pixel sum stays in sample order); 0: one stream. */
-int32_t vt_set_frame_split(uint32_t on);
+/* Path-traced frames of 2+ samples without G-buffer run as min(chunks, spp) sample chunks on
+ * as many streams (chunks <= 4, default 4; 0 or 1: one stream).  Results are identical: each
+ * chunk's accumulation follows the previous one's, so the per-pixel sum stays in sample order. */
+int32_t vt_set_frame_split(uint32_t chunks);
 /* Number of LBVH builds so far (end_update_instances calls that rebuilt the tree). */
 uint64_t vt_lbvh_builds(void);
 

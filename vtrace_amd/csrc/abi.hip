@@ -9,6 +9,7 @@
 // (lib/command.c:111-251).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -23,6 +24,7 @@
 namespace {
 
 constexpr uint32_t kMaxTextures = 65536;  // lib/common.h:35
+constexpr uint32_t kMaxChunks = 4;         // sample chunks (streams) per frame
 
 struct State {
     bool inited = false;
@@ -62,14 +64,15 @@ struct State {
     void* wave_mem = nullptr;   // wavefront path state + queues
     size_t wave_cap = 0;        // pixel samples it holds
     WaveBuffers wave;
-    // two-stream frames: samples [spp/2, spp) on stream2 with their own wavefront block
-    bool split = true;
-    hipStream_t stream2 = nullptr;
-    hipEvent_t ev_a = nullptr, ev_b = nullptr;
-    void* wave2_mem = nullptr;
-    size_t wave2_cap = 0;
-    WaveBuffers wave2;
-    float4* acc = nullptr;      // per pixel running sum between the two sample chunks
+    // frames in sample chunks on several streams: chunk k > 0 on xstream[k] with its own
+    // wavefront block; ev_chunk[k] marks chunk k accumulated
+    uint32_t split = kMaxChunks;  // chunks per path-traced frame (1 = one stream)
+    hipStream_t xstream[kMaxChunks] = {};
+    hipEvent_t ev_chunk[kMaxChunks] = {};
+    void* xwave_mem[kMaxChunks] = {};
+    size_t xwave_cap[kMaxChunks] = {};
+    WaveBuffers xwave[kMaxChunks];
+    float4* acc = nullptr;      // per pixel running sum between the sample chunks
     size_t acc_cap = 0;
     unsigned long long* stats = nullptr;  // VT_MAX_STAGES x VT_STAT_COUNT
     uint8_t stage_kinds[VT_MAX_STAGES] = {};
@@ -152,33 +155,36 @@ int ensure_wave_buffers() {
     return 0;
 }
 
-// second wavefront block and the per-pixel running sum for two-stream frames
-int ensure_split_buffers(uint32_t spp) {
-    const size_t paths = padded_paths(g.params.width, g.params.height, spp - spp / 2);
-    if (paths > g.wave2_cap || !g.wave2_mem) {
-        hipFree(g.wave2_mem);
-        g.wave2_mem = nullptr;
-        g.wave2_cap = 0;
-        hipError_t e = hipMalloc(&g.wave2_mem, vt_wave_bytes((uint32_t)paths));
-        if (e != hipSuccess) {
-            set_err("wavefront buffer allocation", e);
-            return -1;
+// wavefront blocks of chunks 1..n-1 and the per-pixel running sum for split frames
+int ensure_split_buffers(uint32_t spp, uint32_t n) {
+    for (uint32_t k = 1; k < n; ++k) {
+        const uint32_t s0 = k * spp / n, s1 = (k + 1) * spp / n;
+        const size_t paths = padded_paths(g.params.width, g.params.height, s1 - s0);
+        if (paths > g.xwave_cap[k] || !g.xwave_mem[k]) {
+            hipFree(g.xwave_mem[k]);
+            g.xwave_mem[k] = nullptr;
+            g.xwave_cap[k] = 0;
+            hipError_t e = hipMalloc(&g.xwave_mem[k], vt_wave_bytes((uint32_t)paths));
+            if (e != hipSuccess) {
+                set_err("wavefront buffer allocation", e);
+                return -1;
+            }
+            vt_wave_bind(g.xwave[k], g.xwave_mem[k], (uint32_t)paths);
+            g.xwave_cap[k] = paths;
         }
-        vt_wave_bind(g.wave2, g.wave2_mem, (uint32_t)paths);
-        g.wave2_cap = paths;
+        g.xwave[k].gsamp = nullptr;
     }
-    g.wave2.gsamp = nullptr;
-    const size_t n = (size_t)g.params.width * g.params.height;
-    if (n > g.acc_cap || !g.acc) {
+    const size_t np = (size_t)g.params.width * g.params.height;
+    if (np > g.acc_cap || !g.acc) {
         hipFree(g.acc);
         g.acc = nullptr;
         g.acc_cap = 0;
-        hipError_t e = hipMalloc(&g.acc, n * sizeof(float4));
+        hipError_t e = hipMalloc(&g.acc, np * sizeof(float4));
         if (e != hipSuccess) {
             set_err("accumulation buffer allocation", e);
             return -1;
         }
-        g.acc_cap = n;
+        g.acc_cap = np;
     }
     return 0;
 }
@@ -398,37 +404,36 @@ SceneGPU scene_gpu() {
     return S;
 }
 
-// A path-traced frame of two or more samples runs as two sample chunks on two streams
-// (samples [0, spp/2) on the library stream, [spp/2, spp) on stream2, each with its own
-// wavefront block), so each chunk's launches fill the GPU while the other's drain; the second
-// chunk's accumulation waits for the first's, keeping the per-pixel sum in sample order.
-int render_split(const FrameGPU& F, const SceneGPU& S) {
+// A path-traced frame of n >= 2 samples runs as up to g.split sample chunks on as many streams
+// (chunk 0 on the library stream, chunk k on xstream[k], each with its own wavefront block),
+// so each chunk's launches fill the GPU while another's drain.  Chunk k accumulates after
+// chunk k-1, keeping the per-pixel sum in sample order; the library stream waits for the last.
+int render_split(const FrameGPU& F, const SceneGPU& S, uint32_t n) {
     const uint32_t spp = F.spp;
-    if (ensure_split_buffers(spp) != 0) return -1;
-    ReqGPU A;
-    std::memset(&A, 0, sizeof(A));
-    A.F = F;
-    A.S = S;
-    A.hits = g.hits;
-    A.acc = g.acc;
-    A.rad = reinterpret_cast<float4*>(g.rad);
-    A.gbuf = nullptr;
-    A.s0 = 0;
-    A.s1 = spp / 2;
-    A.final_chunk = 0;
-    A.tile_w = 8;
-    ReqGPU B = A;
-    B.s0 = spp / 2;
-    B.s1 = spp;
-    B.final_chunk = 1;
-    uint8_t kinds_b[VT_MAX_STAGES];
-    uint32_t ns_b = 0;
-    hipStreamWaitEvent(g.stream2, g.ev0, 0);
-    int rc = vt_trace_batch(&A, 1, g.wave, g.stats, nullptr, g.stage_kinds, &g.nstages, g.stream);
-    hipEventRecord(g.ev_a, g.stream);
-    if (rc == 0) rc = vt_trace_batch(&B, 1, g.wave2, g.stats, nullptr, kinds_b, &ns_b, g.stream2, g.ev_a);
-    hipEventRecord(g.ev_b, g.stream2);
-    hipStreamWaitEvent(g.stream, g.ev_b, 0);
+    if (ensure_split_buffers(spp, n) != 0) return -1;
+    ReqGPU R;
+    std::memset(&R, 0, sizeof(R));
+    R.F = F;
+    R.S = S;
+    R.hits = g.hits;
+    R.acc = g.acc;
+    R.rad = reinterpret_cast<float4*>(g.rad);
+    R.gbuf = nullptr;
+    R.tile_w = 8;
+    int rc = 0;
+    for (uint32_t k = 0; k < n && rc == 0; ++k) {
+        R.s0 = k * spp / n;
+        R.s1 = (k + 1) * spp / n;
+        R.final_chunk = k + 1 == n ? 1 : 0;
+        hipStream_t s = k == 0 ? g.stream : g.xstream[k];
+        if (k > 0) hipStreamWaitEvent(s, g.ev0, 0);
+        uint8_t kinds[VT_MAX_STAGES];
+        uint32_t ns = 0;
+        rc = vt_trace_batch(&R, 1, k == 0 ? g.wave : g.xwave[k], g.stats, nullptr, k == 0 ? g.stage_kinds : kinds,
+                            k == 0 ? &g.nstages : &ns, s, k == 0 ? nullptr : g.ev_chunk[k - 1]);
+        hipEventRecord(g.ev_chunk[k], s);
+    }
+    hipStreamWaitEvent(g.stream, g.ev_chunk[n - 1], 0);
     return rc;
 }
 
@@ -437,8 +442,9 @@ int render_frame(const FrameGPU& F) {
     if (F.stats) hipMemsetAsync(g.stats, 0, (size_t)VT_MAX_STAGES * VT_STAT_COUNT * 8, g.stream);
     hipEventRecord(g.ev0, g.stream);
     int rc;
-    if (g.split && F.mode >= 1 && F.spp >= 2 && !g.gbuf_on)
-        rc = render_split(F, scene_gpu());
+    const uint32_t n = std::min(g.split, std::min(F.spp, kMaxChunks));
+    if (n >= 2 && F.mode >= 1 && !g.gbuf_on)
+        rc = render_split(F, scene_gpu(), n);
     else
         rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.gbuf_on ? g.gbuf : nullptr,
                              g.wave, g.stats, nullptr, g.stage_kinds, &g.nstages, g.stream);
@@ -540,9 +546,6 @@ uint64_t entry(void) {
     g.device = 0;
     if ((e = hipSetDevice(g.device)) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking)) != hipSuccess ||
-        (e = hipStreamCreateWithFlags(&g.stream2, hipStreamNonBlocking)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&g.ev_a, hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&g.ev_b, hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreate(&g.ev0)) != hipSuccess || (e = hipEventCreate(&g.ev1)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&g.inst_copied, hipEventDisableTiming)) != hipSuccess ||
         (e = hipMalloc(&g.vol_table, (size_t)kMaxTextures * sizeof(VolumeDesc))) != hipSuccess ||
@@ -550,6 +553,13 @@ uint64_t entry(void) {
         (e = hipMalloc(&g.srgb_dec, 256 * 4)) != hipSuccess || (e = hipMalloc(&g.srgb_thr, 255 * 4)) != hipSuccess) {
         set_err("init", e);
         return ((uint64_t)(uint32_t)e << 32) | 1u;
+    }
+    for (uint32_t k = 0; k < kMaxChunks; ++k) {  // default priority, like the library stream
+        if ((k > 0 && (e = hipStreamCreateWithFlags(&g.xstream[k], hipStreamNonBlocking)) != hipSuccess) ||
+            (e = hipEventCreateWithFlags(&g.ev_chunk[k], hipEventDisableTiming)) != hipSuccess) {
+            set_err("init", e);
+            return ((uint64_t)(uint32_t)e << 32) | 1u;
+        }
     }
     float dec[256], thr[255];
     for (int i = 0; i < 256; ++i) dec[i] = (float)srgb_lin((double)i / 255.0);
@@ -624,12 +634,15 @@ void cleanup(void) {
     hipFree(g.inst_dev); hipFree(g.vol_table); hipFree(g.staging); hipFree(g.remap);
     hipFree(g.hits); hipFree(g.rad); hipFree(g.bgra); hipFree(g.gbuf); hipFree(g.gsamp); hipFree(g.stats); hipFree(g.srgb_dec); hipFree(g.srgb_thr);
     hipFree(g.wave_mem);
-    hipStreamSynchronize(g.stream2);
-    hipFree(g.wave2_mem); hipFree(g.acc);
+    for (uint32_t k = 0; k < kMaxChunks; ++k) {
+        if (g.xstream[k]) hipStreamSynchronize(g.xstream[k]);
+        hipFree(g.xwave_mem[k]);
+        if (g.ev_chunk[k]) hipEventDestroy(g.ev_chunk[k]);
+        if (g.xstream[k]) hipStreamDestroy(g.xstream[k]);
+    }
+    hipFree(g.acc);
     hipEventDestroy(g.ev0); hipEventDestroy(g.ev1); hipEventDestroy(g.inst_copied);
-    hipEventDestroy(g.ev_a); hipEventDestroy(g.ev_b);
     hipStreamDestroy(g.stream);
-    hipStreamDestroy(g.stream2);
     vt_trace_set_tables(nullptr, nullptr);
     vt_trace_release();
     g = State{};
@@ -836,11 +849,11 @@ int32_t vt_bench_frames(uint32_t warmup, uint32_t iters, float* kernel_ms, float
     return 0;
 }
 
-int32_t vt_set_frame_split(uint32_t on) {
+int32_t vt_set_frame_split(uint32_t chunks) {
     Guard G;
-    if (!g.inited) return -1;
+    if (!g.inited || chunks > kMaxChunks) return -1;
     hipStreamSynchronize(g.stream);
-    g.split = on != 0;
+    g.split = chunks ? chunks : 1;
     return 0;
 }
 
