@@ -148,6 +148,7 @@ def main():
     sc = scenes.config2(args.width, args.height)
     t_gen = time.perf_counter() - t0
     r = R.get_renderer()
+    r.lib.vt_set_frame_split(1 if args.single_stream else 4)
     t0 = time.perf_counter()
     R.load_scene(r, sc, stats=True)
     t_up = time.perf_counter() - t0
