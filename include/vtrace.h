@@ -168,6 +168,9 @@ typedef struct vt_serve_config {
     uint32_t use_graph;         /* 1: replay the iteration's launch sequence from a hipGraph */
     uint32_t policy;            /* VT_SERVE_ROUND_ROBIN, VT_SERVE_HEAD_RIDERS, VT_SERVE_SHORTEST_FIRST */
     uint32_t pipeline;          /* iterations in flight: 1, or 2 (queue the next before waiting) */
+    uint32_t streams;           /* an iteration of 2^18+ pixel samples is spread over up to this many
+                                   streams (1..4, 0 = 1): sub-batches of about equal work, a request's
+                                   samples in consecutive pieces accumulated in order */
 } vt_serve_config;
 
 /* round robin: every admitted request traces its next spp_chunk samples per iteration;
@@ -195,7 +198,7 @@ typedef struct vt_serve_stats {
     uint64_t iterations;
     uint64_t samples;           /* pixel samples traced so far */
     float last_iteration_ms;
-    uint32_t pad;
+    uint32_t sub_batches;       /* sub-batch launch sequences so far (>= iterations) */
 } vt_serve_stats;
 
 /* Sets up the serving loop (replaces any previous one). */

@@ -92,6 +92,34 @@ def test_alone_vs_mixed_and_graph_replays():
         assert np.array_equal(results["mixed"][i], results["graph"][i]), f"request {i}"
 
 
+@pytest.mark.parametrize("graph,policy", [(False, serving.ROUND_ROBIN), (True, serving.ROUND_ROBIN),
+                                          (True, serving.HEAD_RIDERS)])
+def test_sub_batches_on_streams_equal_plain(graph, policy):
+    """Iterations large enough to be spread over several streams (requests cut into consecutive
+    sample pieces of consecutive sub-batches) give every request the plain render bit for bit."""
+    r = R.reset_renderer()
+    _, recs = _setup(r)
+    rng = scenes.Rng(91)
+    reqs = []
+    for i, (sc, w, h, spp) in enumerate([(1, 320, 240, 6), (3, 256, 200, 5), (0, 300, 180, 3), (2, 160, 120, 7)]):
+        n = (32, 64, 48, 60)[sc]
+        eye, target = scenes.orbit_camera(rng, (0.0, 0.0, 0.0) if sc < 3 else (30.0, 0.0, 0.0), 1.6 * n, 2.0)
+        reqs.append(serving.Request(sc, w, h, spp, scenes.perspective(scenes.FOV, w / h), scenes.look_at(eye, target),
+                                    seed=900 + i, max_bounces=3, sun_dir=scenes.sun_direction(rng)))
+    loop = serving.ServingLoop(r, max_slots=8, spp_chunk=4, memory_budget=256 << 20, max_work=1 << 20, tile_w=8,
+                               max_bounces=3, use_graph=graph, policy=policy, pipeline=2, streams=4)
+    for rc in recs:
+        loop.add_scene(rc)
+    _say(f"case: serving sub-batches on 4 streams, graph {graph}, policy {policy}")
+    got = loop.run_all(reqs)
+    st = loop.status()
+    loop.close()
+    assert st["sub_batches"] > st["iterations"], st
+    for i, q in enumerate(reqs):
+        want, _ = _plain(r, recs, q)
+        assert np.array_equal(got[i].view(np.uint32), want.view(np.uint32)), f"request {i}"
+
+
 def test_budget_admission_queues_and_completes():
     r = R.reset_renderer()
     _, recs = _setup(r)

@@ -100,6 +100,9 @@ struct Inflight {
     uint64_t work = 0;
 };
 
+constexpr uint32_t kSubBatches = 4;           // streams an iteration is spread over
+constexpr uint64_t kMinSubWork = 1ull << 18;  // pixel samples below which an iteration is not split
+
 struct Serve {
     bool on = false;
     vt_serve_config cfg{};
@@ -107,6 +110,13 @@ struct Serve {
     Arena arena;
     void* wave_mem = nullptr;
     WaveBuffers wave;
+    // sub-batches 1..streams-1: own wavefront block (work budget + riders' quarter), stream,
+    // graphs; acc_done[j] marks sub-batch j accumulated
+    void* sub_mem[kSubBatches] = {};
+    WaveBuffers sub_wave[kSubBatches];
+    hipStream_t sub[kSubBatches] = {};
+    hipGraphExec_t sub_graph[kSubBatches][2] = {};
+    hipEvent_t acc_done[kSubBatches] = {}, start = nullptr;
     std::vector<ServeScene> scenes;
     std::deque<int64_t> queue;
     std::vector<int64_t> active;
@@ -117,6 +127,7 @@ struct Serve {
     Inflight fl[2];
     uint32_t fl_head = 0, fl_count = 0;
     uint64_t iterations = 0, samples = 0;
+    uint32_t sub_batches = 0;
     float last_ms = 0.0f;
 };
 
@@ -137,6 +148,15 @@ void free_all() {
     sv.scenes.clear();
     for (auto& gx : sv.graph)
         if (gx) hipGraphExecDestroy(gx);
+    for (uint32_t j = 0; j < kSubBatches; ++j) {
+        if (sv.sub[j]) hipStreamSynchronize(sv.sub[j]);
+        for (auto& gx : sv.sub_graph[j])
+            if (gx) hipGraphExecDestroy(gx);
+        hipFree(sv.sub_mem[j]);
+        if (sv.sub[j]) hipStreamDestroy(sv.sub[j]);
+        if (sv.acc_done[j]) hipEventDestroy(sv.acc_done[j]);
+    }
+    if (sv.start) hipEventDestroy(sv.start);
     hipFree(sv.arena_mem);
     hipFree(sv.wave_mem);
     for (auto& f : sv.fl) {
@@ -258,12 +278,13 @@ int32_t vt_serve_init(const vt_serve_config* cfg) {
     vt_serve_config c = *cfg;
     if (c.max_slots == 0 || c.max_slots > VT_MAX_BATCH || c.memory_budget < 48 * 64 || c.max_work < 64 ||
         c.spp_chunk == 0 || c.max_bounces > 29 || (c.tile_w && 64u % c.tile_w != 0) || c.policy > 2 ||
-        c.pipeline > 2) {
+        c.pipeline > 2 || c.streams > kSubBatches) {
         vt_rt_set_error("vt_serve_init: invalid configuration");
         return -1;
     }
     if (c.tile_w == 0) c.tile_w = 8;
     if (c.pipeline == 0) c.pipeline = 1;
+    if (c.streams == 0) c.streams = 1;
     // path state: the work budget plus the riders' quarter, or one sample of the largest
     // frame the budget admits
     const size_t max_px = c.memory_budget / 48;
@@ -279,6 +300,15 @@ int32_t vt_serve_init(const vt_serve_config* cfg) {
     for (auto& f : sv.fl)
         ok = ok && hipEventCreateWithFlags(&f.done, hipEventDisableTiming) == hipSuccess &&
              hipEventCreate(&f.t0) == hipSuccess && hipEventCreate(&f.t1) == hipSuccess;
+    const size_t sub_paths = (size_t)c.max_work + c.max_work / 4;
+    ok = ok && hipEventCreateWithFlags(&sv.start, hipEventDisableTiming) == hipSuccess;
+    for (uint32_t j = 0; j < c.streams && ok; ++j) {
+        ok = hipEventCreateWithFlags(&sv.acc_done[j], hipEventDisableTiming) == hipSuccess;
+        if (j > 0)  // default priority, like the library stream
+            ok = ok && hipStreamCreateWithFlags(&sv.sub[j], hipStreamNonBlocking) == hipSuccess &&
+                 hipMalloc(&sv.sub_mem[j], vt_wave_bytes((uint32_t)sub_paths)) == hipSuccess;
+        if (ok && j > 0) vt_wave_bind(sv.sub_wave[j], sv.sub_mem[j], (uint32_t)sub_paths);
+    }
     if (!ok) {
         vt_rt_set_error("vt_serve_init: allocation failed");
         free_all();
@@ -369,11 +399,24 @@ int32_t vt_serve_step(int64_t* completed, uint32_t max_completed) {
     if (sv.fl_count < c.pipeline) choose(chunks, work);
     int rc = 0;
     if (!chunks.empty()) {
-        std::vector<ReqGPU> batch;
         Inflight& f = sv.fl[(sv.fl_head + sv.fl_count) % 2];
         f.finishing.clear();
+        // 3. split the iteration into sub-batches of about equal work: requests in order, a
+        // request's samples cut at a boundary into consecutive pieces of consecutive
+        // sub-batches; sub-batch j accumulates after sub-batch j-1, so each request's pieces
+        // add up in sample order
+        uint64_t max_ps = 0;
+        for (auto& ch : chunks) max_ps = std::max(max_ps, per_sample(sv.reqs[ch.first], c.tile_w));
+        uint32_t nsub = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.streams, work / kMinSubWork));
+        const uint64_t sub_cap = (uint64_t)c.max_work + c.max_work / 4;
+        uint64_t target = (work + nsub - 1) / nsub;
+        if (nsub > 1 && target + max_ps > sub_cap) nsub = 1, target = work;
+        std::vector<std::vector<ReqGPU>> batch(nsub);
+        uint32_t j = 0;
+        uint64_t load = 0;
         for (auto& ch : chunks) {
             ServeReq& r = sv.reqs[ch.first];
+            const uint64_t ps = per_sample(r, c.tile_w);
             ReqGPU R;
             std::memset(&R, 0, sizeof(R));
             R.F = r.F;
@@ -389,26 +432,57 @@ int32_t vt_serve_step(int64_t* completed, uint32_t max_completed) {
             R.hits = reinterpret_cast<HitRecord*>(base);
             R.acc = reinterpret_cast<float4*>(base + npx * 16);
             R.rad = reinterpret_cast<float4*>(base + npx * 32);
-            R.s0 = r.next_s;
-            R.s1 = r.next_s + ch.second;
-            R.final_chunk = R.s1 == r.q.spp ? 1u : 0u;
             R.tile_w = c.tile_w;
-            batch.push_back(R);
-            r.next_s = R.s1;
-            if (R.final_chunk) f.finishing.push_back(ch.first);
+            uint32_t left = ch.second;
+            while (left > 0) {
+                if (j + 1 < nsub && load >= target) {
+                    ++j;
+                    load = 0;
+                }
+                uint32_t take = left;
+                if (j + 1 < nsub)
+                    take = (uint32_t)std::min<uint64_t>(left, std::max<uint64_t>(1, (target - load + ps - 1) / ps));
+                R.s0 = r.next_s;
+                R.s1 = r.next_s + take;
+                R.final_chunk = R.s1 == r.q.spp ? 1u : 0u;
+                batch[j].push_back(R);
+                load += ps * take;
+                r.next_s = R.s1;
+                left -= take;
+                if (R.final_chunk) f.finishing.push_back(ch.first);
+            }
         }
-        // 3. one launch for all chunks
+        while (nsub > 1 && batch[nsub - 1].empty()) --nsub;
+        for (uint32_t k = 0; k < nsub; ++k)
+            if (batch[k].size() > VT_MAX_BATCH) nsub = 0;  // cannot happen: pieces <= chunks + nsub - 1
+        if (nsub == 0) {
+            vt_rt_set_error("vt_serve_step: batch too large");
+            return -1;
+        }
+        // 4. one launch sequence per sub-batch, sub-batch 0 on the library stream
         bool deep = false;
-        for (auto& R : batch) deep = deep || R.S.bvh_depth + 1u > 16u;
+        for (auto& ch : chunks) deep = deep || sv.scenes[sv.reqs[ch.first].q.scene].depth + 1u > 16u;
         hipEventRecord(f.t0, s);
-        if (c.use_graph) {
-            hipGraphExec_t& gx = sv.graph[deep ? 1 : 0];
-            if (!gx) rc = vt_trace_capture(c.max_bounces, deep, sv.wave, s, &gx);
-            if (rc == 0) rc = vt_trace_upload_tables(batch.data(), (uint32_t)batch.size(), sv.wave, s, nullptr);
-            if (rc == 0) rc = hipGraphLaunch(gx, s) == hipSuccess ? 0 : -1;
-        } else {
-            rc = vt_trace_batch(batch.data(), (uint32_t)batch.size(), sv.wave, nullptr, nullptr, nullptr, nullptr, s);
+        hipEventRecord(sv.start, s);
+        for (uint32_t k = 0; k < nsub && rc == 0; ++k) {
+            hipStream_t sk = k == 0 ? s : sv.sub[k];
+            const WaveBuffers& wk = k == 0 ? sv.wave : sv.sub_wave[k];
+            if (k > 0) hipStreamWaitEvent(sk, sv.start, 0);
+            hipEvent_t after = k == 0 ? nullptr : sv.acc_done[k - 1];
+            if (c.use_graph) {
+                hipGraphExec_t& gx = k == 0 ? sv.graph[deep ? 1 : 0] : sv.sub_graph[k][deep ? 1 : 0];
+                if (!gx) rc = vt_trace_capture(c.max_bounces, deep, wk, sk, &gx, false);
+                if (rc == 0) rc = vt_trace_upload_tables(batch[k].data(), (uint32_t)batch[k].size(), wk, sk, nullptr);
+                if (rc == 0) rc = hipGraphLaunch(gx, sk) == hipSuccess ? 0 : -1;
+                if (rc == 0 && after) hipStreamWaitEvent(sk, after, 0);
+                if (rc == 0) rc = vt_trace_accumulate(wk, sk);
+            } else {
+                rc = vt_trace_batch(batch[k].data(), (uint32_t)batch[k].size(), wk, nullptr, nullptr, nullptr, nullptr,
+                                    sk, after);
+            }
+            hipEventRecord(sv.acc_done[k], sk);
         }
+        if (nsub > 1) hipStreamWaitEvent(s, sv.acc_done[nsub - 1], 0);
         hipEventRecord(f.t1, s);
         hipEventRecord(f.done, s);
         if (rc != 0) {
@@ -416,6 +490,7 @@ int32_t vt_serve_step(int64_t* completed, uint32_t max_completed) {
             return -1;
         }
         f.work = work;
+        sv.sub_batches += nsub;
         ++sv.fl_count;
     }
     // 4. retire: keep at most pipeline - 1 iterations in flight after a launch; with nothing
@@ -463,6 +538,7 @@ int32_t vt_serve_status(vt_serve_stats* st) {
     st->iterations = sv.iterations;
     st->samples = sv.samples;
     st->last_iteration_ms = sv.last_ms;
+    st->sub_batches = sv.sub_batches;
     st->tile_w = sv.cfg.tile_w;
     return 0;
 }

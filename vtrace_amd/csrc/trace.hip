@@ -1294,7 +1294,8 @@ bool vt_batch_layout(const ReqGPU* reqs, uint32_t n, uint32_t npaths, BatchGPU& 
 
 template <bool ST, bool DEEP>
 static int launch_batch(const BatchGPU& Bh, const WaveBuffers& wb, unsigned long long* stats, hipEvent_t* ev,
-                        uint8_t* kinds, uint32_t* nstages, hipStream_t s, hipEvent_t acc_after = nullptr) {
+                        uint8_t* kinds, uint32_t* nstages, hipStream_t s, hipEvent_t acc_after = nullptr,
+                        bool with_acc = true) {
     const dim3 grid(g_persistent_blocks), block(kBlock);
     const ReqGPU* reqs = reinterpret_cast<const ReqGPU*>(wb.tables);
     const BatchGPU* B = reinterpret_cast<const BatchGPU*>(static_cast<const char*>(wb.tables) + sizeof(ReqGPU) * VT_MAX_BATCH);
@@ -1322,10 +1323,12 @@ static int launch_batch(const BatchGPU& Bh, const WaveBuffers& wb, unsigned long
         end();
         in_idx = out_idx;
     }
-    if (acc_after) hipStreamWaitEvent(s, acc_after, 0);  // another chunk's running sum comes first
-    begin(VT_STAGE_ACCUMULATE);
-    hipLaunchKernelGGL(k_accumulate, dim3(g_persistent_blocks), dim3(256), 0, s, reqs, B, wf);
-    end();
+    if (with_acc) {
+        if (acc_after) hipStreamWaitEvent(s, acc_after, 0);  // another chunk's running sum comes first
+        begin(VT_STAGE_ACCUMULATE);
+        hipLaunchKernelGGL(k_accumulate, dim3(g_persistent_blocks), dim3(256), 0, s, reqs, B, wf);
+        end();
+    }
     if (nstages) *nstages = stage;
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -1362,17 +1365,26 @@ int vt_trace_batch(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, unsign
 
 // Captures the launch sequence of a batch (fixed grids, tables read from device memory)
 // into a graph; the table upload stays outside the graph.
-int vt_trace_capture(uint32_t max_bounces, bool deep, const WaveBuffers& wb, hipStream_t s, hipGraphExec_t* out) {
+int vt_trace_capture(uint32_t max_bounces, bool deep, const WaveBuffers& wb, hipStream_t s, hipGraphExec_t* out,
+                     bool with_acc) {
     BatchGPU Bh{};
     Bh.max_bounces = max_bounces;
     hipGraph_t graph;
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return -1;
-    int rc = deep ? launch_batch<false, true>(Bh, wb, nullptr, nullptr, nullptr, nullptr, s)
-                  : launch_batch<false, false>(Bh, wb, nullptr, nullptr, nullptr, nullptr, s);
+    int rc = deep ? launch_batch<false, true>(Bh, wb, nullptr, nullptr, nullptr, nullptr, s, nullptr, with_acc)
+                  : launch_batch<false, false>(Bh, wb, nullptr, nullptr, nullptr, nullptr, s, nullptr, with_acc);
     if (hipStreamEndCapture(s, &graph) != hipSuccess || rc != 0) return -1;
     rc = hipGraphInstantiate(out, graph, nullptr, nullptr, 0) == hipSuccess ? 0 : -1;
     hipGraphDestroy(graph);
     return rc;
+}
+
+// The accumulation launch of a batch whose tables are in wb (after a graph captured without it).
+int vt_trace_accumulate(const WaveBuffers& wb, hipStream_t s) {
+    const ReqGPU* reqs = reinterpret_cast<const ReqGPU*>(wb.tables);
+    const BatchGPU* B = reinterpret_cast<const BatchGPU*>(static_cast<const char*>(wb.tables) + sizeof(ReqGPU) * VT_MAX_BATCH);
+    hipLaunchKernelGGL(k_accumulate, dim3(g_persistent_blocks), dim3(256), 0, s, reqs, B, to_wf(wb));
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 // Uploads a batch's request table only (for a captured graph launched afterwards).

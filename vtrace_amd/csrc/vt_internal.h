@@ -82,7 +82,10 @@ bool vt_batch_layout(const ReqGPU* reqs, uint32_t n, uint32_t npaths, BatchGPU& 
 // of the same pixels accumulating on another stream).
 int vt_trace_batch(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, unsigned long long* stats, hipEvent_t* ev,
                    uint8_t* kinds, uint32_t* nstages, hipStream_t s, hipEvent_t acc_after = nullptr);
-int vt_trace_capture(uint32_t max_bounces, bool deep, const WaveBuffers& wb, hipStream_t s, hipGraphExec_t* out);
+// with_acc = false: the graph stops before the accumulation (vt_trace_accumulate launches it)
+int vt_trace_capture(uint32_t max_bounces, bool deep, const WaveBuffers& wb, hipStream_t s, hipGraphExec_t* out,
+                     bool with_acc = true);
+int vt_trace_accumulate(const WaveBuffers& wb, hipStream_t s);
 int vt_trace_upload_tables(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, hipStream_t s, BatchGPU* layout);
 void vt_trace_release();
 
