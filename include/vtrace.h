@@ -105,6 +105,11 @@ void vt_set_cursor(double x, double y);
  * as many streams (chunks <= 4, default 4; 0 or 1: one stream).  Results are identical: each
  * chunk's accumulation follows the previous one's, so the per-pixel sum stays in sample order. */
 int32_t vt_set_frame_split(uint32_t chunks);
+/* Like end_update_instances for a list of the same length as the last committed one (moved
+ * instances): the records are committed and the LBVH keeps its topology, its boxes refitted
+ * bottom-up (no Morton sort or rebuild).  Hits are those of a rebuilt tree; the traversal
+ * order, and so the node counters, can differ.  0, or -1 if the count differs. */
+int32_t vt_refit_instances(uint32_t instance_count);
 /* Number of LBVH builds so far (end_update_instances calls that rebuilt the tree). */
 uint64_t vt_lbvh_builds(void);
 

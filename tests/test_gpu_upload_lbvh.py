@@ -75,3 +75,58 @@ def test_lbvh_matches_oracle(n):
     # inverse rows 0..2 (column-major 3x4) bit-exact
     inv_rows = inv.reshape(n, 4, 4)[:, :, :3].reshape(n, 12)
     assert np.array_equal(inst["inv"].view(np.uint32), inv_rows.view(np.uint32))
+
+
+@pytest.mark.parametrize("n", [1, 2, 37, 1000])
+def test_refit_keeps_topology_and_bounds_moved_instances(n):
+    """vt_refit_instances: same left/right links as the last build; leaf boxes equal the oracle's
+    instance boxes of the moved records; every internal box is the union of its children's."""
+    _say(f"case: lbvh refit n={n}")
+    r = R.reset_renderer()
+    for k in range(3):
+        r.upload(scenes.Volume(scenes.gen_volume((8, 8, 8), k, 0.5), scenes.gen_palette(k)))
+    recs = _records(n, 300 + n)
+    r.set_instances(recs)
+    before, _ = r.lbvh()
+    builds = r.lbvh_builds()
+    moved = recs.copy()
+    rng = np.random.default_rng(n)
+    moved[:, 12:15] += rng.uniform(-200.0, 200.0, (n, 3)).astype(np.float32)  # translation column
+    r.refit_instances(moved)
+    assert r.lbvh_builds() == builds
+    after, inst = r.lbvh()
+    assert np.array_equal(after["left"], before["left"]) and np.array_equal(after["right"], before["right"])
+    _, aabb, _, _ = O.instance_prep(moved, 3)
+    leaves = after[n - 1:]
+    for k in range(n):
+        i = int(leaves["left"][k])
+        assert np.array_equal(leaves["lo"][k], aabb[i, :3]) and np.array_equal(leaves["hi"][k], aabb[i, 3:]), k
+    for k in range(n - 1):
+        lft, rgt = after[after["left"][k]], after[after["right"][k]]
+        assert np.array_equal(after["lo"][k], np.minimum(lft["lo"], rgt["lo"]))
+        assert np.array_equal(after["hi"][k], np.maximum(lft["hi"], rgt["hi"]))
+    with pytest.raises(R.RendererError):
+        r.refit_instances(moved[: max(1, n - 1)] if n > 1 else np.concatenate([moved, moved]))
+
+
+def test_refit_frames_match_oracle_hits():
+    """After a refit the frame's hits equal the oracle's (which builds its own tree for the moved
+    instances): the closest hit does not depend on the traversal order."""
+    sc = scenes.multi_volume_scene(count=8, n=64, world=512, seed=33, width=160, height=120, spp=1, max_bounces=2)
+    r = R.reset_renderer()
+    ids = R.load_scene(r, sc)
+    r.render(sc.P, sc.C)
+    moved = np.array(sc.records, np.float32).copy()
+    rng = np.random.default_rng(5)
+    moved[:, 12:15] += rng.uniform(-40.0, 40.0, (moved.shape[0], 3)).astype(np.float32)
+    recs = moved.copy()
+    recs.view(np.uint32)[:, 15] = np.array([ids[t] for t in moved.view(np.uint32)[:, 15]], dtype=np.uint32)
+    _say("case: refit then render, 8 instances moved")
+    r.refit_instances(recs)
+    r.render(sc.P, sc.C)
+    sc2 = scenes.Scene(sc.volumes, moved, sc.P, sc.C, sc.params, name="moved")
+    o_hits, o_rad, _ = O.render(sc2, use_bvh=True)
+    from parity_util import assert_hits_equal, radiance_error
+    assert_hits_equal(r.hits(), o_hits, "refit")
+    mx, _ = radiance_error(r.radiance(), o_rad)
+    assert mx <= 1e-5

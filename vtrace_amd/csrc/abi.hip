@@ -717,6 +717,29 @@ int32_t end_update_instances(uint32_t instance_count) {
     return 0;
 }
 
+int32_t vt_refit_instances(uint32_t instance_count) {
+    Guard G;
+    if (!g.inited) return -1;
+    if (instance_count == 0) instance_count = 1;  // as end_update_instances
+    if (instance_count != g.inst_count || g.inst_prev.size() != (size_t)instance_count * 16) {
+        set_err("vt_refit_instances: the instance count differs from the last build");
+        return -1;
+    }
+    hipError_t e = hipMemcpyAsync(g.inst_dev, g.inst_host, (size_t)instance_count * 64, hipMemcpyHostToDevice, g.stream);
+    if (e != hipSuccess) {
+        set_err("instance copy", e);
+        return -1;
+    }
+    hipEventRecord(g.inst_copied, g.stream);
+    if (vt_lbvh_refit(g.bvh, g.inst_dev, instance_count, g.vol_table, (uint32_t)g.vols.size(), g.stream) != 0) {
+        set_err("lbvh refit", hipGetLastError());
+        return -1;
+    }
+    g.inst_prev.assign(g.inst_host, g.inst_host + (size_t)instance_count * 16);
+    g.inst_prev_vols = g.vols.size();
+    return 0;
+}
+
 uint64_t vt_lbvh_builds(void) {
     Guard G;
     return g.lbvh_builds;

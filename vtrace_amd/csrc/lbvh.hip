@@ -305,6 +305,11 @@ __global__ void k_refit(uint32_t n, BvhNode* nodes, const int32_t* parent, uint3
     }
 }
 
+__global__ void k_zero_counters(uint32_t* counters, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) counters[i] = 0u;
+}
+
 // Depth of the tree (edges on the longest root-leaf path): one thread per leaf walks up.
 __global__ void k_depth(uint32_t n, const int32_t* parent, uint32_t* depth_out) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -346,6 +351,23 @@ void vt_lbvh_free(LbvhWork& w) {
     hipFree(w.keys[0]); hipFree(w.keys[1]); hipFree(w.vals[0]); hipFree(w.vals[1]);
     hipFree(w.parent); hipFree(w.counters); hipFree(w.bounds); hipFree(w.depth);
     w = LbvhWork{};
+}
+
+// Refit: new records for the same n instances keep the last build's topology (leaf order and
+// parents); instance data and leaf boxes are recomputed, internal boxes rebuilt bottom-up.
+int vt_lbvh_refit(LbvhWork& w, const float* records_dev, uint32_t n, const VolumeDesc* vols_dev, uint32_t nvol,
+                  hipStream_t s) {
+    if (n == 0 || n > w.cap || !w.inst) return -1;
+    const uint32_t tb = 256, nb = (n + tb - 1) / tb;
+    BvhNode* leaves_tmp = reinterpret_cast<BvhNode*>(w.cen + 3 * (size_t)w.cap);
+    hipLaunchKernelGGL(k_instance_prep, dim3(nb), dim3(tb), 0, s, records_dev, n, vols_dev, nvol, w.inst, leaves_tmp,
+                       w.cen);
+    hipLaunchKernelGGL(k_leaves, dim3(nb), dim3(tb), 0, s, w.vals[0], n, leaves_tmp, w.nodes);
+    if (n > 1) {
+        hipLaunchKernelGGL(k_zero_counters, dim3(nb), dim3(tb), 0, s, w.counters, n - 1);
+        hipLaunchKernelGGL(k_refit, dim3(nb), dim3(tb), 0, s, n, w.nodes, w.parent, w.counters);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int vt_lbvh_build(LbvhWork& w, const float* records_dev, uint32_t n, const VolumeDesc* vols_dev, uint32_t nvol,

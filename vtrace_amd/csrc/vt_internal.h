@@ -48,6 +48,8 @@ int vt_lbvh_reserve(LbvhWork& w, uint32_t n);
 void vt_lbvh_free(LbvhWork& w);
 int vt_lbvh_build(LbvhWork& w, const float* records_dev, uint32_t n, const VolumeDesc* vols_dev, uint32_t nvol,
                   hipStream_t s);
+int vt_lbvh_refit(LbvhWork& w, const float* records_dev, uint32_t n, const VolumeDesc* vols_dev, uint32_t nvol,
+                  hipStream_t s);
 
 // trace.hip: wavefront path state and queues (one block of device memory, see vt_wave_bind)
 struct WaveBuffers {

@@ -233,6 +233,18 @@ class Renderer:
         if self.lib.end_update_instances(n) != 0:
             raise RendererError(f"end_update_instances failed: {self.error()}")
 
+    def refit_instances(self, records: np.ndarray):
+        """Moved instances, same count as the last committed list: vt_refit_instances."""
+        recs = np.ascontiguousarray(records, dtype=np.float32).reshape(-1, 16)
+        n = recs.shape[0]
+        ptr = self.lib.start_update_instances(n)
+        if not ptr:
+            raise RendererError("start_update_instances failed")
+        buf = np.ctypeslib.as_array(ptr, shape=(max(n, 1) * 16,))
+        buf[: n * 16] = recs.reshape(-1)
+        if self.lib.vt_refit_instances(n) != 0:
+            raise RendererError(f"vt_refit_instances failed: {self.error()}")
+
     # -- headless extensions ----------------------------------------------------------------
     def set_params(self, p: "scenes.RenderParams", stats: bool = False):
         s = VtRenderParams()
