@@ -1295,8 +1295,9 @@ bool vt_batch_layout(const ReqGPU* reqs, uint32_t n, uint32_t npaths, BatchGPU& 
 template <bool ST, bool DEEP>
 static int launch_batch(const BatchGPU& Bh, const WaveBuffers& wb, unsigned long long* stats, hipEvent_t* ev,
                         uint8_t* kinds, uint32_t* nstages, hipStream_t s, hipEvent_t acc_after = nullptr,
-                        bool with_acc = true) {
-    const dim3 grid(g_persistent_blocks), block(kBlock);
+                        bool with_acc = true, uint32_t blocks = 0) {
+    const uint32_t nblk = blocks ? blocks : g_persistent_blocks;
+    const dim3 grid(nblk), block(kBlock);
     const ReqGPU* reqs = reinterpret_cast<const ReqGPU*>(wb.tables);
     const BatchGPU* B = reinterpret_cast<const BatchGPU*>(static_cast<const char*>(wb.tables) + sizeof(ReqGPU) * VT_MAX_BATCH);
     WF wf = to_wf(wb);
@@ -1326,7 +1327,7 @@ static int launch_batch(const BatchGPU& Bh, const WaveBuffers& wb, unsigned long
     if (with_acc) {
         if (acc_after) hipStreamWaitEvent(s, acc_after, 0);  // another chunk's running sum comes first
         begin(VT_STAGE_ACCUMULATE);
-        hipLaunchKernelGGL(k_accumulate, dim3(g_persistent_blocks), dim3(256), 0, s, reqs, B, wf);
+        hipLaunchKernelGGL(k_accumulate, dim3(nblk), dim3(256), 0, s, reqs, B, wf);
         end();
     }
     if (nstages) *nstages = stage;
@@ -1335,7 +1336,7 @@ static int launch_batch(const BatchGPU& Bh, const WaveBuffers& wb, unsigned long
 
 // Uploads the request table (stream-ordered) and launches the batched pipeline.
 int vt_trace_batch(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, unsigned long long* stats, hipEvent_t* ev,
-                   uint8_t* kinds, uint32_t* nstages, hipStream_t s, hipEvent_t acc_after) {
+                   uint8_t* kinds, uint32_t* nstages, hipStream_t s, hipEvent_t acc_after, uint32_t blocks) {
     if (!g_srgb_dec || n == 0 || n > VT_MAX_BATCH || !wb.tables) return -1;
     if (ring_init() != 0) return -1;
     const int slot = g_ring.next;
@@ -1357,10 +1358,10 @@ int vt_trace_batch(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, unsign
     st = st && stats != nullptr;
     if (hipMemcpyAsync(wb.tables, h, kTableBytes, hipMemcpyHostToDevice, s) != hipSuccess) return -1;
     hipEventRecord(g_ring.ev[slot], s);
-    if (deep) return st ? launch_batch<true, true>(*Bh, wb, stats, ev, kinds, nstages, s, acc_after)
-                        : launch_batch<false, true>(*Bh, wb, stats, ev, kinds, nstages, s, acc_after);
-    return st ? launch_batch<true, false>(*Bh, wb, stats, ev, kinds, nstages, s, acc_after)
-              : launch_batch<false, false>(*Bh, wb, stats, ev, kinds, nstages, s, acc_after);
+    if (deep) return st ? launch_batch<true, true>(*Bh, wb, stats, ev, kinds, nstages, s, acc_after, true, blocks)
+                        : launch_batch<false, true>(*Bh, wb, stats, ev, kinds, nstages, s, acc_after, true, blocks);
+    return st ? launch_batch<true, false>(*Bh, wb, stats, ev, kinds, nstages, s, acc_after, true, blocks)
+              : launch_batch<false, false>(*Bh, wb, stats, ev, kinds, nstages, s, acc_after, true, blocks);
 }
 
 // Captures the launch sequence of a batch (fixed grids, tables read from device memory)

@@ -82,8 +82,10 @@ int vt_encode_srgb8(const float4* rad, uint32_t* bgra, uint32_t n, hipStream_t s
 bool vt_batch_layout(const ReqGPU* reqs, uint32_t n, uint32_t npaths, BatchGPU& B);
 // acc_after (optional): the accumulation launch waits for this event (a previous sample chunk
 // of the same pixels accumulating on another stream).
+// blocks (optional): persistent workgroups per launch (0 = the library's default, 4 per CU)
 int vt_trace_batch(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, unsigned long long* stats, hipEvent_t* ev,
-                   uint8_t* kinds, uint32_t* nstages, hipStream_t s, hipEvent_t acc_after = nullptr);
+                   uint8_t* kinds, uint32_t* nstages, hipStream_t s, hipEvent_t acc_after = nullptr,
+                   uint32_t blocks = 0);
 // with_acc = false: the graph stops before the accumulation (vt_trace_accumulate launches it)
 int vt_trace_capture(uint32_t max_bounces, bool deep, const WaveBuffers& wb, hipStream_t s, hipGraphExec_t* out,
                      bool with_acc = true);
