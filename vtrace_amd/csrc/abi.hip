@@ -468,6 +468,7 @@ int render_frame(const FrameGPU& F) {
 std::mutex& vt_rt_lock() { return g_lock; }
 bool vt_rt_inited() { return g.inited; }
 int vt_rt_device() { return g.device; }
+uint32_t vt_rt_cus() { return g.cus; }
 hipStream_t vt_rt_stream() { return g.stream; }
 const VolumeDesc* vt_rt_vol_table() { return g.vol_table; }
 uint32_t vt_rt_nvol() { return (uint32_t)g.vols.size(); }

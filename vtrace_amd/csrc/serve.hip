@@ -115,7 +115,7 @@ struct Serve {
     void* sub_mem[kSubBatches] = {};
     WaveBuffers sub_wave[kSubBatches];
     hipStream_t sub[kSubBatches] = {};
-    hipGraphExec_t sub_graph[kSubBatches][2] = {};
+    hipGraphExec_t sub_graph[kSubBatches][2][kSubBatches] = {};  // [sub-batch][deep][sub-batches - 1]
     hipEvent_t acc_done[kSubBatches] = {}, start = nullptr;
     std::vector<ServeScene> scenes;
     std::deque<int64_t> queue;
@@ -150,8 +150,9 @@ void free_all() {
         if (gx) hipGraphExecDestroy(gx);
     for (uint32_t j = 0; j < kSubBatches; ++j) {
         if (sv.sub[j]) hipStreamSynchronize(sv.sub[j]);
-        for (auto& gx : sv.sub_graph[j])
-            if (gx) hipGraphExecDestroy(gx);
+        for (auto& per_deep : sv.sub_graph[j])
+            for (auto& gx : per_deep)
+                if (gx) hipGraphExecDestroy(gx);
         hipFree(sv.sub_mem[j]);
         if (sv.sub[j]) hipStreamDestroy(sv.sub[j]);
         if (sv.acc_done[j]) hipEventDestroy(sv.acc_done[j]);
@@ -464,21 +465,24 @@ int32_t vt_serve_step(int64_t* completed, uint32_t max_completed) {
         for (auto& ch : chunks) deep = deep || sv.scenes[sv.reqs[ch.first].q.scene].depth + 1u > 16u;
         hipEventRecord(f.t0, s);
         hipEventRecord(sv.start, s);
+        // persistent workgroups per launch: 4 per CU alone, 2 per CU with 3-4 sub-batches in flight
+        const uint32_t cus = vt_rt_cus();
+        const uint32_t blocks = (nsub > 1 && cus) ? cus * std::min(4u, std::max(2u, 8u / nsub)) : 0u;
         for (uint32_t k = 0; k < nsub && rc == 0; ++k) {
             hipStream_t sk = k == 0 ? s : sv.sub[k];
             const WaveBuffers& wk = k == 0 ? sv.wave : sv.sub_wave[k];
             if (k > 0) hipStreamWaitEvent(sk, sv.start, 0);
             hipEvent_t after = k == 0 ? nullptr : sv.acc_done[k - 1];
             if (c.use_graph) {
-                hipGraphExec_t& gx = k == 0 ? sv.graph[deep ? 1 : 0] : sv.sub_graph[k][deep ? 1 : 0];
-                if (!gx) rc = vt_trace_capture(c.max_bounces, deep, wk, sk, &gx, false);
+                hipGraphExec_t& gx = nsub == 1 ? sv.graph[deep ? 1 : 0] : sv.sub_graph[k][deep ? 1 : 0][nsub - 1];
+                if (!gx) rc = vt_trace_capture(c.max_bounces, deep, wk, sk, &gx, false, blocks);
                 if (rc == 0) rc = vt_trace_upload_tables(batch[k].data(), (uint32_t)batch[k].size(), wk, sk, nullptr);
                 if (rc == 0) rc = hipGraphLaunch(gx, sk) == hipSuccess ? 0 : -1;
                 if (rc == 0 && after) hipStreamWaitEvent(sk, after, 0);
-                if (rc == 0) rc = vt_trace_accumulate(wk, sk);
+                if (rc == 0) rc = vt_trace_accumulate(wk, sk, blocks);
             } else {
                 rc = vt_trace_batch(batch[k].data(), (uint32_t)batch[k].size(), wk, nullptr, nullptr, nullptr, nullptr,
-                                    sk, after);
+                                    sk, after, blocks);
             }
             hipEventRecord(sv.acc_done[k], sk);
         }

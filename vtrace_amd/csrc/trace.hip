@@ -1367,13 +1367,13 @@ int vt_trace_batch(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, unsign
 // Captures the launch sequence of a batch (fixed grids, tables read from device memory)
 // into a graph; the table upload stays outside the graph.
 int vt_trace_capture(uint32_t max_bounces, bool deep, const WaveBuffers& wb, hipStream_t s, hipGraphExec_t* out,
-                     bool with_acc) {
+                     bool with_acc, uint32_t blocks) {
     BatchGPU Bh{};
     Bh.max_bounces = max_bounces;
     hipGraph_t graph;
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return -1;
-    int rc = deep ? launch_batch<false, true>(Bh, wb, nullptr, nullptr, nullptr, nullptr, s, nullptr, with_acc)
-                  : launch_batch<false, false>(Bh, wb, nullptr, nullptr, nullptr, nullptr, s, nullptr, with_acc);
+    int rc = deep ? launch_batch<false, true>(Bh, wb, nullptr, nullptr, nullptr, nullptr, s, nullptr, with_acc, blocks)
+                  : launch_batch<false, false>(Bh, wb, nullptr, nullptr, nullptr, nullptr, s, nullptr, with_acc, blocks);
     if (hipStreamEndCapture(s, &graph) != hipSuccess || rc != 0) return -1;
     rc = hipGraphInstantiate(out, graph, nullptr, nullptr, 0) == hipSuccess ? 0 : -1;
     hipGraphDestroy(graph);
@@ -1381,10 +1381,10 @@ int vt_trace_capture(uint32_t max_bounces, bool deep, const WaveBuffers& wb, hip
 }
 
 // The accumulation launch of a batch whose tables are in wb (after a graph captured without it).
-int vt_trace_accumulate(const WaveBuffers& wb, hipStream_t s) {
+int vt_trace_accumulate(const WaveBuffers& wb, hipStream_t s, uint32_t blocks) {
     const ReqGPU* reqs = reinterpret_cast<const ReqGPU*>(wb.tables);
     const BatchGPU* B = reinterpret_cast<const BatchGPU*>(static_cast<const char*>(wb.tables) + sizeof(ReqGPU) * VT_MAX_BATCH);
-    hipLaunchKernelGGL(k_accumulate, dim3(g_persistent_blocks), dim3(256), 0, s, reqs, B, to_wf(wb));
+    hipLaunchKernelGGL(k_accumulate, dim3(blocks ? blocks : g_persistent_blocks), dim3(256), 0, s, reqs, B, to_wf(wb));
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -88,8 +88,8 @@ int vt_trace_batch(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, unsign
                    uint32_t blocks = 0);
 // with_acc = false: the graph stops before the accumulation (vt_trace_accumulate launches it)
 int vt_trace_capture(uint32_t max_bounces, bool deep, const WaveBuffers& wb, hipStream_t s, hipGraphExec_t* out,
-                     bool with_acc = true);
-int vt_trace_accumulate(const WaveBuffers& wb, hipStream_t s);
+                     bool with_acc = true, uint32_t blocks = 0);
+int vt_trace_accumulate(const WaveBuffers& wb, hipStream_t s, uint32_t blocks = 0);
 int vt_trace_upload_tables(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, hipStream_t s, BatchGPU* layout);
 void vt_trace_release();
 
@@ -100,6 +100,7 @@ bool vt_inverse4(const float* m, float* out);
 std::mutex& vt_rt_lock();
 bool vt_rt_inited();
 int vt_rt_device();
+uint32_t vt_rt_cus();   // compute units of the device (0 if unknown)
 hipStream_t vt_rt_stream();
 const VolumeDesc* vt_rt_vol_table();
 uint32_t vt_rt_nvol();
