@@ -101,9 +101,10 @@ int32_t vt_add_textures(const uint8_t* rgba8, uint32_t count, uint32_t width, ui
  * previous one to last_mouse_x/y, lib/entry.c:157-163); keys are written into the struct from
  * get_input_data_pointer directly, as the reference's key callback does (lib/entry.c:31-52). */
 void vt_set_cursor(double x, double y);
-/* Path-traced frames of 2+ samples without G-buffer run as min(chunks, spp) sample chunks on
- * as many streams (chunks <= 4, default 4; 0 or 1: one stream).  Results are identical: each
- * chunk's accumulation follows the previous one's, so the per-pixel sum stays in sample order. */
+/* Path-traced frames of 2+ samples run as min(chunks, spp) sample chunks on as many streams
+ * (chunks <= 4, default 4; 0 or 1: one stream).  Results are identical: each chunk's
+ * accumulation follows the previous one's, so the per-pixel sums (radiance and G-buffer) stay
+ * in sample order. */
 int32_t vt_set_frame_split(uint32_t chunks);
 /* Like end_update_instances for a list of the same length as the last committed one (moved
  * instances): the records are committed and the LBVH keeps its topology, its boxes refitted
@@ -263,6 +264,10 @@ int32_t vt_dn_denoise(const float* rad_rgba, const float* gbuf8, uint32_t h, uin
 /* Denoises the last frame of render_tick (mode 1/2 with the G-buffer enabled) in place on
  * the device and copies the result out. */
 int32_t vt_dn_denoise_frame(float* out_rgba, float* gpu_ms);
+/* Output blend: denoise calls return alpha * network output + (1 - alpha) * input radiance,
+ * alpha = alpha[k] for spp = 2^k (linear in log2 spp between entries, clamped to the ends);
+ * n = 0 removes the table (alpha 1).  Weights in [0, 1], n <= 32. */
+int32_t vt_dn_set_blend(const float* alpha, uint32_t n);
 void vt_dn_shutdown(void);
 
 #ifdef __cplusplus

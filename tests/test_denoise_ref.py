@@ -144,3 +144,22 @@ def test_loss_gradient_includes_remodulation():
         y2[idx] -= eps
         fd = (U.UNetRef.loss(y1, t, x)[0] - U.UNetRef.loss(y2, t, x)[0]) / (2 * eps)
         assert abs(fd - gy[idx]) < 1e-6 * max(1.0, abs(fd))
+
+
+def test_blend_weight_is_the_relmse_minimiser():
+    """denoise.blend_weight: the closed form equals the minimum of the blended relMSE over a fine
+    grid of weights (clamped to [0, 1]); exact estimates get weight 1."""
+    from vtrace_amd import denoise as DN
+    rng = np.random.default_rng(3)
+    for trial in range(20):
+        ref = rng.uniform(0.0, 2.0, size=(24, 32, 4))
+        noisy = ref + rng.normal(0.0, 0.3, size=ref.shape)
+        den = ref + rng.normal(0.0, rng.uniform(0.05, 0.6), size=ref.shape) + rng.uniform(-0.2, 0.2)
+        a = DN.blend_weight(noisy, den, ref)
+        grid = np.linspace(0.0, 1.0, 2001)
+        errs = [DN.relmse(g * den + (1.0 - g) * noisy, ref) for g in grid]
+        best = grid[int(np.argmin(errs))]
+        assert abs(a - best) <= 1e-3 + 1e-12, (trial, a, best)
+        assert DN.relmse(a * den + (1 - a) * noisy, ref) <= min(errs) + 1e-12
+    ref = rng.uniform(0.0, 1.0, size=(8, 8, 4))
+    assert DN.blend_weight(ref + 0.1, ref, ref) == 1.0

@@ -130,6 +130,38 @@ def test_denoise_path_matches_reference_forward(dn):
     assert np.all(out[..., 3] == 1.0)
 
 
+def test_output_blend(dn):
+    """vt_dn_set_blend: weight 1 is the network output bit for bit, 0 the input radiance, 0.25
+    the FP32 blend of the two; the table is linear in log2(spp) and clamped to its ends."""
+    r, d = dn
+    d.set_params(_params(d, 9))
+    rng = np.random.default_rng(17)
+    h, w = 32, 48
+    rad = np.abs(rng.normal(0.5, 0.5, size=(h, w, 4))).astype(np.float32)
+    gb = rng.uniform(0.0, 1.0, size=(h, w, 8)).astype(np.float32)
+    d.set_blend([])
+    net, _ = d.denoise(rad, gb, 4)
+    _say("case: output blend weights 1, 0, 0.25 and a two-entry table")
+    d.set_blend([1.0])
+    assert np.array_equal(d.denoise(rad, gb, 4)[0].view(np.uint32), net.view(np.uint32))
+    d.set_blend([0.0])
+    out0 = d.denoise(rad, gb, 4)[0]
+    assert np.array_equal(out0[..., :3].view(np.uint32), rad[..., :3].view(np.uint32))
+    a = np.float32(0.25)
+    want = a * net[..., :3] + (np.float32(1.0) - a) * rad[..., :3]
+    d.set_blend([0.25])
+    assert np.array_equal(d.denoise(rad, gb, 4)[0][..., :3].view(np.uint32), want.view(np.uint32))
+    d.set_blend([0.2, 0.6])   # spp 1 -> 0.2, spp 2 -> 0.6, spp 4 -> clamped to 0.6
+    for spp, alpha in ((1, 0.2), (2, 0.6), (4, 0.6)):
+        d.set_blend([])
+        n_spp, _ = d.denoise(rad, gb, spp)
+        d.set_blend([0.2, 0.6])
+        got = d.denoise(rad, gb, spp)[0][..., :3]
+        al = np.float32(alpha)
+        assert np.array_equal(got.view(np.uint32), (al * n_spp[..., :3] + (np.float32(1.0) - al) * rad[..., :3]).view(np.uint32))
+    d.set_blend([])
+
+
 def _view(seed, size=128):
     return scenes.single_volume_scene(64, 0.30, seed, size, size, mode=2, spp=1, max_bounces=3, jitter=1,
                                       viewport_square=0, name=f"dn{seed}")

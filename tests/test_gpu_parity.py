@@ -226,20 +226,25 @@ def test_bgra8_encode_matches_oracle():
     assert len(np.unique(bgra)) > 50
 
 
-@pytest.mark.parametrize("spp,chunks", [(2, 2), (3, 3), (4, 4), (5, 4), (4, 2)])
-def test_split_frames_equal_one_stream(spp, chunks):
+@pytest.mark.parametrize("spp,chunks,gbuffer", [(2, 2, False), (3, 3, False), (4, 4, False), (5, 4, False),
+                                                (4, 2, False), (4, 4, True), (5, 3, True)])
+def test_split_frames_equal_one_stream(spp, chunks, gbuffer):
     """Sample chunks on several streams give the one-stream frame bit for bit (hits, radiance,
-    counters): every chunk accumulates after the previous one, in sample order."""
+    G-buffer, counters): every chunk accumulates after the previous one, in sample order."""
     from vtrace_amd import renderer as R
     sc = scenes.multi_volume_scene(count=5, n=64, world=512, seed=21, width=160, height=96, spp=spp, max_bounces=3)
-    _say(f"case: split spp={spp} chunks={chunks}")
+    _say(f"case: split spp={spp} chunks={chunks} gbuffer={gbuffer}")
     r = R.reset_renderer()
     R.load_scene(r, sc, stats=True)
+    r.enable_gbuffer(gbuffer)
     r.lib.vt_set_frame_split(1)
     r.render(sc.P, sc.C)
     h1, rad1, st1 = r.hits().copy(), r.radiance().copy(), r.stats()
+    gb1 = r.gbuffer().copy() if gbuffer else None
     r.lib.vt_set_frame_split(chunks)
     r.render(sc.P, sc.C)
     assert np.array_equal(r.hits(), h1)
     assert np.array_equal(r.radiance().view(np.uint32), rad1.view(np.uint32))
     assert r.stats() == st1
+    if gbuffer:
+        assert np.array_equal(r.gbuffer().view(np.uint32), gb1.view(np.uint32))

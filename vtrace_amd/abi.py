@@ -20,7 +20,7 @@ EXTENSION_EXPORTS = ("vt_set_params", "vt_get_params", "vt_add_volume_indexed", 
                      "vt_serve_status", "vt_serve_shutdown", "vt_enable_gbuffer", "vt_read_gbuffer",
                      "vt_dn_init", "vt_dn_layers", "vt_dn_param_count", "vt_dn_set_params", "vt_dn_get_params",
                      "vt_dn_grad", "vt_dn_features", "vt_dn_add_frame", "vt_dn_clear_frames", "vt_dn_train",
-                     "vt_dn_denoise", "vt_dn_denoise_frame", "vt_dn_shutdown")
+                     "vt_dn_denoise", "vt_dn_denoise_frame", "vt_dn_set_blend", "vt_dn_shutdown")
 
 
 class RenderTickInfo(C.Structure):

@@ -75,6 +75,10 @@ struct State {
     WaveBuffers xwave[kMaxChunks];
     float4* acc = nullptr;      // per pixel running sum between the sample chunks
     size_t acc_cap = 0;
+    float4* gacc = nullptr;     // with the G-buffer: per pixel running G-buffer sum (2 float4)
+    size_t gacc_cap = 0;
+    float4* xgsamp[kMaxChunks] = {};  // with the G-buffer: per-sample state of chunks 1..n-1
+    size_t xgsamp_cap[kMaxChunks] = {};
     unsigned long long* stats = nullptr;  // VT_MAX_STAGES x VT_STAT_COUNT
     uint8_t stage_kinds[VT_MAX_STAGES] = {};
     uint32_t nstages = 0;
@@ -174,6 +178,20 @@ int ensure_split_buffers(uint32_t spp, uint32_t n) {
             g.xwave_cap[k] = paths;
         }
         g.xwave[k].gsamp = nullptr;
+        if (g.gbuf_on) {
+            if (paths > g.xgsamp_cap[k] || !g.xgsamp[k]) {
+                hipFree(g.xgsamp[k]);
+                g.xgsamp[k] = nullptr;
+                g.xgsamp_cap[k] = 0;
+                hipError_t e = hipMalloc(&g.xgsamp[k], paths * 2 * sizeof(float4));
+                if (e != hipSuccess) {
+                    set_err("G-buffer sample state allocation", e);
+                    return -1;
+                }
+                g.xgsamp_cap[k] = paths;
+            }
+            g.xwave[k].gsamp = g.xgsamp[k];
+        }
     }
     const size_t np = (size_t)g.params.width * g.params.height;
     if (np > g.acc_cap || !g.acc) {
@@ -186,6 +204,17 @@ int ensure_split_buffers(uint32_t spp, uint32_t n) {
             return -1;
         }
         g.acc_cap = np;
+    }
+    if (g.gbuf_on && (np > g.gacc_cap || !g.gacc)) {
+        hipFree(g.gacc);
+        g.gacc = nullptr;
+        g.gacc_cap = 0;
+        hipError_t e = hipMalloc(&g.gacc, np * 2 * sizeof(float4));
+        if (e != hipSuccess) {
+            set_err("G-buffer accumulation allocation", e);
+            return -1;
+        }
+        g.gacc_cap = np;
     }
     return 0;
 }
@@ -419,7 +448,8 @@ int render_split(const FrameGPU& F, const SceneGPU& S, uint32_t n) {
     R.hits = g.hits;
     R.acc = g.acc;
     R.rad = reinterpret_cast<float4*>(g.rad);
-    R.gbuf = nullptr;
+    R.gbuf = g.gbuf_on ? g.gbuf : nullptr;
+    R.gacc = g.gbuf_on ? g.gacc : nullptr;
     R.tile_w = 8;
     // persistent workgroups per launch: 4 per CU fill the GPU alone; with 3-4 chunks in flight,
     // 2 per CU let two chunks' launches share it (measured on the 1080p 4-spp frame)
@@ -447,7 +477,7 @@ int render_frame(const FrameGPU& F) {
     hipEventRecord(g.ev0, g.stream);
     int rc;
     const uint32_t n = std::min(g.split, std::min(F.spp, kMaxChunks));
-    if (n >= 2 && F.mode >= 1 && !g.gbuf_on)
+    if (n >= 2 && F.mode >= 1)
         rc = render_split(F, scene_gpu(), n);
     else
         rc = vt_trace_launch(F, scene_gpu(), g.hits, reinterpret_cast<float4*>(g.rad), g.gbuf_on ? g.gbuf : nullptr,
@@ -648,6 +678,8 @@ void cleanup(void) {
         if (g.xstream[k]) hipStreamDestroy(g.xstream[k]);
     }
     hipFree(g.acc);
+    hipFree(g.gacc);
+    for (uint32_t k = 0; k < kMaxChunks; ++k) hipFree(g.xgsamp[k]);
     hipEventDestroy(g.ev0); hipEventDestroy(g.ev1); hipEventDestroy(g.inst_copied);
     hipStreamDestroy(g.stream);
     vt_trace_set_tables(nullptr, nullptr);

@@ -482,9 +482,10 @@ __global__ void k_prep_weights(const float* __restrict__ params, const LayerDev*
 }
 
 // network output (hp x wp x 3 log-irradiance corrections, FP32) -> h x w RGBA32F radiance
-// (irradiance remodulated with the albedo of the features x, clamped at 0)
+// (irradiance remodulated with the albedo of the features x, clamped at 0); alpha < 1 blends
+// with the input radiance: alpha * denoised + (1 - alpha) * input
 __global__ void k_output(const float* __restrict__ y, const uint16_t* __restrict__ x, uint32_t h, uint32_t w,
-                         uint32_t wp, float4* __restrict__ out) {
+                         uint32_t wp, const float4* __restrict__ rad, float alpha, float4* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= h * w) return;
     const uint32_t xx = i % w, yy = i / w;
@@ -495,6 +496,13 @@ __global__ void k_output(const float* __restrict__ y, const uint16_t* __restrict
     for (int c = 0; c < 3; ++c) {
         float dE;
         r[c] = fmaxf((bf2f(f[3 + c]) + kAlbedoEps) * irradiance(bf2f(f[c]), v[c], &dE), 0.0f);
+    }
+    if (alpha < 1.0f) {
+        const float4 in = rad[i];
+        const float b = 1.0f - alpha;
+        r[0] = alpha * r[0] + b * in.x;
+        r[1] = alpha * r[1] + b * in.y;
+        r[2] = alpha * r[2] + b * in.z;
     }
     out[i] = make_float4(r[0], r[1], r[2], 1.0f);
 }
@@ -550,9 +558,24 @@ struct Net {
     void* io = nullptr;
     size_t io_cap = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    // output blend weights for spp = 2^k, k < nblend (none: the network output alone)
+    float blend[32] = {};
+    uint32_t nblend = 0;
 };
 
 Net net;
+
+// blend weight for spp samples: linear in log2(spp) between the table's entries, clamped to
+// its ends; 1 without a table
+float blend_alpha(uint32_t spp) {
+    if (net.nblend == 0) return 1.0f;
+    const float t = log2f((float)(spp ? spp : 1u));
+    if (t <= 0.0f) return net.blend[0];
+    if (t >= (float)(net.nblend - 1)) return net.blend[net.nblend - 1];
+    const uint32_t k = (uint32_t)t;
+    const float f = t - (float)k;
+    return net.blend[k] + f * (net.blend[k + 1] - net.blend[k]);
+}
 
 void set_layer(int i, uint32_t cin, uint32_t cout, bool relu) {
     net.L[i].cin = cin;
@@ -895,7 +918,8 @@ int denoise_device(const float4* rad, const float4* gbuf, uint32_t h, uint32_t w
     hipLaunchKernelGGL(k_features, dim3(blocks((size_t)hp * wp)), dim3(256), 0, s, rad, gbuf, h, w, hp, wp,
                        spp_feature(spp), net.x0.p);
     if (forward(1, hp, wp, s) != 0) return -1;
-    hipLaunchKernelGGL(k_output, dim3(blocks((size_t)h * w)), dim3(256), 0, s, net.y, net.x0.p, h, w, wp, out_dev);
+    hipLaunchKernelGGL(k_output, dim3(blocks((size_t)h * w)), dim3(256), 0, s, net.y, net.x0.p, h, w, wp, rad,
+                       blend_alpha(spp), out_dev);
     hipEventRecord(net.e1, s);
     if (hipStreamSynchronize(s) != hipSuccess) return -1;
     if (ms) hipEventElapsedTime(ms, net.e0, net.e1);
@@ -1243,6 +1267,16 @@ int32_t vt_dn_denoise_frame(float* out_rgba, float* gpu_ms) {
     }
     hipMemcpyAsync(out_rgba, out, npx * 16, hipMemcpyDeviceToHost, s);
     return hipStreamSynchronize(s) == hipSuccess ? 0 : -1;
+}
+
+int32_t vt_dn_set_blend(const float* alpha, uint32_t n) {
+    std::lock_guard<std::mutex> lk(vt_rt_lock());
+    if (!net.on || n > 32 || (n && !alpha)) return -1;
+    for (uint32_t k = 0; k < n; ++k)
+        if (!(alpha[k] >= 0.0f && alpha[k] <= 1.0f)) return -1;
+    for (uint32_t k = 0; k < n; ++k) net.blend[k] = alpha[k];
+    net.nblend = n;
+    return 0;
 }
 
 void vt_dn_shutdown(void) {

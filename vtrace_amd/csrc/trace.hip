@@ -1074,22 +1074,30 @@ __device__ void accumulate_pixel(const ReqGPU* __restrict__ reqs, const BatchGPU
         acc[1] += L.y + Ls.y;
         acc[2] += L.z + Ls.z;
     }
-    if (Rq.final_chunk) {
-        const float fs = (float)(Rq.F.spp ? Rq.F.spp : 1u);
-        Rq.rad[p] = make_float4(acc[0] / fs, acc[1] / fs, acc[2] / fs, 1.0f);
-        if (Rq.gbuf && wf.gsamp && Rq.s0 == 0) {  // G-buffer: mean over the samples, in sample order
-            float g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            for (uint32_t s = 0; s < Rq.s1; ++s) {
-                const size_t q = 2 * (size_t)(wb + tm.index(px, py, s, 0u));
-                const float4 a = wf.gsamp[q], b = wf.gsamp[q + 1];
-                g[0] += a.x; g[1] += a.y; g[2] += a.z; g[3] += a.w;
-                g[4] += b.x; g[5] += b.y; g[6] += b.z; g[7] += b.w;
-            }
+    const float fs = (float)(Rq.F.spp ? Rq.F.spp : 1u);
+    if (Rq.final_chunk) Rq.rad[p] = make_float4(acc[0] / fs, acc[1] / fs, acc[2] / fs, 1.0f);
+    else Rq.acc[p] = make_float4(acc[0], acc[1], acc[2], 0.0f);
+    // G-buffer: mean over the samples, summed in sample order (a running sum across chunks)
+    if (Rq.gbuf && wf.gsamp && (Rq.s0 == 0 || Rq.gacc) && (Rq.final_chunk || Rq.gacc)) {
+        float g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (Rq.s0 > 0) {
+            const float4 a = Rq.gacc[2 * p], b = Rq.gacc[2 * p + 1];
+            g[0] = a.x; g[1] = a.y; g[2] = a.z; g[3] = a.w;
+            g[4] = b.x; g[5] = b.y; g[6] = b.z; g[7] = b.w;
+        }
+        for (uint32_t s = Rq.s0; s < Rq.s1; ++s) {
+            const size_t q = 2 * (size_t)(wb + tm.index(px, py, s, Rq.s0));
+            const float4 a = wf.gsamp[q], b = wf.gsamp[q + 1];
+            g[0] += a.x; g[1] += a.y; g[2] += a.z; g[3] += a.w;
+            g[4] += b.x; g[5] += b.y; g[6] += b.z; g[7] += b.w;
+        }
+        if (Rq.final_chunk) {
             Rq.gbuf[2 * p] = make_float4(g[0] / fs, g[1] / fs, g[2] / fs, g[3] / fs);
             Rq.gbuf[2 * p + 1] = make_float4(g[4] / fs, g[5] / fs, g[6] / fs, g[7] / fs);
+        } else {
+            Rq.gacc[2 * p] = make_float4(g[0], g[1], g[2], g[3]);
+            Rq.gacc[2 * p + 1] = make_float4(g[4], g[5], g[6], g[7]);
         }
-    } else {
-        Rq.acc[p] = make_float4(acc[0], acc[1], acc[2], 0.0f);
     }
     if (Rq.s0 == 0 && wf.shadowed[wb + tm.index(px, py, 0u, 0u)] && (Rq.hits[p].flags & 1u)) Rq.hits[p].flags |= 2u;
 }

@@ -100,7 +100,8 @@ struct ReqGPU {
     float4* acc;            // per pixel running radiance sum over samples (FP32, in sample order)
     float4* rad;            // per pixel final radiance, written by the last chunk (acc / spp)
     float4* gbuf;           // optional, 2 per pixel averaged over samples: (albedo rgb, distance),
-                            // (normal, hit fraction); needs the wave buffers' gsamp and one chunk
+                            // (normal, hit fraction); needs the wave buffers' gsamp
+    float4* gacc;           // with gbuf over several chunks: per pixel running G-buffer sum (2 float4)
     uint32_t s0, s1;        // samples [s0, s1) are traced in this launch
     uint32_t final_chunk;   // 1: s1 == spp
     uint32_t tile_w;        // pixel tile of 64 work items: tile_w x (64 / tile_w)

@@ -53,6 +53,8 @@ def declare(lib):
     lib.vt_dn_denoise.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, f]
     lib.vt_dn_denoise_frame.restype = C.c_int32
     lib.vt_dn_denoise_frame.argtypes = [C.c_void_p, f]
+    lib.vt_dn_set_blend.restype = C.c_int32
+    lib.vt_dn_set_blend.argtypes = [C.c_void_p, C.c_uint32]
     lib.vt_dn_shutdown.restype = None
     lib.vt_dn_shutdown.argtypes = []
     lib._vt_dn_declared = True
@@ -63,6 +65,17 @@ def relmse(x: np.ndarray, ref: np.ndarray) -> float:
     x = np.asarray(x, dtype=np.float64)[..., :3]
     r = np.asarray(ref, dtype=np.float64)[..., :3]
     return float(np.mean((x - r) ** 2 / (r * r + 0.01)))
+
+
+def blend_weight(noisy: np.ndarray, den: np.ndarray, ref: np.ndarray) -> float:
+    """alpha in [0, 1] minimising mean((alpha d + (1 - alpha) n - r)^2 / (r^2 + 0.01)) (the relMSE
+    of the blend): the zero of its derivative, sum w (n - r)(n - d) / sum w (n - d)^2."""
+    n, d, r = (np.asarray(v, dtype=np.float64)[..., :3] for v in (noisy, den, ref))
+    w = 1.0 / (r * r + 0.01)
+    den_ = float(np.sum(w * (n - d) ** 2))
+    if den_ <= 0.0:
+        return 1.0
+    return float(np.clip(np.sum(w * (n - r) * (n - d)) / den_, 0.0, 1.0))
 
 
 class Denoiser:
@@ -162,6 +175,30 @@ class Denoiser:
             raise RendererError(f"vt_dn_denoise failed: {self.r.error()}")
         return out, float(ms.value)
 
+    def set_blend(self, alphas) -> None:
+        """Output blend weights for spp = 1, 2, 4, ... (vt_dn_set_blend); [] removes them."""
+        a = np.ascontiguousarray(np.asarray(alphas, dtype=np.float32).reshape(-1))
+        if self.lib.vt_dn_set_blend(a.ctypes.data if a.size else None, a.size) != 0:
+            raise RendererError(f"vt_dn_set_blend failed: {self.r.error()}")
+
+    def calibrate_blend(self, r, scene: "scenes.Scene", max_log2_spp: int = 8, target_spp: int = 1024,
+                        seed: int = 31) -> list:
+        """Per spp = 2^k (k <= max_log2_spp), the blend weight that minimises the relMSE of
+        alpha * network output + (1 - alpha) * input on `scene` against a target_spp render of it
+        (a scene kept out of training and evaluation); sets and returns the table."""
+        from .renderer import load_scene
+        load_scene(r, scene)
+        target, _, _ = render_spp(r, scene, target_spp, seed=seed * 1000 + 7)
+        self.set_blend([])
+        alphas = []
+        for k in range(max_log2_spp + 1):
+            spp = 1 << k
+            rad, gb, _ = render_spp(r, scene, spp, seed=seed * 1000 + 100 + k, gbuffer=True)
+            out, _ = self.denoise(rad, gb, spp)
+            alphas.append(blend_weight(rad, out, target))
+        self.set_blend(alphas)
+        return alphas
+
     def denoise_frame(self):
         """Denoises the renderer's last frame (G-buffer enabled) without a host round trip."""
         w, h = self.r.window_width, self.r.window_height
@@ -219,10 +256,11 @@ def crossing_at(curve: list, threshold: float):
 def benchmark(r, eval_scene: "scenes.Scene", train_seeds=(101, 102, 103, 104, 105, 106),
               train_spp=(1, 4, 16, 64, 256), target_spp: int = 1024, judge_spp: int = 4096, iters: int = 8000,
               batch: int = 8, crop: int = 128,
-              lr: float = 2e-3, threshold: float = 1e-3, log=None) -> dict:
-    """Trains on C2-style scenes (other seeds than the evaluation scene), then measures the GPU
-    time each method needs to reach relMSE <= threshold against a judge_spp render of
-    eval_scene: the plain path (spp doubling from 1) and render + denoise (spp doubling
+              lr: float = 2e-3, threshold: float = 1e-3, calib_seed: int = 107, log=None) -> dict:
+    """Trains on C2-style scenes (other seeds than the evaluation scene), calibrates the output
+    blend on one more seed, then measures the GPU time each method needs to reach relMSE <=
+    threshold against a judge_spp render of eval_scene: the plain path (spp doubling from 1,
+    frames without G-buffer) and render + denoise, network alone and blended (spp doubling
     from 1).  Times are device times (render kernels + denoiser kernels)."""
     say = log or (lambda *a: None)
     p = eval_scene.params
@@ -244,21 +282,34 @@ def benchmark(r, eval_scene: "scenes.Scene", train_seeds=(101, 102, 103, 104, 10
     t_train = time.perf_counter() - t0
     d.clear_frames()
     say(f"denoiser: {iters} iterations in {t_train:.1f} s, loss {losses[:50].mean():.4f} -> {losses[-50:].mean():.4f}")
+    # output blend weights, calibrated on a scene of its own (neither trained on nor evaluated)
+    calib = scenes.multi_volume_scene(16, 256, 2048, calib_seed, p.width, p.height, 1, p.max_bounces)
+    alphas = d.calibrate_blend(r, calib, max_log2_spp=8, target_spp=target_spp, seed=calib_seed)
+    del calib.volumes[:]
+    say("denoiser: blend weights " + ", ".join(f"{1 << k}:{a:.2f}" for k, a in enumerate(alphas)))
     from .renderer import load_scene
     load_scene(r, eval_scene)
     judge, _, judge_ms = render_spp(r, eval_scene, judge_spp, seed=777)
-    plain, den = [], []
+    plain, den, blended = [], [], []
     spp = 1
     while spp <= target_spp:
-        rad, gb, ms = render_spp(r, eval_scene, spp, seed=4242, gbuffer=True)
-        e = relmse(rad, judge)
-        plain.append({"spp": spp, "ms": round(ms, 3), "relmse": e})
+        # plain: the frame alone (no G-buffer); the denoiser's input: the same samples with it
+        rad_plain, _, ms_plain = render_spp(r, eval_scene, spp, seed=4242)
+        e = relmse(rad_plain, judge)
+        plain.append({"spp": spp, "ms": round(ms_plain, 3), "relmse": e})
         if spp <= 256:
+            rad, gb, ms = render_spp(r, eval_scene, spp, seed=4242, gbuffer=True)
+            d.set_blend([])
             out, dms = d.denoise(rad, gb, spp)
             den.append({"spp": spp, "ms": round(ms + dms, 3), "render_ms": round(ms, 3), "denoise_ms": round(dms, 4),
                         "relmse": relmse(out, judge)})
-        say(f"denoiser: spp {spp}: plain {e:.3e}" + (f", denoised {den[-1]['relmse']:.3e}" if spp <= 256 else ""))
-        if e <= threshold and den and den[-1]["relmse"] <= threshold:
+            d.set_blend(alphas)
+            outb, bms = d.denoise(rad, gb, spp)
+            blended.append({"spp": spp, "ms": round(ms + bms, 3), "render_ms": round(ms, 3),
+                            "denoise_ms": round(bms, 4), "relmse": relmse(outb, judge)})
+        say(f"denoiser: spp {spp}: plain {e:.3e}" +
+            (f", denoised {den[-1]['relmse']:.3e}, blended {blended[-1]['relmse']:.3e}" if spp <= 256 else ""))
+        if e <= threshold and den and den[-1]["relmse"] <= threshold and blended[-1]["relmse"] <= threshold:
             break
         spp *= 2
 
@@ -287,15 +338,18 @@ def benchmark(r, eval_scene: "scenes.Scene", train_seeds=(101, 102, 103, 104, 10
                                                    "data_seconds": round(t_data, 1),
                                                    "loss_first50": float(losses[:50].mean()),
                                                    "loss_last200": float(losses[-200:].mean())},
-           "plain_curve": plain, "denoised_curve": den}
-    for name, curve in (("plain", plain), ("denoised", den)):
+           "blend_weights": [round(a, 4) for a in alphas], "blend_calibration_seed": calib_seed,
+           "plain_curve": plain, "denoised_curve": den, "blended_curve": blended}
+    for name, curve in (("plain", plain), ("denoised", den), ("blended", blended)):
         c = crossing(curve)
         out[f"{name}_time_to_threshold_ms"] = round(c[0], 3) if c else None
         out[f"{name}_spp_at_threshold"] = round(c[1], 1) if c else None
         out[f"{name}_extrapolated"] = bool(c[2]) if c else None
     out["denoised_best_relmse"] = min(c["relmse"] for c in den) if den else None
     if out["plain_time_to_threshold_ms"] and out["denoised_time_to_threshold_ms"]:
-        out["speedup"] = round(out["plain_time_to_threshold_ms"] / out["denoised_time_to_threshold_ms"], 3)
+        out["speedup_network_alone"] = round(out["plain_time_to_threshold_ms"] / out["denoised_time_to_threshold_ms"], 3)
+    if out["plain_time_to_threshold_ms"] and out["blended_time_to_threshold_ms"]:
+        out["speedup"] = round(out["plain_time_to_threshold_ms"] / out["blended_time_to_threshold_ms"], 3)
     # equal-quality speedups at low spp: plain time to reach the denoised error of each input spp
     eq = []
     for c in den:
