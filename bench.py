@@ -127,7 +127,7 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--serving-requests", type=int, default=2000,
                     help="requests in the serving stream (0 skips the serving measurement)")
-    ap.add_argument("--denoiser-iters", type=int, default=8000,
+    ap.add_argument("--denoiser-iters", type=int, default=24000,
                     help="training iterations of the denoiser (0 skips the denoiser measurement)")
     ap.add_argument("--streaming", type=int, default=1, help="1: streamed terrain measurement, 0: skip it")
     ap.add_argument("--single-stream", action="store_true",

@@ -254,7 +254,7 @@ def crossing_at(curve: list, threshold: float):
 
 # ---- benchmark ------------------------------------------------------------------------------
 def benchmark(r, eval_scene: "scenes.Scene", train_seeds=(101, 102, 103, 104, 105, 106),
-              train_spp=(1, 4, 16, 64, 256), target_spp: int = 1024, judge_spp: int = 4096, iters: int = 8000,
+              train_spp=(1, 4, 16, 64, 256), target_spp: int = 1024, judge_spp: int = 4096, iters: int = 24000,
               batch: int = 8, crop: int = 128,
               lr: float = 2e-3, threshold: float = 1e-3, calib_seed: int = 107, log=None) -> dict:
     """Trains on C2-style scenes (other seeds than the evaluation scene), calibrates the output
