@@ -613,10 +613,47 @@ __device__ __forceinline__ bool better(const HitG& c, const HitG& b) {
 }
 __device__ __forceinline__ float prune_bound(float t) { return t + fabsf(t) * 0.0009765625f + 0.0009765625f; }
 
-// Closest (any_hit: any) hit over all instances except `skip`, through the LBVH.
+// LBVH nodes a workgroup stages in LDS when its launch traces one scene of at most 64
+// instances (a node fetch is then an LDS read instead of a dependent global load).
+constexpr uint32_t kLdsNodes = 127;
+
+// A workgroup's LDS copy of the tree (explicit LDS address space, so node reads are ds_reads).
+struct TreeLds {
+    const __attribute__((address_space(3))) BvhNode* nodes;
+    bool on;
+};
+
+__device__ __forceinline__ BvhNode lds_node(const TreeLds& t, uint32_t i) {
+    typedef const __attribute__((address_space(3))) uint32_t LdsU32;
+    LdsU32* p = (LdsU32*)(t.nodes + i);
+    BvhNode n;
+    n.lo[0] = __uint_as_float(p[0]); n.lo[1] = __uint_as_float(p[1]); n.lo[2] = __uint_as_float(p[2]);
+    n.hi[0] = __uint_as_float(p[3]); n.hi[1] = __uint_as_float(p[4]); n.hi[2] = __uint_as_float(p[5]);
+    n.left = p[6];
+    n.right = p[7];
+    return n;
+}
+
+// Stages the batch's tree into `tree` when it has one request whose tree fits (on = false:
+// traversal reads the nodes from global memory).  Block-uniform.
+__device__ __forceinline__ TreeLds stage_tree(const ReqGPU* reqs, const BatchGPU* B, BvhNode* tree) {
+    TreeLds t{(const __attribute__((address_space(3))) BvhNode*)(tree), false};
+    if (B->nreq != 1) return t;
+    const uint32_t ninst = reqs[0].S.ninst;
+    const uint32_t nn = ninst ? 2u * ninst - 1u : 0u;
+    if (nn == 0 || nn > kLdsNodes) return t;
+    const BvhNode* src = reqs[0].S.nodes;
+    for (uint32_t i = threadIdx.x; i < nn; i += blockDim.x) tree[i] = gld(src + i);
+    __syncthreads();
+    t.on = true;
+    return t;
+}
+
+// Closest (any_hit: any) hit over all instances except `skip`, through the LBVH (nodes from
+// `tree` in LDS when given).
 template <bool PRIMARY, bool ST, bool DEEP>
 __device__ HitG query(const SceneGPU& S, const FrameGPU& F, V3 O, V3 D, uint32_t skip, bool any_hit, Ctr<ST>& ctr,
-                      const Lds& stk_lds, const Lds& mslot) {
+                      const Lds& stk_lds, const Lds& mslot, TreeLds tree) {
     HitG best;
     best.hit = false;
     best.inst = VT_EMPTY;
@@ -629,7 +666,7 @@ __device__ HitG query(const SceneGPU& S, const FrameGPU& F, V3 O, V3 D, uint32_t
     stk.push(0u);
     while (stk.sp > 0) {
         const uint32_t ni = stk.pop();
-        const BvhNode N = gld(S.nodes + ni);
+        const BvhNode N = tree.on ? lds_node(tree, ni) : gld(S.nodes + ni);
         ctr.node();
         float tn, tf;
         if (!slab_node(O, D, inv_d, N, &tn, &tf) || tf < 0.0f) continue;
@@ -695,7 +732,7 @@ __device__ void surface_of(const SceneGPU& S, const HitG& h, SurfaceRec& sf) {
 
 template <bool ST, bool DEEP>
 __device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const SurfaceRec& sf, V3 w, bool any_hit,
-                          Ctr<ST>& ctr, const Lds& stk_lds, const Lds& mslot) {
+                          Ctr<ST>& ctr, const Lds& stk_lds, const Lds& mslot, TreeLds tree) {
     ctr.ray();
     HitG h;
     h.hit = false;
@@ -725,7 +762,7 @@ __device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const SurfaceRec
         ctr.hit();
         return h;
     }
-    h = query<false, ST, DEEP>(S, F, V3{sf.X[0], sf.X[1], sf.X[2]}, w, origin, any_hit, ctr, stk_lds, mslot);
+    h = query<false, ST, DEEP>(S, F, V3{sf.X[0], sf.X[1], sf.X[2]}, w, origin, any_hit, ctr, stk_lds, mslot, tree);
     if (h.hit) ctr.hit();
     return h;
 }
@@ -896,7 +933,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
                                                     const float* __restrict__ srgb_dec, WF wf,
                                                     unsigned long long* stats) {
     __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
+    __shared__ BvhNode tree_lds[kLdsNodes];
     const Lds stk{&smem[threadIdx.x]}, mslot{&smem[kStackLds * kBlock + threadIdx.x]};
+    const TreeLds tree = stage_tree(reqs, B, tree_lds);
     Ctr<ST> ctr;
     const uint32_t nreq = B->nreq;
     const uint32_t nwork = B->work_base[nreq];
@@ -920,7 +959,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         }
         const V3 D = camera_ray(F, (float)px + jx, (float)py + jy);
         ctr.ray();
-        HitG h = query<true, ST, DEEP>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, VT_EMPTY, false, ctr, stk, mslot);
+        HitG h = query<true, ST, DEEP>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, VT_EMPTY, false, ctr, stk, mslot, tree);
         float4 L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float beta[3] = {1.0f, 1.0f, 1.0f};
         uint8_t shadowed = 0;
@@ -991,7 +1030,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     uint32_t q, uint32_t depth, uint32_t in_idx, uint32_t out_idx, uint32_t fetch_idx, uint32_t bounce,
     unsigned long long* stats) {
     __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
+    __shared__ BvhNode tree_lds[kLdsNodes];
     const Lds stk{&smem[threadIdx.x]}, mslot{&smem[kStackLds * kBlock + threadIdx.x]};
+    const TreeLds tree = stage_tree(reqs, B, tree_lds);
     Ctr<ST> ctr;
     const uint32_t count = wf.counters[in_idx];
     const uint32_t nb = bounce ? count : 0u;
@@ -1008,7 +1049,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         const SceneGPU S = Rq.S;
         const FrameGPU& F = Rq.F;
         const V3 w = is_bounce ? V3{dw.x, dw.y, dw.z} : V3{F.sun[0], F.sun[1], F.sun[2]};
-        HitG h = secondary<ST, DEEP>(S, F, sf, w, !is_bounce, ctr, stk, mslot);
+        HitG h = secondary<ST, DEEP>(S, F, sf, w, !is_bounce, ctr, stk, mslot, tree);
         if (!is_bounce) {
             if (!h.hit) {
                 float4 L = wf.Ls[path];
@@ -1141,7 +1182,8 @@ __global__ __launch_bounds__(kBlock) void k_render0(const FrameGPU F, const Scen
             }
             const V3 D = camera_ray(F, (float)px + jx, (float)py + jy);
             ctr.ray();
-            HitG h = query<true, ST, DEEP>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, VT_EMPTY, false, ctr, stk, mslot);
+            HitG h = query<true, ST, DEEP>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, VT_EMPTY, false, ctr, stk, mslot,
+                                           TreeLds{});
             if (h.hit) {
                 ctr.hit();
                 uint8_t pal = 0;
