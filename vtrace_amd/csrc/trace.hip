@@ -208,8 +208,9 @@ struct DdaWalk {
 };
 
 // The walk over the bricked volume.  FAST: every delta is finite, so 0 * delta is 0, no side
-// distance can become NaN and GLSL's min(y, x) = (y < x ? y : x) in "side <= min(a, b)" is two
-// comparisons; otherwise the selects reproduce min's NaN behaviour exactly.  A brick is left
+// distance can become NaN and GLSL's min(y, x) = (y < x ? y : x) in "side <= min(a, b)" is
+// "side <= min3(sx, sy, sz)" (one min3 for the three axes); otherwise the selects reproduce
+// min's NaN behaviour exactly.  A brick is left
 // through one voxel per stepping axis (the first voxel outside it on that side), which is
 // what the loops test.
 template <bool ST, bool FAST>
@@ -230,9 +231,10 @@ __device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST
     {                                                                                   \
         bool mx, my, mz;                                                                \
         if (FAST) {                                                                     \
-            mx = sx <= sy && sx <= sz;                                                  \
-            my = sy <= sz && sy <= sx;                                                  \
-            mz = sz <= sx && sz <= sy;                                                  \
+            const float tm = fminf(fminf(sx, sy), sz);                                  \
+            mx = sx <= tm;                                                              \
+            my = sy <= tm;                                                              \
+            mz = sz <= tm;                                                              \
         } else {                                                                        \
             mx = sx <= gmin(sy, sz);                                                    \
             my = sy <= gmin(sz, sx);                                                    \
@@ -261,6 +263,41 @@ __device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST
         entry = gld(V.brick_map + (uint32_t)(bx8 >> 3) + bdx * ((uint32_t)(by8 >> 3) + bdy * (uint32_t)(bz8 >> 3)));
         ctr.brick(!vt_brick_empty(entry));
         if (vt_brick_empty(entry)) {
+            if (FAST) {
+                // Every direction component is nonzero, so every axis steps towards its exit
+                // voxel and each step moves at least one axis: the walk leaves the volume
+                // before max_steps.  rx/ry/rz count the steps each axis still needs to leave
+                // this brick; the loop stops before the step that leaves it, which is taken
+                // below with its bookkeeping.  "s <= min3(sx, sy, sz)" is the step test.
+                uint32_t rx = (uint32_t)(stx > 0 ? ox - vx : vx - ox);
+                uint32_t ry = (uint32_t)(sty > 0 ? oy - vy : vy - oy);
+                uint32_t rz = (uint32_t)(stz > 0 ? oz - vz : vz - oz);
+                float t;
+                for (;;) {
+                    t = fminf(fminf(sx, sy), sz);
+                    const bool mx = sx <= t, my = sy <= t, mz = sz <= t;
+                    rx -= mx ? 1u : 0u;
+                    ry -= my ? 1u : 0u;
+                    rz -= mz ? 1u : 0u;
+                    if (min(rx, min(ry, rz)) == 0u) break;
+                    sx += mx ? dx : zx;
+                    sy += my ? dy : zy;
+                    sz += mz ? dz : zz;
+                    ++steps;
+                }
+                lmx = sx <= t;
+                lmy = sy <= t;
+                lmz = sz <= t;
+                t_last = lmx ? sx : (lmy ? sy : sz);
+                sx += lmx ? dx : zx;
+                sy += lmy ? dy : zy;
+                sz += lmz ? dz : zz;
+                ++steps;
+                vx = ox - stx * (int)rx;
+                vy = oy - sty * (int)ry;
+                vz = oz - stz * (int)rz;
+                continue;
+            }
             do {
                 VT_DDA_STEP();
             } while (steps < max_steps && vx != ox && vy != oy && vz != oz);
@@ -331,8 +368,8 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
 // with selects so zero direction components never produce NaN) tests occupancy from the
 // LDS-staged mask until it hits or leaves the brick; the position where it leaves is
 // q + t * unit dir with the exit axes snapped to the voxel boundary.
-// FAST: every delta is finite, so no side distance is NaN and "side <= gmin(a, b)" is two
-// comparisons (otherwise the selects keep GLSL's min semantics for NaN sides).
+// FAST: every delta is finite, so no side distance is NaN and "side <= gmin(a, b)" is
+// "side <= min3(sx, sy, sz)" (otherwise the selects keep GLSL's min semantics for NaN sides).
 template <bool ST, bool FAST>
 __device__ __forceinline__ void dda2_walk(const VolumeDesc& V, const float p[3], const int v0[3], V3 m, Dda& r,
                                           Ctr<ST>& ctr, const Lds& mslot) {
@@ -422,9 +459,10 @@ __device__ __forceinline__ void dda2_walk(const VolumeDesc& V, const float p[3],
             }
             bool mx, my, mz;
             if (FAST) {
-                mx = sx <= sy && sx <= sz;
-                my = sy <= sz && sy <= sx;
-                mz = sz <= sx && sz <= sy;
+                const float tm = fminf(fminf(sx, sy), sz);
+                mx = sx <= tm;
+                my = sy <= tm;
+                mz = sz <= tm;
             } else {
                 mx = sx <= gmin(sy, sz);
                 my = sy <= gmin(sz, sx);
