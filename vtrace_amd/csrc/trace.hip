@@ -204,7 +204,8 @@ struct DdaWalk {
     int stx, sty, stz;
     uint32_t steps, entry, li;
     float t_last;
-    bool hit, lmx, lmy, lmz;     // axes of the last step (the mask, built once at the end)
+    uint32_t lm;                 // axes of the last step (bit a: axis a stepped)
+    bool hit;
 };
 
 // The walk over the bricked volume.  FAST: every delta is finite, so 0 * delta is 0, no side
@@ -224,7 +225,8 @@ __device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST
     const int stx = w.stx, sty = w.sty, stz = w.stz;
     uint32_t steps = 0, entry = VT_EMPTY, li = 0;
     float t_last = 0.0f;
-    bool hit = false, lmx = false, lmy = false, lmz = false;
+    uint32_t lm = 0;
+    bool hit = false;
 // One DDA step (trace.frag:83-86): axes whose side distance is the minimum step together;
 // float(mask) * delta is the select between delta and 0 * delta.
 #define VT_DDA_STEP()                                                                   \
@@ -247,9 +249,7 @@ __device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST
         vx += mx ? stx : 0;                                                             \
         vy += my ? sty : 0;                                                             \
         vz += mz ? stz : 0;                                                             \
-        lmx = mx;                                                                       \
-        lmy = my;                                                                       \
-        lmz = mz;                                                                       \
+        lm = (mx ? 1u : 0u) | (my ? 2u : 0u) | (mz ? 4u : 0u);                          \
         ++steps;                                                                        \
     }
     while (steps < max_steps && (uint32_t)vx < (uint32_t)nx && (uint32_t)vy < (uint32_t)ny &&
@@ -285,9 +285,8 @@ __device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST
                     sz += mz ? dz : zz;
                     ++steps;
                 }
-                lmx = sx <= t;
-                lmy = sy <= t;
-                lmz = sz <= t;
+                const bool lmx = sx <= t, lmy = sy <= t, lmz = sz <= t;
+                lm = (lmx ? 1u : 0u) | (lmy ? 2u : 0u) | (lmz ? 4u : 0u);
                 t_last = lmx ? sx : (lmy ? sy : sz);
                 sx += lmx ? dx : zx;
                 sy += lmy ? dy : zy;
@@ -304,6 +303,60 @@ __device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST
             continue;
         }
         stage_mask(V.masks, entry, mslot);
+        if (FAST) {
+            // The same remaining-step counts; the current voxel's brick-local coordinates are
+            // (ox, oy, oz) - (bx8, by8, bz8) - st * (rx, ry, rz).  The voxel a step moves into is
+            // tested before the step's side update, so when the loop ends (hit, or the step that
+            // leaves the brick) the sides are those before the last step, which gives its axes
+            // and side exactly as the step test does.
+            uint32_t rx = (uint32_t)(stx > 0 ? ox - vx : vx - ox);
+            uint32_t ry = (uint32_t)(sty > 0 ? oy - vy : vy - oy);
+            uint32_t rz = (uint32_t)(stz > 0 ? oz - vz : vz - oz);
+            const int bit0 = (ox - bx8) + 8 * (oy - by8), az = oz - bz8;
+            int lz = vz - bz8;
+            uint64_t slice = mask_slice(mslot, lz);
+            uint32_t bit = (uint32_t)(vx - bx8) + 8u * (uint32_t)(vy - by8);
+            if (((uint32_t)(slice >> bit) & 1u) == 0u) {
+                float t;
+                for (;;) {
+                    t = fminf(fminf(sx, sy), sz);
+                    const bool mx = sx <= t, my = sy <= t, mz = sz <= t;
+                    rx -= mx ? 1u : 0u;
+                    ry -= my ? 1u : 0u;
+                    rz -= mz ? 1u : 0u;
+                    if (min(rx, min(ry, rz)) == 0u) break;
+                    ++steps;
+                    const int nlz = az - __mul24(stz, (int)rz);
+                    if (nlz != lz) {
+                        lz = nlz;
+                        slice = mask_slice(mslot, lz);
+                    }
+                    bit = (uint32_t)(bit0 - __mul24(stx, (int)rx) - __mul24(8 * sty, (int)ry));
+                    if (((uint32_t)(slice >> bit) & 1u) != 0u) break;
+                    sx += mx ? dx : zx;
+                    sy += my ? dy : zy;
+                    sz += mz ? dz : zz;
+                }
+                const bool lmx = sx <= t, lmy = sy <= t, lmz = sz <= t;
+                lm = (lmx ? 1u : 0u) | (lmy ? 2u : 0u) | (lmz ? 4u : 0u);
+                t_last = lmx ? sx : (lmy ? sy : sz);
+                if (min(rx, min(ry, rz)) == 0u) {   // the step that leaves the brick
+                    sx += lmx ? dx : zx;
+                    sy += lmy ? dy : zy;
+                    sz += lmz ? dz : zz;
+                    ++steps;
+                }
+            }
+            vx = ox - stx * (int)rx;
+            vy = oy - sty * (int)ry;
+            vz = oz - stz * (int)rz;
+            if (min(rx, min(ry, rz)) != 0u) {
+                li = bit + 64u * (uint32_t)lz;
+                hit = true;
+                break;
+            }
+            continue;
+        }
         int cur_z = -1;
         uint64_t slice = 0;
         for (;;) {
@@ -326,7 +379,7 @@ __device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST
 #undef VT_DDA_STEP
     w.vx = vx; w.vy = vy; w.vz = vz;
     w.steps = steps; w.entry = entry; w.li = li; w.t_last = t_last;
-    w.hit = hit; w.lmx = lmx; w.lmy = lmy; w.lmz = lmz;
+    w.hit = hit; w.lm = lm;
 }
 
 template <bool ST>
@@ -348,7 +401,7 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
     else
         dda_walk<ST, false>(V, w, ctr, mslot);
     ctr.steps(w.steps);
-    const uint32_t mask = (w.lmx ? 1u : 0u) | (w.lmy ? 2u : 0u) | (w.lmz ? 4u : 0u);
+    const uint32_t mask = w.lm;
     r.v[0] = w.vx; r.v[1] = w.vy; r.v[2] = w.vz;
     r.steps = w.steps;
     r.mask = w.hit && w.steps > 0 ? mask : 0u;
@@ -444,40 +497,77 @@ __device__ __forceinline__ void dda2_walk(const VolumeDesc& V, const float p[3],
         float sy = ((sg[1] * ((float)v[1] - q[1]) + sg[1] * 0.5f) + 0.5f) * dl[1];
         float sz = ((sg[2] * ((float)v[2] - q[2]) + sg[2] * 0.5f) + 0.5f) * dl[2];
         float tl = 0.0f;
-        for (;;) {
-            const int lz = v[2] - b8[2];
-            if (lz != cur_z) {
-                cur_z = lz;
-                slice = mask_slice(mslot, lz);
+        if (FAST) {
+            // Every axis steps: remaining-step counts per axis as in the primary walk, the voxel's
+            // brick-local coordinates are out - b8 - st * r, and the voxel a step moves into is
+            // tested before the step's side update, so the sides left when the loop ends give the
+            // last step's axes and side.
+            uint32_t rx = (uint32_t)abs(out[0] - v[0]), ry = (uint32_t)abs(out[1] - v[1]), rz = (uint32_t)abs(out[2] - v[2]);
+            const int bit0 = (out[0] - b8[0]) + 8 * (out[1] - b8[1]), az = out[2] - b8[2];
+            int lz = v[2] - b8[2];
+            uint64_t sl = mask_slice(mslot, lz);
+            uint32_t bit = (uint32_t)(v[0] - b8[0]) + 8u * (uint32_t)(v[1] - b8[1]);
+            if (((uint32_t)(sl >> bit) & 1u) == 0u) {
+                float t;
+                for (;;) {
+                    t = fminf(fminf(sx, sy), sz);
+                    const bool mx = sx <= t, my = sy <= t, mz = sz <= t;
+                    rx -= mx ? 1u : 0u;
+                    ry -= my ? 1u : 0u;
+                    rz -= mz ? 1u : 0u;
+                    if (min(rx, min(ry, rz)) == 0u) break;
+                    ++steps;
+                    const int nlz = az - __mul24(st[2], (int)rz);
+                    if (nlz != lz) {
+                        lz = nlz;
+                        sl = mask_slice(mslot, lz);
+                    }
+                    bit = (uint32_t)(bit0 - __mul24(st[0], (int)rx) - __mul24(8 * st[1], (int)ry));
+                    if (((uint32_t)(sl >> bit) & 1u) != 0u) break;
+                    sx = mx ? sx + dl[0] : sx;
+                    sy = my ? sy + dl[1] : sy;
+                    sz = mz ? sz + dl[2] : sz;
+                }
+                const bool lmx = sx <= t, lmy = sy <= t, lmz = sz <= t;
+                mask = (lmx ? 1u : 0u) | (lmy ? 2u : 0u) | (lmz ? 4u : 0u);
+                tl = lmx ? sx : (lmy ? sy : sz);
+                if (min(rx, min(ry, rz)) == 0u) ++steps;   // the step that leaves the brick
             }
-            const uint32_t bit = (uint32_t)(v[0] - b8[0]) + 8u * (uint32_t)(v[1] - b8[1]);
-            if ((slice >> bit) & 1ull) {
+            v[0] = out[0] - st[0] * (int)rx;
+            v[1] = out[1] - st[1] * (int)ry;
+            v[2] = out[2] - st[2] * (int)rz;
+            if (min(rx, min(ry, rz)) != 0u) {
                 li = bit + 64u * (uint32_t)lz;
                 hit = true;
                 t_last = tl;
                 break;
             }
-            bool mx, my, mz;
-            if (FAST) {
-                const float tm = fminf(fminf(sx, sy), sz);
-                mx = sx <= tm;
-                my = sy <= tm;
-                mz = sz <= tm;
-            } else {
-                mx = sx <= gmin(sy, sz);
-                my = sy <= gmin(sz, sx);
-                mz = sz <= gmin(sx, sy);
+        } else {
+            for (;;) {
+                const int lz = v[2] - b8[2];
+                if (lz != cur_z) {
+                    cur_z = lz;
+                    slice = mask_slice(mslot, lz);
+                }
+                const uint32_t bit = (uint32_t)(v[0] - b8[0]) + 8u * (uint32_t)(v[1] - b8[1]);
+                if ((slice >> bit) & 1ull) {
+                    li = bit + 64u * (uint32_t)lz;
+                    hit = true;
+                    t_last = tl;
+                    break;
+                }
+                const bool mx = sx <= gmin(sy, sz), my = sy <= gmin(sz, sx), mz = sz <= gmin(sx, sy);
+                tl = mx ? sx : (my ? sy : sz);
+                sx = mx ? sx + dl[0] : sx;
+                sy = my ? sy + dl[1] : sy;
+                sz = mz ? sz + dl[2] : sz;
+                v[0] += mx ? st[0] : 0;
+                v[1] += my ? st[1] : 0;
+                v[2] += mz ? st[2] : 0;
+                mask = (mx ? 1u : 0u) | (my ? 2u : 0u) | (mz ? 4u : 0u);
+                ++steps;
+                if (v[0] == out[0] || v[1] == out[1] || v[2] == out[2]) break;
             }
-            tl = mx ? sx : (my ? sy : sz);
-            sx = mx ? sx + dl[0] : sx;
-            sy = my ? sy + dl[1] : sy;
-            sz = mz ? sz + dl[2] : sz;
-            v[0] += mx ? st[0] : 0;
-            v[1] += my ? st[1] : 0;
-            v[2] += mz ? st[2] : 0;
-            mask = (mx ? 1u : 0u) | (my ? 2u : 0u) | (mz ? 4u : 0u);
-            ++steps;
-            if (v[0] == out[0] || v[1] == out[1] || v[2] == out[2]) break;
         }
         if (hit) break;
 #pragma unroll
