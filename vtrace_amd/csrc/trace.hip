@@ -42,10 +42,19 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kStackLds = 16;   // LBVH stack entries per thread held in LDS
-// wavefront counters: [0] queue of the primary vertices, [2] primary fetch, [3, 64) fetch
-// counters of the secondary launches, [64 + d + 1] the queue built at depth d
-constexpr uint32_t kCounters = 128;
+// Wavefront counter slots: [0] queue of the primary vertices, [2] primary (and mode 0) fetch,
+// [3, 64) fetch counters of the secondary launches, [64 + d + 1] the queue built at depth d.
+// Device-scope atomics on one address serialise (about 12 ns each on MI355X), which would bound
+// a launch at (items / 64) x 12 ns, so every slot holds one counter per part, kPartStride u32
+// apart: workgroups are dispatched round-robin over the 8 XCDs, workgroup b starts on part
+// b % kParts, and a launch's atomics spread over kParts addresses.  A queue is kParts
+// segments of seg_cap slots; part p of a queue's counters fills segment p.
+constexpr uint32_t kSlots = 128;
 constexpr uint32_t kQueueCounters = 64;
+constexpr uint32_t kParts = 8;
+constexpr uint32_t kPartStride = 64;   // 256 B between the counters of one slot
+constexpr uint32_t kCounters = kSlots * kParts * kPartStride;
+__host__ __device__ constexpr uint32_t seg_cap(uint32_t npaths) { return ((npaths + kParts - 1) / kParts + 63u) & ~63u; }
 
 // GLSL min/max as "y if y < x else x" (also fixes the NaN cases the DDA can produce)
 __device__ __forceinline__ float gmin(float x, float y) { return (y < x) ? y : x; }
@@ -938,31 +947,73 @@ __device__ __forceinline__ V3 camera_ray(const FrameGPU& F, float sx, float sy) 
 }
 
 // ---- persistent work distribution -----------------------------------------------------
-// Each wave takes 64 consecutive items from *fetch until `count` is exhausted.
-template <typename Body>
-__device__ __forceinline__ void for_each_item(uint32_t* fetch, uint32_t count, Body&& body) {
+__host__ __device__ __forceinline__ uint32_t* slot_counters(uint32_t* counters, uint32_t slot) {
+    return counters + slot * kParts * kPartStride;
+}
+
+// Each wave takes 64 consecutive items of a part from that part's counter (fetch + p *
+// kPartStride) until the part is exhausted, starting with part blockIdx.x % kParts and then
+// taking the others in turn.  range(p, &first, &count): part p's items are first + [0, count);
+// body(p, item).  `gone` (LDS, zero at start) marks the parts the block has seen exhausted, so
+// only one wave per block spends an atomic on finding that out.
+template <typename Range, typename Body>
+__device__ __forceinline__ void for_each_item(uint32_t* fetch, uint32_t* gone, Range&& range, Body&& body) {
     const uint32_t lane = threadIdx.x & 63u;
-    for (;;) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(fetch, 64u);
-        base = __shfl(base, 0, 64);
-        if (base >= count) break;
-        const uint32_t i = base + lane;
-        if (i < count) body(i);
+    const uint32_t p0 = blockIdx.x % kParts;
+    for (uint32_t k = 0; k < kParts; ++k) {
+        const uint32_t p = (p0 + k) % kParts;
+        uint32_t first = 0, count = 0;
+        range(p, &first, &count);
+        const uint32_t tiles = (count + 63u) / 64u;
+        for (;;) {
+            if (__hip_atomic_load(gone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & (1u << p)) break;
+            uint32_t t = tiles;
+            if (tiles > 0 && lane == 0) t = atomicAdd(fetch + p * kPartStride, 1u);
+            t = __shfl(t, 0, 64);
+            if (t >= tiles) {
+                if (lane == 0) __hip_atomic_fetch_or(gone, 1u << p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                break;
+            }
+            const uint32_t i = t * 64u + lane;
+            if (i < count) body(p, first + i);
+        }
     }
 }
 
-// Wave-aggregated append: one atomic per wave, slots in lane order.
-__device__ __forceinline__ uint32_t append_slot(uint32_t* counter) {
+// Items [0, nwork) in kParts contiguous ranges of whole 64-item tiles.
+__device__ __forceinline__ void static_part(uint32_t nwork, uint32_t p, uint32_t* first, uint32_t* count) {
+    const uint32_t tiles = (nwork + 63u) / 64u;
+    const uint32_t t0 = (uint32_t)((uint64_t)tiles * p / kParts), t1 = (uint32_t)((uint64_t)tiles * (p + 1u) / kParts);
+    *first = t0 * 64u;
+    *count = min(t1 * 64u, nwork) - min(t0 * 64u, nwork);
+}
+
+// Slots used in segment p of a queue (its counter can pass seg_cap, see append_slot).
+__device__ __forceinline__ uint32_t queue_count(uint32_t* q, uint32_t p, uint32_t cap) {
+    return min(q[p * kPartStride], cap);
+}
+
+// Wave-aggregated append to queue counters q (segments of `cap` slots): the wave's appending
+// lanes take consecutive slots of segment blockIdx.x % kParts in lane order with one atomic;
+// what does not fit there goes on to the next segments (a counter past cap marks its segment
+// full).  The segments together hold every path, so the loop always places all lanes.
+__device__ __forceinline__ uint32_t append_slot(uint32_t* q, uint32_t cap) {
     const uint64_t m = __ballot(1);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
     const uint32_t n = (uint32_t)__popcll(m);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, n);
-    base = __shfl(base, (int)leader, 64);
     const uint64_t lower = lane == 0 ? 0ull : (m & (~0ull >> (64u - lane)));
-    return base + (uint32_t)__popcll(lower);
+    const uint32_t rank = (uint32_t)__popcll(lower);
+    uint32_t slot = 0, placed = 0, p = blockIdx.x % kParts;
+    for (uint32_t k = 0; k < kParts && placed < n; ++k, p = (p + 1u) % kParts) {
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(q + p * kPartStride, n - placed);
+        base = __shfl(base, (int)leader, 64);
+        const uint32_t take = base < cap ? min(cap - base, n - placed) : 0u;
+        if (rank >= placed && rank < placed + take) slot = p * cap + base + (rank - placed);
+        placed += take;
+    }
+    return slot;
 }
 
 // Wavefront buffers (device pointers), see WaveBuffers in vt_internal.h.
@@ -975,7 +1026,8 @@ struct WF {
     SurfaceRec* surf[2];  // per queue slot
     float4* contrib[2];   // per slot: direct-light contribution if the sun ray is clear (w = 1: trace it)
     float4* dir[2];       // per slot: bounce direction (w = 1: trace it)
-    uint32_t* counters;   // [0,1] queue sizes, [2..] fetch counters
+    uint32_t* counters;   // kSlots x kParts counters (queue sizes and fetch counters)
+    uint32_t seg_cap;     // slots per queue segment
     float4* gsamp;        // optional: per path (albedo rgb, distance), (normal, hit)
 };
 
@@ -1063,11 +1115,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
     __shared__ BvhNode tree_lds[kLdsNodes];
     const Lds stk{&smem[threadIdx.x]}, mslot{&smem[kStackLds * kBlock + threadIdx.x]};
+    __shared__ uint32_t gone;
+    if (threadIdx.x == 0) gone = 0u;
     const TreeLds tree = stage_tree(reqs, B, tree_lds);
+    __syncthreads();
     Ctr<ST> ctr;
     const uint32_t nreq = B->nreq;
     const uint32_t nwork = B->work_base[nreq];
-    for_each_item(&wf.counters[2], nwork, [&](uint32_t w) {
+    for_each_item(slot_counters(wf.counters, 2), &gone,
+                  [&](uint32_t p, uint32_t* first, uint32_t* count) { static_part(nwork, p, first, count); },
+                  [&](uint32_t, uint32_t w) {
         // the 64 items of a wave share a request (work ranges are 64-aligned)
         const uint32_t r = __builtin_amdgcn_readfirstlane(find_req(B->work_base, nreq, w));
         const ReqGPU& Rq = reqs[r];
@@ -1133,7 +1190,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
             bool no_sun;
             shade(F, srgb_dec, sf, 0u, max_b, beta, rng, contrib, dirw, no_sun);
             shadowed = no_sun ? 1 : 0;
-            const uint32_t slot = append_slot(&wf.counters[0]);
+            const uint32_t slot = append_slot(slot_counters(wf.counters, 0), wf.seg_cap);
             wf.surf[0][slot] = sf;
             wf.contrib[0][slot] = contrib;
             wf.dir[0][slot] = dirw;
@@ -1147,11 +1204,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     if (ST) ctr.flush(stats);
 }
 
-// The secondary rays of one depth in one launch: items [0, nb) are the bounce rays of queue
-// slots 0..nb-1 (nb = the queue size when paths continue, else 0), items [nb, nb + count) the
-// sun rays of slots 0..count-1.  The two kinds write different per-path state (bounce rays: L,
-// beta, rng and the next queue; sun rays: Ls and the shadow flag), so they run side by side and
-// fill each other's tail.  in_idx / out_idx: counters of this depth's queue and the next one.
+// The secondary rays of one depth in one launch.  Part p covers segment p of this depth's queue
+// (count slots): items [0, nb) are the bounce rays of its slots 0..nb-1 (nb = count when paths
+// continue, else 0), items [nb, nb + count) the sun rays of slots 0..count-1.  The two kinds
+// write different per-path state (bounce rays: L, beta, rng and the next queue; sun rays: Ls and
+// the shadow flag), so they run side by side and fill each other's tail.  in_idx / out_idx:
+// counter slots of this depth's queue and the next one.
 template <bool ST, bool DEEP>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_secondary(
     const ReqGPU* __restrict__ reqs, const BatchGPU* __restrict__ B, const float* __restrict__ srgb_dec, WF wf,
@@ -1160,15 +1218,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
     __shared__ BvhNode tree_lds[kLdsNodes];
     const Lds stk{&smem[threadIdx.x]}, mslot{&smem[kStackLds * kBlock + threadIdx.x]};
+    __shared__ uint32_t gone;
+    if (threadIdx.x == 0) gone = 0u;
     const TreeLds tree = stage_tree(reqs, B, tree_lds);
+    __syncthreads();
     Ctr<ST> ctr;
-    const uint32_t count = wf.counters[in_idx];
-    const uint32_t nb = bounce ? count : 0u;
+    uint32_t* const qin = slot_counters(wf.counters, in_idx);
+    const uint32_t cap = wf.seg_cap;
     const uint32_t nq = 1u - q;
     const uint32_t nreq = B->nreq;
-    for_each_item(&wf.counters[fetch_idx], nb + count, [&](uint32_t item) {
+    for_each_item(slot_counters(wf.counters, fetch_idx), &gone,
+                  [&](uint32_t p, uint32_t* first, uint32_t* n) {
+                      const uint32_t c = queue_count(qin, p, cap);
+                      *first = 0u;
+                      *n = bounce ? 2u * c : c;
+                  },
+                  [&](uint32_t p, uint32_t item) {
+        const uint32_t nb = bounce ? queue_count(qin, p, cap) : 0u;
         const bool is_bounce = item < nb;
-        const uint32_t slot = is_bounce ? item : item - nb;
+        const uint32_t slot = p * cap + (is_bounce ? item : item - nb);
         const float4 dw = is_bounce ? wf.dir[q][slot] : wf.contrib[q][slot];  // w = 1: trace it
         if (dw.w == 0.0f) return;
         const SurfaceRec sf = wf.surf[q][slot];
@@ -1210,7 +1278,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         bool no_sun;
         const uint32_t max_b = F.mode == 2 ? F.max_bounces : 0u;
         shade(F, srgb_dec, ns, depth + 1u, max_b, beta, rng, contrib, dirw, no_sun);
-        const uint32_t ns_slot = append_slot(&wf.counters[out_idx]);
+        const uint32_t ns_slot = append_slot(slot_counters(wf.counters, out_idx), cap);
         wf.surf[nq][ns_slot] = ns;
         wf.contrib[nq][ns_slot] = contrib;
         wf.dir[nq][ns_slot] = dirw;
@@ -1278,9 +1346,9 @@ __global__ void k_accumulate(const ReqGPU* __restrict__ reqs, const BatchGPU* __
         accumulate_pixel(reqs, B, wf, t);
 }
 
-__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+__global__ void k_zero_counters(uint32_t* c) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
+    if (i < kSlots * kParts) c[i * kPartStride] = 0u;
 }
 
 // Mode 0: one thread per pixel, samples looped in-thread (primary colour only).
@@ -1290,9 +1358,13 @@ __global__ __launch_bounds__(kBlock) void k_render0(const FrameGPU F, const Scen
                                                     uint32_t* fetch, uint32_t nwork, unsigned long long* stats) {
     __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
     const Lds stk{&smem[threadIdx.x]}, mslot{&smem[kStackLds * kBlock + threadIdx.x]};
+    __shared__ uint32_t gone;
+    if (threadIdx.x == 0) gone = 0u;
+    __syncthreads();
     Ctr<ST> ctr;
     const uint32_t spp = F.spp ? F.spp : 1u;
-    for_each_item(fetch, nwork, [&](uint32_t w) {
+    for_each_item(fetch, &gone, [&](uint32_t p, uint32_t* first, uint32_t* count) { static_part(nwork, p, first, count); },
+                  [&](uint32_t, uint32_t w) {
         uint32_t px, py, s0;
         if (!work_pixel(F, w, 1u, &px, &py, &s0)) return;
         const uint32_t pix = py * F.width + px;
@@ -1373,7 +1445,8 @@ void vt_trace_set_tables(const float* dec_dev, const float* thr_dev) {
 void vt_trace_set_persistent_blocks(uint32_t blocks) { g_persistent_blocks = blocks ? blocks : 2048; }
 
 size_t vt_wave_bytes(uint32_t npaths) {
-    return (size_t)npaths * (16 + 16 + 16 + 4 + 1) + 2 * (size_t)npaths * (64 + 16 + 16) + kCounters * 4 + 16 * 256 +
+    const size_t qcap = (size_t)kParts * seg_cap(npaths);
+    return (size_t)npaths * (16 + 16 + 16 + 4 + 1) + 2 * qcap * (64 + 16 + 16) + (size_t)kCounters * 4 + 16 * 256 +
            sizeof(ReqGPU) * VT_MAX_BATCH + sizeof(BatchGPU);
 }
 
@@ -1390,12 +1463,13 @@ int vt_wave_bind(WaveBuffers& wb, void* mem, uint32_t npaths) {
     wb.beta = take((size_t)npaths * 16);
     wb.rng = take((size_t)npaths * 4);
     wb.shadowed = take((size_t)npaths);
+    const size_t qcap = (size_t)kParts * seg_cap(npaths);
     for (int q = 0; q < 2; ++q) {
-        wb.surf[q] = take((size_t)npaths * 64);
-        wb.contrib[q] = take((size_t)npaths * 16);
-        wb.dir[q] = take((size_t)npaths * 16);
+        wb.surf[q] = take(qcap * 64);
+        wb.contrib[q] = take(qcap * 16);
+        wb.dir[q] = take(qcap * 16);
     }
-    wb.counters = reinterpret_cast<uint32_t*>(take(kCounters * 4));
+    wb.counters = reinterpret_cast<uint32_t*>(take((size_t)kCounters * 4));
     wb.tables = take(sizeof(ReqGPU) * VT_MAX_BATCH + sizeof(BatchGPU));
     return 0;
 }
@@ -1413,6 +1487,7 @@ static WF to_wf(const WaveBuffers& wb) {
         w.dir[q] = reinterpret_cast<float4*>(wb.dir[q]);
     }
     w.counters = wb.counters;
+    w.seg_cap = seg_cap(wb.npaths);
     w.gsamp = reinterpret_cast<float4*>(wb.gsamp);
     return w;
 }
@@ -1489,12 +1564,12 @@ static int launch_batch(const BatchGPU& Bh, const WaveBuffers& wb, unsigned long
         ++stage;
     };
     auto st_at = [&](uint32_t k) { return stats ? stats + (size_t)k * VT_STAT_COUNT : nullptr; };
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kCounters), 0, s, wb.counters, kCounters);
+    hipLaunchKernelGGL(k_zero_counters, dim3(kSlots * kParts / 256), dim3(256), 0, s, wb.counters);
     begin(VT_STAGE_PRIMARY);
     hipLaunchKernelGGL((k_primary<ST, DEEP>), grid, block, 0, s, reqs, B, g_srgb_dec, wf, st_at(stage));
     end();
     uint32_t fetch = 3, in_idx = 0;
-    for (uint32_t d = 0; d <= Bh.max_bounces && fetch < kQueueCounters && kQueueCounters + d + 1 < kCounters; ++d) {
+    for (uint32_t d = 0; d <= Bh.max_bounces && fetch < kQueueCounters && kQueueCounters + d + 1 < kSlots; ++d) {
         const uint32_t out_idx = kQueueCounters + d + 1;
         begin(VT_STAGE_SECONDARY);
         hipLaunchKernelGGL((k_secondary<ST, DEEP>), grid, block, 0, s, reqs, B, g_srgb_dec, wf, d & 1u, d, in_idx,
@@ -1587,11 +1662,11 @@ static int launch_render0(const FrameGPU& F, const SceneGPU& S, HitRecord* hits,
                           unsigned long long* stats, hipEvent_t* ev, uint8_t* kinds, uint32_t* nstages,
                           hipStream_t s) {
     const uint32_t tiles = ((F.width + 7u) / 8u) * ((F.height + 7u) / 8u);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, wb.counters, 64u);
+    hipLaunchKernelGGL(k_zero_counters, dim3(kSlots * kParts / 256), dim3(256), 0, s, wb.counters);
     if (ev) hipEventRecord(ev[0], s);
     if (kinds) kinds[0] = VT_STAGE_RENDER0;
     hipLaunchKernelGGL((k_render0<ST, DEEP>), dim3(g_persistent_blocks), dim3(kBlock), 0, s, F, S, g_srgb_dec, hits,
-                       rad, &wb.counters[2], tiles * 64u, stats);
+                       rad, slot_counters(wb.counters, 2), tiles * 64u, stats);
     if (ev) hipEventRecord(ev[1], s);
     if (nstages) *nstages = 1;
     return hipGetLastError() == hipSuccess ? 0 : -1;
