@@ -867,41 +867,121 @@ __device__ void surface_of(const SceneGPU& S, const HitG& h, SurfaceRec& sf) {
     sf.face = h.face;
 }
 
+// Secondary query from a surface: the origin instance from the surface point first, then (if
+// that walk misses) the other instances the ray enters, in the tree's traversal order with the
+// closest-hit pruning of query().  Every instance walk runs through one call of the walk (one
+// walk per loop iteration), so lanes walking the origin and lanes walking another instance
+// share its instructions instead of running two copies of the walk one after the other.
 template <bool ST, bool DEEP>
 __device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const SurfaceRec& sf, V3 w, bool any_hit,
                           Ctr<ST>& ctr, const Lds& stk_lds, const Lds& mslot, TreeLds tree) {
     ctr.ray();
-    HitG h;
-    h.hit = false;
+    HitG best;
+    best.hit = false;
+    best.inst = VT_EMPTY;
+    best.tn = INFINITY;
     const uint32_t origin = sf.inst;
-    const InstanceGPU* I = &S.inst[origin];
-    float inv[12];
+    const V3 O{sf.X[0], sf.X[1], sf.X[2]};
+    const V3 inv_d{1.0f / w.x, 1.0f / w.y, 1.0f / w.z};
+    TravStack<DEEP> stk;
+    stk.lds = stk_lds;
+    stk.sp = 0;
+    // the instance to walk next and its entry (the origin: the surface point and start voxel)
+    uint32_t inst = origin;
+    float mpos[3] = {sf.p[0], sf.p[1], sf.p[2]};
+    int v0[3] = {sf.vstart[0], sf.vstart[1], sf.vstart[2]};
+    float tn = 0.0f;
+    uint8_t face0 = (uint8_t)sf.face;   // face when the walk hits its start voxel
+    V3 mdir;
+    {
+        const InstanceGPU* I = &S.inst[origin];
+        float inv[12];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) inv[k] = gld(&I->inv[k]);
-    const V3 mdir = xf34(inv, w, 0.0f);
-    ctr.instance();
-    const VolumeDesc V = S.vols[I->tex];
-    Dda r;
-    dda2_run<ST>(V, sf.p, sf.vstart, mdir, r, ctr, mslot);
-    if (r.hit) {
-        h.hit = true;
-        h.inst = origin;
-        h.tn = 0.0f;
-        h.d = r;
-        h.mpos[0] = sf.p[0]; h.mpos[1] = sf.p[1]; h.mpos[2] = sf.p[2];
-        h.mdir = mdir;
-        if (r.steps > 0) {
-            int a = lowest_axis(r.mask);
-            h.face = face_from(a, comp(mdir, a));
-        } else {
-            h.face = (uint8_t)sf.face;
-        }
-        ctr.hit();
-        return h;
+        for (int k = 0; k < 12; ++k) inv[k] = gld(&I->inv[k]);
+        mdir = xf34(inv, w, 0.0f);
     }
-    h = query<false, ST, DEEP>(S, F, V3{sf.X[0], sf.X[1], sf.X[2]}, w, origin, any_hit, ctr, stk_lds, mslot, tree);
-    if (h.hit) ctr.hit();
-    return h;
+    bool walk = true;
+    for (;;) {
+        if (walk) {
+            ctr.instance();
+            const VolumeDesc V = S.vols[S.inst[inst].tex];
+            Dda r;
+            dda2_run<ST>(V, mpos, v0, mdir, r, ctr, mslot);
+            if (r.hit) {
+                HitG h;
+                h.hit = true;
+                h.inst = inst;
+                h.tn = tn;
+                h.d = r;
+                h.mpos[0] = mpos[0]; h.mpos[1] = mpos[1]; h.mpos[2] = mpos[2];
+                h.mdir = mdir;
+                if (r.steps > 0) {
+                    const int a = lowest_axis(r.mask);
+                    h.face = face_from(a, comp(mdir, a));
+                } else {
+                    h.face = face0;
+                }
+                if (inst == origin) {
+                    ctr.hit();
+                    return h;
+                }
+                if (better(h, best)) {
+                    best = h;
+                    if (any_hit) break;
+                }
+            }
+            if (inst == origin && S.ninst > 0) stk.push(0u);   // the scene query follows the origin walk
+            walk = false;
+        }
+        // the next instance the ray enters (query() with enter_instance for secondary rays)
+        while (stk.sp > 0) {
+            const uint32_t ni = stk.pop();
+            const BvhNode N = tree.on ? lds_node(tree, ni) : gld(S.nodes + ni);
+            ctr.node();
+            float a0, a1;
+            if (!slab_node(O, w, inv_d, N, &a0, &a1) || a1 < 0.0f) continue;
+            if (best.hit && a0 > prune_bound(best.tn)) continue;
+            if (N.right != VT_EMPTY) {
+                stk.push(N.right);
+                stk.push(N.left);
+                continue;
+            }
+            const uint32_t i = N.left;
+            if (i == origin) continue;
+            const InstanceGPU* I = &S.inst[i];
+            if (!gld(&I->valid)) continue;
+            float inv[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) inv[k] = gld(&I->inv[k]);
+            const V3 o = xf34(inv, O, 1.0f);
+            const V3 d = xf34(inv, w, 0.0f);
+            const SlabR sl = slab_unit(o, d);
+            if (!sl.ok || !(sl.tn > 0.0f)) continue;
+            const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+            const float size[3] = {I->size[0], I->size[1], I->size[2]};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                float mp;
+                if (a == sl.axis) {
+                    mp = dd[a] > 0.0f ? -0.5f : 0.5f;
+                } else {
+                    const float x = oo[a] + sl.tn * dd[a];
+                    mp = x < -0.5f ? -0.5f : (x > 0.5f ? 0.5f : x);
+                }
+                mpos[a] = (mp + 0.5f) * size[a];
+                v0[a] = (int)floorf(gmin(mpos[a], size[a] - 1.0f));
+            }
+            mdir = d;
+            tn = sl.tn;
+            face0 = face_from(sl.axis, dd[sl.axis]);
+            inst = i;
+            walk = true;
+            break;
+        }
+        if (!walk) break;
+    }
+    if (best.hit) ctr.hit();
+    return best;
 }
 
 __device__ void cosine_dir(Rng& rng, V3 n, V3& w) {
