@@ -457,10 +457,14 @@ __device__ __forceinline__ void dda2_walk(const VolumeDesc& V, const float p[3],
     const uint32_t guard = 2u * (V.bdim[0] + V.bdim[1] + V.bdim[2]) + 8u;
     for (uint32_t it = 0; it < guard; ++it) {
         if (!((uint32_t)v[0] < (uint32_t)n[0] && (uint32_t)v[1] < (uint32_t)n[1] && (uint32_t)v[2] < (uint32_t)n[2])) break;
-        const int b8[3] = {v[0] & ~7, v[1] & ~7, v[2] & ~7};
+        int b8[3] = {v[0] & ~7, v[1] & ~7, v[2] & ~7};
         entry = gld(V.brick_map + (uint32_t)(b8[0] >> 3) + bdx * ((uint32_t)(b8[1] >> 3) + bdy * (uint32_t)(b8[2] >> 3)));
         ctr.brick(!vt_brick_empty(entry));
-        if (vt_brick_empty(entry)) {
+        // Consecutive empty cubes are crossed in this inner loop (one jump per iteration), so
+        // lanes crossing empty space keep jumping together before the non-empty bricks are
+        // handled.
+        bool outside = false;
+        while (vt_brick_empty(entry)) {
             // the cube of empty bricks of this brick's radius is crossed in one step
             const int rad = 8 * (int)(entry & 0xFFu);
             float t[3];
@@ -493,8 +497,15 @@ __device__ __forceinline__ void dda2_walk(const VolumeDesc& V, const float p[3],
             }
             mask = mk;
             ++steps;
-            continue;
+            if (++it >= guard || !((uint32_t)v[0] < (uint32_t)n[0] && (uint32_t)v[1] < (uint32_t)n[1] && (uint32_t)v[2] < (uint32_t)n[2])) {
+                outside = true;
+                break;
+            }
+            b8[0] = v[0] & ~7; b8[1] = v[1] & ~7; b8[2] = v[2] & ~7;
+            entry = gld(V.brick_map + (uint32_t)(b8[0] >> 3) + bdx * ((uint32_t)(b8[1] >> 3) + bdy * (uint32_t)(b8[2] >> 3)));
+            ctr.brick(!vt_brick_empty(entry));
         }
+        if (outside) break;
         stage_mask(V.masks, entry, mslot);
         int cur_z = -1;
         uint64_t slice = 0;
