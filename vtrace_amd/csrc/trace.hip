@@ -301,9 +301,9 @@ __device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST
                 sy += lmy ? dy : zy;
                 sz += lmz ? dz : zz;
                 ++steps;
-                vx = ox - stx * (int)rx;
-                vy = oy - sty * (int)ry;
-                vz = oz - stz * (int)rz;
+                vx = ox - __mul24(stx, (int)rx);
+                vy = oy - __mul24(sty, (int)ry);
+                vz = oz - __mul24(stz, (int)rz);
                 continue;
             }
             do {
@@ -356,9 +356,9 @@ __device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST
                     ++steps;
                 }
             }
-            vx = ox - stx * (int)rx;
-            vy = oy - sty * (int)ry;
-            vz = oz - stz * (int)rz;
+            vx = ox - __mul24(stx, (int)rx);
+            vy = oy - __mul24(sty, (int)ry);
+            vz = oz - __mul24(stz, (int)rz);
             if (min(rx, min(ry, rz)) != 0u) {
                 li = bit + 64u * (uint32_t)lz;
                 hit = true;
@@ -553,9 +553,9 @@ __device__ __forceinline__ void dda2_walk(const VolumeDesc& V, const float p[3],
                 tl = lmx ? sx : (lmy ? sy : sz);
                 if (min(rx, min(ry, rz)) == 0u) ++steps;   // the step that leaves the brick
             }
-            v[0] = out[0] - st[0] * (int)rx;
-            v[1] = out[1] - st[1] * (int)ry;
-            v[2] = out[2] - st[2] * (int)rz;
+            v[0] = out[0] - __mul24(st[0], (int)rx);
+            v[1] = out[1] - __mul24(st[1], (int)ry);
+            v[2] = out[2] - __mul24(st[2], (int)rz);
             if (min(rx, min(ry, rz)) != 0u) {
                 li = bit + 64u * (uint32_t)lz;
                 hit = true;
