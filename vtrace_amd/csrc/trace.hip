@@ -1300,8 +1300,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
 // continue, else 0), items [nb, nb + count) the sun rays of slots 0..count-1.  The two kinds
 // write different per-path state (bounce rays: L, beta, rng and the next queue; sun rays: Ls and
 // the shadow flag), so they run side by side and fill each other's tail.  in_idx / out_idx:
-// counter slots of this depth's queue and the next one.
-template <bool ST, bool DEEP>
+// counter slots of this depth's queue and the next one.  ONE: the batch holds one request (its
+// scene and frame are then wave-uniform, read with scalar loads).
+template <bool ST, bool DEEP, bool ONE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_secondary(
     const ReqGPU* __restrict__ reqs, const BatchGPU* __restrict__ B, const float* __restrict__ srgb_dec, WF wf,
     uint32_t q, uint32_t depth, uint32_t in_idx, uint32_t out_idx, uint32_t fetch_idx, uint32_t bounce,
@@ -1332,7 +1333,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         if (dw.w == 0.0f) return;
         const SurfaceRec sf = wf.surf[q][slot];
         const uint32_t path = sf.path;
-        const ReqGPU& Rq = reqs[find_req(B->work_base, nreq, path)];
+        const ReqGPU& Rq = reqs[ONE ? 0u : find_req(B->work_base, nreq, path)];
         const SceneGPU S = Rq.S;
         const FrameGPU& F = Rq.F;
         const V3 w = is_bounce ? V3{dw.x, dw.y, dw.z} : V3{F.sun[0], F.sun[1], F.sun[2]};
@@ -1639,7 +1640,7 @@ bool vt_batch_layout(const ReqGPU* reqs, uint32_t n, uint32_t npaths, BatchGPU& 
 template <bool ST, bool DEEP>
 static int launch_batch(const BatchGPU& Bh, const WaveBuffers& wb, unsigned long long* stats, hipEvent_t* ev,
                         uint8_t* kinds, uint32_t* nstages, hipStream_t s, hipEvent_t acc_after = nullptr,
-                        bool with_acc = true, uint32_t blocks = 0) {
+                        bool with_acc = true, uint32_t blocks = 0, bool allow_one = true) {
     const uint32_t nblk = blocks ? blocks : g_persistent_blocks;
     const dim3 grid(nblk), block(kBlock);
     const ReqGPU* reqs = reinterpret_cast<const ReqGPU*>(wb.tables);
@@ -1663,8 +1664,13 @@ static int launch_batch(const BatchGPU& Bh, const WaveBuffers& wb, unsigned long
     for (uint32_t d = 0; d <= Bh.max_bounces && fetch < kQueueCounters && kQueueCounters + d + 1 < kSlots; ++d) {
         const uint32_t out_idx = kQueueCounters + d + 1;
         begin(VT_STAGE_SECONDARY);
-        hipLaunchKernelGGL((k_secondary<ST, DEEP>), grid, block, 0, s, reqs, B, g_srgb_dec, wf, d & 1u, d, in_idx,
-                           out_idx, fetch++, d < Bh.max_bounces ? 1u : 0u, st_at(stage));
+        if (allow_one && Bh.nreq == 1)
+            hipLaunchKernelGGL((k_secondary<ST, DEEP, true>), grid, block, 0, s, reqs, B, g_srgb_dec, wf, d & 1u, d,
+                               in_idx, out_idx, fetch, d < Bh.max_bounces ? 1u : 0u, st_at(stage));
+        else
+            hipLaunchKernelGGL((k_secondary<ST, DEEP, false>), grid, block, 0, s, reqs, B, g_srgb_dec, wf, d & 1u, d,
+                               in_idx, out_idx, fetch, d < Bh.max_bounces ? 1u : 0u, st_at(stage));
+        ++fetch;
         end();
         in_idx = out_idx;
     }
@@ -1716,8 +1722,10 @@ int vt_trace_capture(uint32_t max_bounces, bool deep, const WaveBuffers& wb, hip
     Bh.max_bounces = max_bounces;
     hipGraph_t graph;
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return -1;
-    int rc = deep ? launch_batch<false, true>(Bh, wb, nullptr, nullptr, nullptr, nullptr, s, nullptr, with_acc, blocks)
-                  : launch_batch<false, false>(Bh, wb, nullptr, nullptr, nullptr, nullptr, s, nullptr, with_acc, blocks);
+    int rc = deep ? launch_batch<false, true>(Bh, wb, nullptr, nullptr, nullptr, nullptr, s, nullptr, with_acc, blocks,
+                                              false)
+                  : launch_batch<false, false>(Bh, wb, nullptr, nullptr, nullptr, nullptr, s, nullptr, with_acc, blocks,
+                                               false);
     if (hipStreamEndCapture(s, &graph) != hipSuccess || rc != 0) return -1;
     rc = hipGraphInstantiate(out, graph, nullptr, nullptr, 0) == hipSuccess ? 0 : -1;
     hipGraphDestroy(graph);
