@@ -342,7 +342,7 @@ def benchmark(renderer, nreq: int = 2000, util: float = 0.75, seed: int = 11, pr
     loop = ServingLoop(renderer, tile_w=tune["tile_w"], use_graph=use_graph, **cfg)
     for rc in recs:
         loop.add_scene(rc)
-    loop.run_all(probe_reqs[:4])  # warm-up (graph capture, first-touch of buffers)
+    loop.run_all(probe_reqs)  # warm-up: graph captures of the probe's batch shapes, first-touch of buffers
     t0 = time.perf_counter()
     loop.run_all(probe_reqs)
     cap_loop = len(probe_reqs) / (time.perf_counter() - t0)
