@@ -111,6 +111,35 @@ def test_edge_camera_inside_box_and_axis_aligned():
     assert (g_hits["inst"] == 0xFFFFFFFF).all()
 
 
+def test_diagonal_ties_and_partial_bricks():
+    """Square frame with an axis-aligned camera over x = y: the diagonal pixels' rays have equal x
+    and y direction components and entry coordinates, so their DDA steps x and y together (side
+    ties, trace.frag:80-83); 44^3 ends in partial bricks.  Mode 1 adds sun rays along (1, 1, 1)
+    (equal components again, the two-level walk's ties).  On the oracle these views end 10-32
+    walks with a multi-axis step."""
+    n = 44
+    vol = scenes.Volume(scenes.gen_volume((n, n, n), 9, 0.15), scenes.gen_palette(9))
+    rec = scenes.instance_record(scenes.model_matrix((0.0, 0.0, 0.0), float(n)), 0)[None, :]
+    s3 = float(np.float32(1.0) / np.sqrt(np.float32(3.0)))
+    for mode in (0, 1):
+        p = scenes.RenderParams(256, 256, viewport_square=0, mode=mode, sun_dir=(s3, s3, s3))
+        for eye in ((0.37, 0.37, 0.9 * n), (-0.25 * n, -0.25 * n, 0.8 * n), (2.0, 2.0, 0.7 * n)):
+            sc = scenes.Scene([vol], rec, scenes.perspective(scenes.FOV, 1.0),
+                              scenes.look_at(eye, (eye[0], eye[1], 0.0)), p)
+            _say(f"case: diagonal mode={mode} eye={eye}")
+            g_hits, g_rad, g_st = gpu_render(sc, stats=True)
+            o_hits, o_rad, o_st = O.render(sc, use_bvh=True)
+            assert_hits_equal(g_hits, o_hits, f"diagonal {mode} {eye}")
+            hit = o_hits["inst"] != 0xFFFFFFFF
+            assert (np.isin(o_hits["mask"], [3, 5, 6, 7]) & hit).sum() >= 5
+            if mode == 0:
+                assert np.array_equal(g_rad.view(np.uint32), o_rad.view(np.uint32))
+            else:
+                mx, _ = radiance_error(g_rad, o_rad)
+                assert mx <= REL_TOL
+            assert g_st == o_st, (g_st, o_st)
+
+
 def test_invalid_texture_id_and_zero_instances():
     n = 16
     vol = scenes.Volume(scenes.gen_volume((n, n, n), 9, 0.5), scenes.gen_palette(9))
