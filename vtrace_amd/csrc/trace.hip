@@ -689,8 +689,9 @@ __device__ __forceinline__ bool slab_node(V3 o, V3 d, V3 id, const BvhNode& N, f
     return ok && tn <= tf;
 }
 
-// Enter instance i from outside along (O, D).  PRIMARY rebuilds the fragment inputs.
-template <bool PRIMARY, bool ST>
+// Primary ray: enter instance i from outside along (O, D), rebuilding the fragment inputs of
+// trace.frag (proxy point, clip test, camera ray through it in model space), and walk it.
+template <bool ST>
 __device__ bool enter_instance(const SceneGPU& S, const FrameGPU& F, uint32_t i, V3 O, V3 D, HitG& h, Ctr<ST>& ctr,
                                const Lds& mslot) {
     const InstanceGPU* I = &S.inst[i];
@@ -714,7 +715,7 @@ __device__ bool enter_instance(const SceneGPU& S, const FrameGPU& F, uint32_t i,
         }
     }
     V3 mdir;
-    if (PRIMARY) {
+    {
         float M[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k) M[k] = gld(&I->M[k]);
@@ -724,8 +725,6 @@ __device__ bool enter_instance(const SceneGPU& S, const FrameGPU& F, uint32_t i,
         const float4 q = mv4(F.AB, sp);
         const V3 dir = normalize3(V3{q.x, q.y, q.z});
         mdir = xf34(inv, dir, 0.0f);
-    } else {
-        mdir = d;
     }
     ctr.instance();
     const VolumeDesc V = S.vols[I->tex];
@@ -738,8 +737,7 @@ __device__ bool enter_instance(const SceneGPU& S, const FrameGPU& F, uint32_t i,
         v0[a] = (int)floorf(gmin(mpos[a], size[a] - 1.0f));
     }
     Dda r;
-    if (PRIMARY) dda_run<ST>(V, mpos, v0, mdir, r, ctr, mslot);
-    else dda2_run<ST>(V, mpos, v0, mdir, r, ctr, mslot);
+    dda_run<ST>(V, mpos, v0, mdir, r, ctr, mslot);
     if (!r.hit) return false;
     h.hit = true;
     h.inst = i;
@@ -797,11 +795,11 @@ __device__ __forceinline__ TreeLds stage_tree(const ReqGPU* reqs, const BatchGPU
     return t;
 }
 
-// Closest (any_hit: any) hit over all instances except `skip`, through the LBVH (nodes from
-// `tree` in LDS when given).
-template <bool PRIMARY, bool ST, bool DEEP>
-__device__ HitG query(const SceneGPU& S, const FrameGPU& F, V3 O, V3 D, uint32_t skip, bool any_hit, Ctr<ST>& ctr,
-                      const Lds& stk_lds, const Lds& mslot, TreeLds tree) {
+// Primary ray: closest hit over all instances through the LBVH (nodes from `tree` in LDS when
+// given).  Secondary rays have their own traversal (secondary()).
+template <bool ST, bool DEEP>
+__device__ HitG query(const SceneGPU& S, const FrameGPU& F, V3 O, V3 D, Ctr<ST>& ctr, const Lds& stk_lds,
+                      const Lds& mslot, TreeLds tree) {
     HitG best;
     best.hit = false;
     best.inst = VT_EMPTY;
@@ -820,13 +818,8 @@ __device__ HitG query(const SceneGPU& S, const FrameGPU& F, V3 O, V3 D, uint32_t
         if (!slab_node(O, D, inv_d, N, &tn, &tf) || tf < 0.0f) continue;
         if (best.hit && tn > prune_bound(best.tn)) continue;
         if (N.right == VT_EMPTY) {
-            const uint32_t i = N.left;
-            if (i == skip) continue;
             HitG h;
-            if (enter_instance<PRIMARY, ST>(S, F, i, O, D, h, ctr, mslot) && better(h, best)) {
-                best = h;
-                if (any_hit) return best;
-            }
+            if (enter_instance<ST>(S, F, N.left, O, D, h, ctr, mslot) && better(h, best)) best = h;
         } else {
             stk.push(N.right);
             stk.push(N.left);
@@ -1235,7 +1228,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         }
         const V3 D = camera_ray(F, (float)px + jx, (float)py + jy);
         ctr.ray();
-        HitG h = query<true, ST, DEEP>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, VT_EMPTY, false, ctr, stk, mslot, tree);
+        HitG h = query<ST, DEEP>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, ctr, stk, mslot, tree);
         float4 L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float beta[3] = {1.0f, 1.0f, 1.0f};
         uint8_t shadowed = 0;
@@ -1474,8 +1467,7 @@ __global__ __launch_bounds__(kBlock) void k_render0(const FrameGPU F, const Scen
             }
             const V3 D = camera_ray(F, (float)px + jx, (float)py + jy);
             ctr.ray();
-            HitG h = query<true, ST, DEEP>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, VT_EMPTY, false, ctr, stk, mslot,
-                                           TreeLds{});
+            HitG h = query<ST, DEEP>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, ctr, stk, mslot, TreeLds{});
             if (h.hit) {
                 ctr.hit();
                 uint8_t pal = 0;
