@@ -232,6 +232,10 @@ __device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST
     float sx = w.sx, sy = w.sy, sz = w.sz;
     const float dx = w.dx, dy = w.dy, dz = w.dz, zx = w.zx, zy = w.zy, zz = w.zz;
     const int stx = w.stx, sty = w.sty, stz = w.stz;
+    // exit voxel of a brick along a stepping axis: min(b8 + c, l) (c = 8, l = n when stepping
+    // up; c = -1, l = INT_MAX when stepping down)
+    const int cx = stx > 0 ? 8 : -1, cy = sty > 0 ? 8 : -1, cz = stz > 0 ? 8 : -1;
+    const int lx = stx > 0 ? nx : INT_MAX, ly = sty > 0 ? ny : INT_MAX, lz_ = stz > 0 ? nz : INT_MAX;
     uint32_t steps = 0, entry = VT_EMPTY, li = 0;
     float t_last = 0.0f;
     uint32_t lm = 0;
@@ -265,10 +269,11 @@ __device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST
            (uint32_t)vz < (uint32_t)nz) {
         // entering a brick: the current voxel is inside the volume
         const int bx8 = vx & ~7, by8 = vy & ~7, bz8 = vz & ~7;
-        // the voxel through which each stepping axis leaves this brick (or the volume)
-        const int ox = stx > 0 ? min(bx8 + 8, nx) : (stx < 0 ? bx8 - 1 : INT_MIN);
-        const int oy = sty > 0 ? min(by8 + 8, ny) : (sty < 0 ? by8 - 1 : INT_MIN);
-        const int oz = stz > 0 ? min(bz8 + 8, nz) : (stz < 0 ? bz8 - 1 : INT_MIN);
+        // the voxel through which each stepping axis leaves this brick (or the volume); in the
+        // FAST walk every axis steps
+        const int ox = FAST ? min(bx8 + cx, lx) : (stx > 0 ? min(bx8 + 8, nx) : (stx < 0 ? bx8 - 1 : INT_MIN));
+        const int oy = FAST ? min(by8 + cy, ly) : (sty > 0 ? min(by8 + 8, ny) : (sty < 0 ? by8 - 1 : INT_MIN));
+        const int oz = FAST ? min(bz8 + cz, lz_) : (stz > 0 ? min(bz8 + 8, nz) : (stz < 0 ? bz8 - 1 : INT_MIN));
         entry = gld(V.brick_map + (uint32_t)(bx8 >> 3) + bdx * ((uint32_t)(by8 >> 3) + bdy * (uint32_t)(bz8 >> 3)));
         ctr.brick(!vt_brick_empty(entry));
         if (vt_brick_empty(entry)) {
@@ -278,9 +283,9 @@ __device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST
                 // before max_steps.  rx/ry/rz count the steps each axis still needs to leave
                 // this brick; the loop stops before the step that leaves it, which is taken
                 // below with its bookkeeping.  "s <= min3(sx, sy, sz)" is the step test.
-                uint32_t rx = (uint32_t)(stx > 0 ? ox - vx : vx - ox);
-                uint32_t ry = (uint32_t)(sty > 0 ? oy - vy : vy - oy);
-                uint32_t rz = (uint32_t)(stz > 0 ? oz - vz : vz - oz);
+                uint32_t rx = (uint32_t)__mul24(stx, ox - vx);
+                uint32_t ry = (uint32_t)__mul24(sty, oy - vy);
+                uint32_t rz = (uint32_t)__mul24(stz, oz - vz);
                 float t;
                 for (;;) {
                     t = fminf(fminf(sx, sy), sz);
@@ -318,9 +323,9 @@ __device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST
             // tested before the step's side update, so when the loop ends (hit, or the step that
             // leaves the brick) the sides are those before the last step, which gives its axes
             // and side exactly as the step test does.
-            uint32_t rx = (uint32_t)(stx > 0 ? ox - vx : vx - ox);
-            uint32_t ry = (uint32_t)(sty > 0 ? oy - vy : vy - oy);
-            uint32_t rz = (uint32_t)(stz > 0 ? oz - vz : vz - oz);
+            uint32_t rx = (uint32_t)__mul24(stx, ox - vx);
+            uint32_t ry = (uint32_t)__mul24(sty, oy - vy);
+            uint32_t rz = (uint32_t)__mul24(stz, oz - vz);
             const int bit0 = (ox - bx8) + 8 * (oy - by8), az = oz - bz8;
             int lz = vz - bz8;
             uint64_t slice = mask_slice(mslot, lz);
