@@ -277,3 +277,30 @@ def test_split_frames_equal_one_stream(spp, chunks, gbuffer):
     assert r.stats() == st1
     if gbuffer:
         assert np.array_equal(r.gbuffer().view(np.uint32), gb1.view(np.uint32))
+
+
+def test_full_size_benchmark_frame():
+    """The benchmark workload at its full size (16 x 256^3 instances, 1920x1080, 4 spp, <= 4
+    bounces) on the benchmark's path: the four-chunk frame equals the one-stream frame bit for
+    bit, and bands of rows of it match the oracle (hits bit-exact, radiance within 1e-5)."""
+    from vtrace_amd import renderer as R
+    sc = scenes.config2()
+    _say("case: full-size benchmark frame 1920x1080 4 spp")
+    r = R.reset_renderer()
+    R.load_scene(r, sc)
+    r.lib.vt_set_frame_split(1)
+    r.render(sc.P, sc.C)
+    h1, rad1 = r.hits().copy(), r.radiance().copy()
+    r.lib.vt_set_frame_split(4)
+    r.render(sc.P, sc.C)
+    assert np.array_equal(r.hits(), h1)
+    assert np.array_equal(r.radiance().view(np.uint32), rad1.view(np.uint32))
+    assert (h1["inst"] != 0xFFFFFFFF).mean() > 0.2
+    osc = O.OracleScene(sc, use_bvh=True)
+    for y0 in (0, 200, 360, 540, 720, 1064):
+        o_hits, o_rad, _ = osc.render(rows=(y0, y0 + 16), threads=16)
+        assert_hits_equal(h1[y0:y0 + 16], o_hits[y0:y0 + 16], f"rows {y0}")
+        mx, _ = radiance_error(rad1[y0:y0 + 16], o_rad[y0:y0 + 16])
+        _say(f"rows {y0}-{y0 + 15}: radiance rel err max={mx:.3e}")
+        assert mx <= REL_TOL
+    osc.close()
