@@ -304,3 +304,20 @@ def test_full_size_benchmark_frame():
         _say(f"rows {y0}-{y0 + 15}: radiance rel err max={mx:.3e}")
         assert mx <= REL_TOL
     osc.close()
+
+
+def test_full_size_sparse_volume_frame():
+    """The sparse 1024^3 workload at its full size (5 % fill, 3840x2160, 1 spp, 2 bounces; a
+    128^3 brick map with empty-cube radii up to 31): bands of rows against the oracle."""
+    sc = scenes.config3()
+    _say("case: full-size 1024^3 3840x2160 frame")
+    g_hits, g_rad, _ = gpu_render(sc)
+    assert (g_hits["inst"] != 0xFFFFFFFF).mean() > 0.05
+    osc = O.OracleScene(sc, use_bvh=True)
+    for y0 in (600, 1080, 1500, 2000):
+        o_hits, o_rad, _ = osc.render(rows=(y0, y0 + 8), threads=16)
+        assert_hits_equal(g_hits[y0:y0 + 8], o_hits[y0:y0 + 8], f"rows {y0}")
+        mx, _ = radiance_error(g_rad[y0:y0 + 8], o_rad[y0:y0 + 8])
+        _say(f"rows {y0}-{y0 + 7}: radiance rel err max={mx:.3e}")
+        assert mx <= REL_TOL
+    osc.close()
