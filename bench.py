@@ -45,6 +45,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -60,12 +61,50 @@ def algorithmic_bytes(st: dict) -> int:
     return 32 * st["nodes"] + 64 * st["instances"] + 4 * st["bricks"] + 64 * st["bricks_nonempty"] + 5 * st["hits"]
 
 
+def host_cpus() -> dict:
+    """CPU model, nproc, the CPUs this process may run on and its cgroup CPU quota."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    usable = allowed if quota is None else max(1, min(allowed, int(quota)))
+    return {"model": model, "nproc": os.cpu_count() or 1, "affinity": allowed, "cgroup_quota": quota,
+            "usable": usable}
+
+
+def native_oracle() -> tuple[str, str]:
+    """Builds the oracle with -O3 -march=native on this host (the CPU baseline's build); falls
+    back to the -O2 checker build if the compiler is unavailable."""
+    odir = os.path.join(ROOT, "oracle")
+    try:
+        subprocess.run(["make", "-s", "-B", "-C", odir, "native"], check=True, timeout=300, capture_output=True)
+        return os.path.join(odir, "_build", "liboracle_native.so"), "-O3 -march=native"
+    except (OSError, subprocess.SubprocessError):
+        return os.path.join(odir, "_build", "liboracle.so"), "-O2 (native build failed)"
+
+
 def cpu_baseline(scene, target_s: float = 15.0) -> dict:
-    """Times the oracle on whole frames (or every k-th row when a frame would exceed target_s)."""
+    """Times the oracle on whole frames (or every k-th row when a frame would exceed target_s),
+    on every CPU this process may use."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi as O  # the oracle is the CPU baseline's timed code, never the product
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    path, flags = native_oracle()
+    O.select_library(path)
+    cpus = host_cpus()
+    threads = cpus["usable"]
     osc = O.OracleScene(scene, use_bvh=True)  # volumes / LBVH prepared outside the timed sample
     H = scene.params.height
     probe = list(range(32, H, 64))
@@ -89,8 +128,11 @@ def cpu_baseline(scene, target_s: float = 15.0) -> dict:
     osc.close()
     p = scene.params
     return {"value": round(rays / elapsed / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C++ restatement of the same path, -O2, {threads} host threads over rows), {what} of "
-                      f"the {p.width}x{p.height} {p.spp}-spp workload: {rays} rays in {elapsed:.1f} s"}
+            "cpu_model": cpus["model"], "nproc": cpus["nproc"], "affinity_cpus": cpus["affinity"],
+            "cgroup_cpu_quota": cpus["cgroup_quota"],
+            "sample": f"oracle (C++ restatement of the same path, {flags}, {threads} host threads over rows = "
+                      f"every CPU this process may use), {what} of the {p.width}x{p.height} {p.spp}-spp "
+                      f"workload: {rays} rays in {elapsed:.1f} s"}
 
 
 def calibrate_copy(nbytes: int = 256 << 20):
@@ -104,6 +146,61 @@ def calibrate_copy(nbytes: int = 256 << 20):
     print(json.dumps({"calibration_copy_bytes": nbytes}), flush=True)
 
 
+def _kernel_key(name: str):
+    """'void (anonymous namespace)::k_secondary<false, false, true>(...)' -> ('k_secondary', False);
+    the flag is True for the counting instantiations (first template argument true)."""
+    import re
+    m = re.search(r"\b(k_\w+|__amd_rocclr_\w+)(<([^>]*)>)?", name)
+    if not m:
+        return None, False
+    args = (m.group(3) or "").split(",")
+    return m.group(1), args[0].strip() == "true"
+
+
+def summarize_counters(dirs) -> dict:
+    """Mean per dispatch of every counter of every kernel in rocprofv3 --pmc outputs (CSV),
+    counting-frame instantiations left out."""
+    import csv
+    import glob
+    sums, disp = {}, {}
+    for d in dirs:
+        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            for row in csv.DictReader(open(path)):
+                k, counting = _kernel_key(row.get("Kernel_Name", ""))
+                if k is None or counting:
+                    continue
+                c = row["Counter_Name"]
+                sums.setdefault(k, {}).setdefault(c, 0.0)
+                sums[k][c] += float(row["Counter_Value"])
+                disp.setdefault(k, {}).setdefault(c, set()).add(row.get("Dispatch_Id") or row.get("Correlation_Id"))
+    return {k: {c: v / max(1, len(disp[k][c])) for c, v in cs.items()} for k, cs in sums.items()}
+
+
+def summarize_profiles(args) -> None:
+    """Writes profiles/traffic.json (FETCH_SIZE / WRITE_SIZE per launch, scaled by a plain copy
+    of known size) and profiles/pmc.json (SQ / TCC / TCP counters per launch) from rocprofv3
+    output directories."""
+    cmd = " ".join(sys.argv)
+    if args.fetch and args.write:
+        fetch, write = summarize_counters([args.fetch]), summarize_counters([args.write])
+        cf, cw = summarize_counters([args.cal_fetch]), summarize_counters([args.cal_write])
+        copy = "__amd_rocclr_copyBuffer"
+        fk = cf[copy]["FETCH_SIZE"] * 1024.0 / (256 << 20)
+        wk = cw[copy]["WRITE_SIZE"] * 1024.0 / (256 << 20)
+        kernels = {}
+        for k in sorted(set(fetch) & set(write)):
+            kernels[k] = {"fetch_bytes_per_launch": int(fetch[k]["FETCH_SIZE"] * 1024.0 / fk),
+                          "write_bytes_per_launch": int(write[k]["WRITE_SIZE"] * 1024.0 / wk)}
+        out = {"workload": args.workload, "command": cmd,
+               "calibration": {"copy_bytes": 256 << 20, "fetch_factor": round(fk, 4), "write_factor": round(wk, 4),
+                               "copy_kernel": copy},
+               "kernels": kernels}
+        json.dump(out, open(os.path.join(ROOT, "profiles", "traffic.json"), "w"), indent=1)
+    if args.pmc:
+        out = {"workload": args.workload, "command": cmd, "kernels": summarize_counters(args.pmc)}
+        json.dump(out, open(os.path.join(ROOT, "profiles", "pmc.json"), "w"), indent=1)
+
+
 def load_traffic(kernel: str):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -114,6 +211,41 @@ def load_traffic(kernel: str):
         return int(k["fetch_bytes_per_launch"] + k["write_bytes_per_launch"])
     except (KeyError, ValueError, TypeError):
         return None
+
+
+def load_pmc(kernel: str, workload: str):
+    """Lane utilisation, VALU busy and L2 hit rate of a kernel from profiles/pmc.json (None if
+    the profile is missing or was taken on another workload)."""
+    path = os.path.join(ROOT, "profiles", "pmc.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        t = json.load(open(path))
+        if t.get("workload") != workload:
+            return None
+        c = t["kernels"][kernel]
+        out = {}
+        if c.get("SQ_ACTIVE_INST_VALU"):
+            out["lane_utilisation"] = round(c["SQ_THREAD_CYCLES_VALU"] / (64.0 * c["SQ_ACTIVE_INST_VALU"]), 4)
+        if "VALUBusy" in c:
+            out["valu_busy_pct"] = round(c["VALUBusy"], 1)
+        if c.get("TCC_HIT_sum") is not None and c.get("TCC_MISS_sum") is not None:
+            out["l2_hit"] = round(c["TCC_HIT_sum"] / max(1.0, c["TCC_HIT_sum"] + c["TCC_MISS_sum"]), 4)
+        return out or None
+    except (KeyError, ValueError, TypeError, ZeroDivisionError):
+        return None
+
+
+def limiter(hbm_frac_measured, pmc) -> str:
+    """What binds the kernel, from its measured HBM bandwidth and lane utilisation."""
+    if hbm_frac_measured is not None and hbm_frac_measured >= 0.5:
+        return "hbm"
+    if pmc and pmc.get("lane_utilisation") is not None and pmc["lane_utilisation"] < 0.5 and \
+            (hbm_frac_measured is None or hbm_frac_measured < 0.2):
+        return "valu-divergence"
+    if pmc and pmc.get("valu_busy_pct", 0) >= 70:
+        return "valu"
+    return "latency"
 
 
 def main():
@@ -132,7 +264,18 @@ def main():
     ap.add_argument("--streaming", type=int, default=1, help="1: streamed terrain measurement, 0: skip it")
     ap.add_argument("--single-stream", action="store_true",
                     help="render every frame on one stream (kernel traces then match the stage table)")
+    ap.add_argument("--summarize-profiles", action="store_true",
+                    help="write profiles/traffic.json and profiles/pmc.json from rocprofv3 output directories")
+    ap.add_argument("--workload", default="c2_16x256cube_1920x1080_4spp_4b")
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--cal-fetch")
+    ap.add_argument("--cal-write")
+    ap.add_argument("--pmc", nargs="*")
     args = ap.parse_args()
+    if args.summarize_profiles:
+        summarize_profiles(args)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the renderer has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
@@ -207,6 +350,9 @@ def main():
     frame_bytes = sum(e["bytes"] for e in per_kind.values())
     mrays = st["rays"] / (wall_ms * 1e-3) / 1e6
     kname = KERNEL_NAMES[dom]
+    traffic = load_traffic(kname)
+    hbm_measured = traffic / (launch_ms * 1e-3) / 1e9 if traffic else None
+    pmc = load_pmc(kname, sc.name)
     out = {
         "metric": "Mrays/s @1080p 4spp; p50/p99 render latency; time to relMSE 1e-3 vs plain",
         "value": round(mrays, 2),
@@ -224,8 +370,11 @@ def main():
                    "width": sc.params.width, "height": sc.params.height, "spp": sc.params.spp,
                    "max_bounces": sc.params.max_bounces, "rays_per_frame": st["rays"],
                    "primary_hit_fraction": round(hit_frac, 4), "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kname),
+        "roofline": {"bound": limiter(hbm_measured / HBM_PEAK_GBS if hbm_measured else None, pmc),
+                     "roof": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "hbm_gbs_measured": round(hbm_measured, 1) if hbm_measured else None,
+                     "pmc": pmc,
                      "kernel": kname, "launches_per_frame": d["launches"], "launch_ms": round(launch_ms, 4),
                      "algorithmic_bytes_per_launch": int(bytes_launch),
                      "frame_algorithmic_bytes": int(frame_bytes),

@@ -35,6 +35,11 @@
 //                           axis-aligned closed forms, mirror symmetry, max_steps cap
 //   orc_render (path)       pinned: white-furnace / single-plane closed forms, FP32-vs-FP64
 //                           geometry replay
+//   cosine_dir              pinned (orc_cosine_dir_samples): n.w follows P(cos <= c) = c^2
+//                           (KS test), mean cos = 2/3 within its CI, uniform azimuth, unit
+//                           length; the same checks reject uniform-hemisphere sampling
+//   orc_srgb_encode         pinned: encode(decode(k)) == k for all 256 codes, monotone,
+//                           equals round(255 * published sRGB OETF) away from code boundaries
 
 #include <algorithm>
 #include <cmath>
@@ -962,30 +967,9 @@ void surface_of(const Scene& S, const Hit& h, Surface& sf) {
     sf.rgba = h.d.rgba;
 }
 
-// Optional per-query work log (diagnostics for the cost distribution of secondary rays).
-struct WorkLog {
-    bool on = false;
-    std::vector<uint64_t> rec;  // per query: kind, steps, bricks, nodes, instances, hit
-} g_log;
-
 // secondary query from a surface: continue in the origin instance, then the rest of the scene
-Hit secondary_impl(const Scene& S, uint32_t origin, const Surface& sf, uint8_t origin_face, const float w[3],
-                   bool any_hit, bool use_bvh, Counters& C);
 Hit secondary(const Scene& S, uint32_t origin, const Surface& sf, uint8_t origin_face, const float w[3], bool any_hit,
               bool use_bvh, Counters& C) {
-    if (!g_log.on || !C.st) return secondary_impl(S, origin, sf, origin_face, w, any_hit, use_bvh, C);
-    orc_stats before = *C.st;
-    Hit h = secondary_impl(S, origin, sf, origin_face, w, any_hit, use_bvh, C);
-    g_log.rec.push_back(any_hit ? 1 : 2);
-    g_log.rec.push_back(C.st->steps - before.steps);
-    g_log.rec.push_back(C.st->bricks - before.bricks);
-    g_log.rec.push_back(C.st->nodes - before.nodes);
-    g_log.rec.push_back(C.st->instances - before.instances);
-    g_log.rec.push_back(h.hit ? 1 : 0);
-    return h;
-}
-Hit secondary_impl(const Scene& S, uint32_t origin, const Surface& sf, uint8_t origin_face, const float w[3],
-                   bool any_hit, bool use_bvh, Counters& C) {
     if (C.st) C.st->rays++;
     Hit h;
     if (continue_in_instance(S, origin, sf.p, sf.vstart, w, h, C)) {
@@ -1047,9 +1031,6 @@ void write_hit(const Hit& h, orc_hit& out) {
 extern "C" {
 
 uint32_t orc_pcg_hash(uint32_t v) { return pcg_hash(v); }
-void orc_worklog_enable(int on) { g_log.on = on != 0; g_log.rec.clear(); }
-uint64_t orc_worklog_size(void) { return g_log.rec.size() / 6; }
-void orc_worklog_fetch(uint64_t* out) { std::memcpy(out, g_log.rec.data(), g_log.rec.size() * 8); }
 uint32_t orc_rng_seed(uint32_t seed, uint32_t frame, uint32_t pixel, uint32_t sample) {
     return rng_seed(seed, frame, pixel, sample);
 }
@@ -1063,6 +1044,16 @@ uint32_t orc_srgb_encode(float lin) {
     return k;
 }
 uint32_t orc_nearest_texel(int v, uint32_t dim) { return nearest_texel(v, (float)dim, dim); }
+
+// Bounce sampler test hook: n directions about the unit normal nrm, sample i drawn by
+// cosine_dir from the stream a path of pixel i at sample 0 would start with (rng_seed(seed,
+// 0, i, 0)).  out: 3 floats per sample.
+void orc_cosine_dir_samples(uint32_t seed, const float* nrm, uint32_t n, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        Rng rng{rng_seed(seed, 0u, i, 0u)};
+        cosine_dir(rng, nrm, out + 3 * (size_t)i);
+    }
+}
 
 // DDA test hooks: run the reference DDA (level 0) or the two-level secondary DDA (level 1)
 // on one volume from position mpos / start voxel v0 along mdir.  out: hit, x, y, z, steps, mask.

@@ -40,6 +40,15 @@ assert HIT_DTYPE.itemsize == 16
 _lib = None
 
 
+def select_library(path: str) -> None:
+    """Load the oracle from `path` (another build of oracle.cpp, e.g. the -O3 -march=native
+    build the CPU baseline times) instead of the checker build.  Must precede the first lib()."""
+    global ORACLE_LIB
+    if _lib is not None and ORACLE_LIB != path:
+        raise RuntimeError("the oracle library is already loaded from " + ORACLE_LIB)
+    ORACLE_LIB = path
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -60,6 +69,8 @@ def lib():
         L.orc_srgb_encode.argtypes = [C.c_float]
         L.orc_nearest_texel.restype = C.c_uint32
         L.orc_nearest_texel.argtypes = [C.c_int, C.c_uint32]
+        L.orc_cosine_dir_samples.restype = None
+        L.orc_cosine_dir_samples.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]
         L.orc_instance_prep.restype = None
         L.orc_instance_prep.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p]
@@ -242,3 +253,11 @@ def dda(volume, level, mpos, v0, mdir):
     tq = np.zeros(4, dtype=np.float32)
     lib().orc_dda_test(vols, level, mp.ctypes.data, vv.ctypes.data, md.ctypes.data, out.ctypes.data, tq.ctypes.data)
     return bool(out[0]), (int(out[1]), int(out[2]), int(out[3])), int(out[4]), int(out[5]), float(tq[0]), tq[1:].copy()
+
+
+def cosine_dir_samples(seed: int, normal, n: int) -> np.ndarray:
+    """n bounce directions about a unit normal from the oracle's cosine sampler (n x 3 float32)."""
+    nrm = np.ascontiguousarray(normal, dtype=np.float32)
+    out = np.empty((n, 3), dtype=np.float32)
+    lib().orc_cosine_dir_samples(seed, nrm.ctypes.data, n, out.ctypes.data)
+    return out

@@ -58,3 +58,41 @@ def test_scene_library_exports_every_declared_symbol():
     assert "vt_vox_model" in names and "vt_gen_volume" in names
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
+
+
+_BEFORE_ENTRY = r"""
+import ctypes, sys
+sys.path.insert(0, sys.argv[1])
+from vtrace_amd import _native, abi
+lib = _native.render_lib()
+P = (ctypes.c_float * 16)(*([1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0]))
+info = abi.RenderTickInfo(ctypes.cast(P, ctypes.c_void_p), ctypes.cast(P, ctypes.c_void_p))
+w, h = ctypes.c_int32(0), ctypes.c_int32(0)
+data = (ctypes.c_uint8 * 4)(255, 255, 255, 255)
+pal = (ctypes.c_uint32 * 256)()
+buf = (ctypes.c_uint8 * 64)()
+out = [lib.render_tick(ctypes.byref(w), ctypes.byref(h), ctypes.byref(info)),
+       lib.end_update_instances(1),
+       0 if lib.start_update_instances(1) else -1,
+       lib.add_texture(data, 1, 1, 1),
+       lib.vt_add_volume_indexed(data, 1, 1, 1, pal),
+       lib.vt_add_textures(data, 1, 1, 1, 1, None),
+       lib.vt_read_hits(buf, 64),
+       lib.vt_refit_instances(1)]
+lib.cleanup()   # before entry: nothing to free, returns
+print(" ".join(str(x) for x in out))
+print(lib.vt_last_error().decode())
+"""
+
+
+def test_calls_before_entry_fail_with_the_reference_conventions():
+    """Every export used before `entry` fails the way the reference reports failure (-1, or
+    NULL for the mapped instance buffer: lib/memory.c:235-267, lib/entry.c:150-248) without
+    touching the device; run in a fresh process so no earlier `entry` is in effect."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, "-c", _BEFORE_ENTRY, ROOT], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    codes = [int(x) for x in r.stdout.split("\n")[0].split()]
+    assert codes == [-1] * 8, codes
+    assert "not initialised" in r.stdout

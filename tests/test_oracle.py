@@ -440,3 +440,101 @@ def test_lbvh_and_brute_force_frames_agree():
     assert np.array_equal(r0, r1)
     assert s1["nodes"] > 0 and s0["nodes"] == 0
     assert s0["rays"] == s1["rays"] and s0["hits"] == s1["hits"]
+
+
+# ---- bounce sampling (cosine-weighted hemisphere) --------------------------------------------
+# The path tracer's bounce rule has no counterpart in the reference (trace.frag:79 has no
+# shading), so it is pinned by the law it must follow: for a cosine-weighted direction w about
+# n, c = n.w has density 2c on [0, 1], i.e. P(c <= x) = x^2 and E[c] = 2/3, and the azimuth
+# about n is uniform.  The same checks are run on two plausible wrong samplers to show they
+# reject them.
+_NORMALS = [(0.0, 0.0, 1.0), (1.0, 0.0, 0.0), (0.0, -1.0, 0.0), (-1.0, 0.0, 0.0), (1.0, 2.0, -3.0)]
+
+
+def _frame(n):
+    n = np.asarray(n, dtype=np.float64)
+    n = n / np.linalg.norm(n)
+    a = np.array([1.0, 0.0, 0.0]) if abs(n[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+    t = np.cross(n, a)
+    t /= np.linalg.norm(t)
+    return n, t, np.cross(n, t)
+
+
+def _cosine_law_stats(w, n):
+    from scipy import stats
+    n, t, b = _frame(n)
+    w = w.astype(np.float64)
+    c = w @ n
+    phi = np.arctan2(w @ b, w @ t)
+    ks_cos = stats.kstest(np.clip(c, 0.0, 1.0), lambda x: np.clip(x, 0.0, 1.0) ** 2)
+    ks_phi = stats.kstest(phi, stats.uniform(loc=-np.pi, scale=2 * np.pi).cdf)
+    z_mean = (c.mean() - 2.0 / 3.0) / np.sqrt((0.5 - 4.0 / 9.0) / len(c))   # Var[c] = 1/2 - 4/9
+    return c, ks_cos.pvalue, ks_phi.pvalue, z_mean
+
+
+@pytest.mark.parametrize("normal", _NORMALS)
+def test_cosine_dir_follows_the_cosine_law(normal):
+    n = np.asarray(normal, dtype=np.float32)
+    n = (n / np.float32(np.linalg.norm(n))).astype(np.float32)
+    w = O.cosine_dir_samples(11, n, 200_000)
+    assert np.all(np.isfinite(w))
+    assert np.allclose(np.linalg.norm(w.astype(np.float64), axis=1), 1.0, atol=1e-6)
+    c, p_cos, p_phi, z = _cosine_law_stats(w, n)
+    assert np.all(c > 0.0)                   # upper hemisphere only
+    assert p_cos > 1e-3, f"n.w is not distributed as P(c <= x) = x^2 (KS p = {p_cos:.2e})"
+    assert p_phi > 1e-3, f"azimuth is not uniform (KS p = {p_phi:.2e})"
+    assert abs(z) < 5.0, f"mean cos {c.mean():.5f} is {z:.1f} sigma from 2/3"
+
+
+def test_cosine_law_checks_reject_wrong_samplers():
+    """The checks above have the power to catch a wrong sampler: uniform-hemisphere directions
+    (mean cos 1/2) and an un-normalised n + (point in the unit ball) both fail them."""
+    rng = np.random.default_rng(5)
+    n = np.array([0.0, 0.0, 1.0])
+    v = rng.normal(size=(200_000, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    uni = np.where((v @ n)[:, None] < 0, -v, v)              # uniform over the hemisphere
+    _, p_cos, _, z = _cosine_law_stats(uni, n)
+    assert p_cos < 1e-10 and abs(z) > 50
+    ball = v * rng.random((len(v), 1)) ** (1.0 / 3.0)        # uniform in the unit ball
+    a = n + ball
+    a /= np.linalg.norm(a, axis=1, keepdims=True)
+    _, p_cos, _, z = _cosine_law_stats(a, n)
+    assert p_cos < 1e-10 and abs(z) > 10
+
+
+def test_cosine_dir_is_deterministic_per_stream():
+    n = np.array([0.0, 1.0, 0.0], dtype=np.float32)
+    a = O.cosine_dir_samples(3, n, 1000)
+    b = O.cosine_dir_samples(3, n, 1000)
+    c = O.cosine_dir_samples(4, n, 1000)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+
+
+# ---- sRGB encode (B8G8R8A8_SRGB store, swapchain.c:88) ----------------------------------------
+def _srgb_oetf(x):
+    """The published sRGB encoding (IEC 61966-2-1): linear -> non-linear in [0, 1]."""
+    x = np.clip(np.asarray(x, dtype=np.float64), 0.0, 1.0)
+    return np.where(x <= 0.0031308, 12.92 * x, 1.055 * np.power(x, 1.0 / 2.4) - 0.055)
+
+
+def test_srgb_encode_round_trips_every_code():
+    L = O.lib()
+    dec = np.array([L.orc_srgb_decode(k) for k in range(256)], dtype=np.float32)
+    assert dec[0] == 0.0 and dec[255] == 1.0 and np.all(np.diff(dec) > 0)
+    for k in range(256):
+        assert L.orc_srgb_encode(float(dec[k])) == k
+
+
+def test_srgb_encode_matches_the_published_oetf():
+    L = O.lib()
+    rng = np.random.default_rng(2)
+    xs = np.concatenate([rng.random(60_000), rng.random(20_000) * 0.01, np.linspace(0.0, 1.0, 4001)]).astype(np.float32)
+    want_f = 255.0 * _srgb_oetf(xs)
+    clear = np.abs(want_f - np.floor(want_f) - 0.5) > 1e-4       # not on a rounding boundary
+    want = np.floor(want_f + 0.5).astype(np.int64)
+    got = np.array([L.orc_srgb_encode(float(x)) for x in xs], dtype=np.int64)
+    assert clear.mean() > 0.99
+    assert np.array_equal(got[clear], want[clear])
+    assert np.all(np.diff(got[np.argsort(xs, kind="stable")]) >= 0)   # monotone
+    assert L.orc_srgb_encode(-0.25) == 0 and L.orc_srgb_encode(3.0) == 255

@@ -38,6 +38,11 @@ struct State {
     uint32_t inst_capacity = 0;
     uint32_t inst_count = 0;
     uint32_t bvh_depth = 0;
+    // the depth of the last build is copied to pinned memory behind the build; the host reads
+    // it when the next frame picks its traversal-stack variant (no wait in the update itself)
+    uint32_t* depth_host = nullptr;
+    hipEvent_t depth_ready = nullptr;
+    bool depth_pending = false;
     hipEvent_t inst_copied = nullptr;
     std::vector<float> inst_prev;     // records of the last build
     size_t inst_prev_vols = 0;        // textures known at the last build (validity of the records)
@@ -424,6 +429,11 @@ namespace {
 bool build_frame(const float* P, const float* C, FrameGPU& F) { return vt_build_frame(P, C, g.params, F); }
 
 SceneGPU scene_gpu() {
+    if (g.depth_pending) {  // the last build's depth (its kernels ran before this frame's)
+        hipEventSynchronize(g.depth_ready);
+        g.bvh_depth = *g.depth_host;
+        g.depth_pending = false;
+    }
     SceneGPU S;
     S.inst = g.bvh.inst;
     S.nodes = g.bvh.nodes;
@@ -583,6 +593,8 @@ uint64_t entry(void) {
         (e = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreate(&g.ev0)) != hipSuccess || (e = hipEventCreate(&g.ev1)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&g.inst_copied, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&g.depth_ready, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipHostMalloc(&g.depth_host, sizeof(uint32_t), hipHostMallocDefault)) != hipSuccess ||
         (e = hipMalloc(&g.vol_table, (size_t)kMaxTextures * sizeof(VolumeDesc))) != hipSuccess ||
         (e = hipMalloc(&g.stats, (size_t)VT_MAX_STAGES * VT_STAT_COUNT * 8)) != hipSuccess ||
         (e = hipMalloc(&g.srgb_dec, 256 * 4)) != hipSuccess || (e = hipMalloc(&g.srgb_thr, 255 * 4)) != hipSuccess) {
@@ -680,7 +692,8 @@ void cleanup(void) {
     hipFree(g.acc);
     hipFree(g.gacc);
     for (uint32_t k = 0; k < kMaxChunks; ++k) hipFree(g.xgsamp[k]);
-    hipEventDestroy(g.ev0); hipEventDestroy(g.ev1); hipEventDestroy(g.inst_copied);
+    hipEventDestroy(g.ev0); hipEventDestroy(g.ev1); hipEventDestroy(g.inst_copied); hipEventDestroy(g.depth_ready);
+    hipHostFree(g.depth_host);
     hipStreamDestroy(g.stream);
     vt_trace_set_tables(nullptr, nullptr);
     vt_trace_release();
@@ -740,15 +753,15 @@ int32_t end_update_instances(uint32_t instance_count) {
         set_err("lbvh build", hipGetLastError());
         return -1;
     }
-    // the tree depth selects the traversal-stack variant (LDS stack or a deep private stack)
-    uint32_t depth = 0;
-    e = hipMemcpyAsync(&depth, g.bvh.depth, 4, hipMemcpyDeviceToHost, g.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+    // the tree depth selects the traversal-stack variant (LDS stack or a deep private stack);
+    // it is read by the next frame (scene_gpu), so the update does not wait for the build
+    e = hipMemcpyAsync(g.depth_host, g.bvh.depth, 4, hipMemcpyDeviceToHost, g.stream);
+    if (e == hipSuccess) e = hipEventRecord(g.depth_ready, g.stream);
     if (e != hipSuccess) {
         set_err("lbvh depth", e);
         return -1;
     }
-    g.bvh_depth = depth;
+    g.depth_pending = true;
     g.inst_count = instance_count;
     g.inst_prev.assign(g.inst_host, g.inst_host + nrec);
     g.inst_prev_vols = g.vols.size();

@@ -16,9 +16,10 @@
 // counterpart in the reference and follow this project's specification.
 //
 // Execution (modes 1 and 2) is a wavefront: k_primary traces every pixel sample and shades
-// its first hit; then per bounce depth k_shadow traces the sun rays and k_bounce the
-// diffuse rays of the surviving paths, whose hits are appended (wave-aggregated atomics)
-// to a compacted queue for the next depth; k_accumulate sums the samples of each pixel.
+// its first hit; then one k_secondary launch per bounce depth traces, side by side, the sun
+// rays and the diffuse bounce rays of the surfaces queued at that depth, appending the
+// bounce hits (wave-aggregated atomics) to a compacted queue for the next depth;
+// k_accumulate sums the samples of each pixel.
 // Every kernel is persistent: each wave fetches 64 consecutive work items at a time from
 // an atomic counter, so terminated paths never occupy lanes.  Each path's arithmetic is
 // independent of queue order, so results are deterministic.
