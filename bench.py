@@ -337,11 +337,14 @@ def main():
             b += 16 * pixels
         if kind == "accumulate":
             b += 16 * pixels
-        e = per_kind.setdefault(kind, {"launches": 0, "bytes": 0, "ms": 0.0, "rays": 0})
+        e = per_kind.setdefault(kind, {"launches": 0, "bytes": 0, "ms": 0.0, "rays": 0, "counters": {}, "launch_ms": []})
         e["launches"] += 1
         e["bytes"] += b
         e["ms"] += ms
         e["rays"] += s["rays"]
+        e["launch_ms"].append(round(ms, 4))
+        for c, v in s.items():
+            e["counters"][c] = e["counters"].get(c, 0) + v
     dom = max(per_kind, key=lambda k: per_kind[k]["ms"])
     d = per_kind[dom]
     launch_ms = d["ms"] / d["launches"]
@@ -381,7 +384,8 @@ def main():
                      "frame_achieved_gbs": round(frame_bytes / (kernel_ms * 1e-3) / 1e9, 1),
                      "bytes_per_ray": round(frame_bytes / max(1, st["rays"]), 1)},
         "stages": {k: {"launches": v["launches"], "ms": round(v["ms"], 4), "rays": v["rays"],
-                       "GBs": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in per_kind.items()},
+                       "GBs": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1), "launch_ms": v["launch_ms"],
+                       "counters": v["counters"]} for k, v in per_kind.items()},
         "latency_ms": {"p50": round(p50, 4), "p99": round(p99, 4)},
         "kernel_ms_per_frame": round(kernel_ms, 4),
         "single_stream_ms_per_frame": round(single_wall_ms, 4),
