@@ -163,3 +163,27 @@ def test_blend_weight_is_the_relmse_minimiser():
         assert DN.relmse(a * den + (1 - a) * noisy, ref) <= min(errs) + 1e-12
     ref = rng.uniform(0.0, 1.0, size=(8, 8, 4))
     assert DN.blend_weight(ref + 0.1, ref, ref) == 1.0
+
+
+def test_tile_with_halo_reproduces_the_whole_frame_forward():
+    """A tile aligned to the 16-pixel pooling grid with a 144-pixel halo around a block gives
+    the block exactly as the whole-frame forward does (the GPU 1080p check relies on this)."""
+    enc, dec = (32, 32, 48, 64, 96), (64, 48, 32, 32)
+    layers = U.layer_table(enc, dec)
+    n = layers[-1][3] + layers[-1][1]
+    rng = np.random.default_rng(4)
+    p = rng.normal(0.0, 0.05, n)
+    net = U.UNetRef(layers, enc, dec)
+    x = rng.normal(size=(1, 352, 352, 16))
+    y, _ = net.forward(p, x)
+    cy = cx = 160
+    halo = 144
+    t0, t1 = (cy - halo) // 16 * 16, -(-(cy + 32 + halo) // 16) * 16
+    yt, _ = net.forward(p, x[:, t0:t1, t0:t1])
+    a = yt[0, cy - t0:cy - t0 + 32, cx - t0:cx - t0 + 32]
+    b = y[0, cy:cy + 32, cx:cx + 32]
+    assert np.allclose(a, b, rtol=0, atol=1e-10 * np.abs(b).max())
+    # a halo well inside the dependence radius does not
+    t0s = (cy - 48) // 16 * 16
+    ys, _ = net.forward(p, x[:, t0s:t0s + 128, t0s:t0s + 128])
+    assert not np.allclose(ys[0, cy - t0s:cy - t0s + 32, cx - t0s:cx - t0s + 32], b, rtol=0, atol=1e-6)
