@@ -119,6 +119,23 @@ uint64_t vt_lbvh_builds(void);
 int32_t vt_add_volume_indexed(const uint8_t* indices, uint32_t width, uint32_t height, uint32_t depth,
                               const uint32_t* palette);
 
+/* Allocation-free frame: no counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Renders one frame of the resident scene with the current parameters
+ * (vt_set_params) and the matrices of `info`, like render_tick, into the caller's DEVICE
+ * buffers: hits_dev (width*height*16 B, the vt_read_hits layout) and radiance_dev
+ * (width*height*16 B, RGBA32F).  The frame's wavefront state, queues, counters and request
+ * table live in a 128 MiB device array of the library (one per device, held until the
+ * process ends), so the call makes no hipMalloc / hipFree / pinned-host allocation.  Frames
+ * whose state exceeds it return -1 (path-traced frames need about 245 B per pixel sample,
+ * padded to whole 64-pixel tiles: 640x480 at 1 spp fits, 1920x1080 at 1 spp does not).
+ * Results equal render_tick's bit for bit; no G-buffer, no counters; one stream.  The call
+ * waits for its frame.  Every call on a device shares that device's array: calls that overlap
+ * on one device (other threads bypassing the library lock, other libraries' copies) race.
+ * 0, or -1 (vt_last_error says why). */
+int32_t vt_render_into(const render_tick_info* info, void* hits_dev, uint64_t hits_bytes, float* radiance_dev,
+                       uint64_t radiance_bytes);
+
 /* Read-back of the last frame (synchronises).  bytes must be >= width*height*16 (hits and
  * radiance) or width*height*4 (B8G8R8A8 sRGB, the reference's swapchain format). */
 int32_t vt_read_hits(void* dst, uint64_t bytes);

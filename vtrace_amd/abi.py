@@ -20,7 +20,8 @@ EXTENSION_EXPORTS = ("vt_set_params", "vt_get_params", "vt_add_volume_indexed", 
                      "vt_serve_status", "vt_serve_shutdown", "vt_enable_gbuffer", "vt_read_gbuffer",
                      "vt_dn_init", "vt_dn_layers", "vt_dn_param_count", "vt_dn_set_params", "vt_dn_get_params",
                      "vt_dn_grad", "vt_dn_features", "vt_dn_add_frame", "vt_dn_clear_frames", "vt_dn_train",
-                     "vt_dn_denoise", "vt_dn_denoise_frame", "vt_dn_set_blend", "vt_dn_shutdown")
+                     "vt_dn_denoise", "vt_dn_denoise_frame", "vt_dn_set_blend", "vt_dn_shutdown",
+                     "vt_render_into")
 
 
 class RenderTickInfo(C.Structure):
@@ -74,6 +75,8 @@ def declare(lib: C.CDLL) -> None:
     lib.vt_get_params.argtypes = [C.POINTER(VtRenderParams)]
     lib.vt_add_volume_indexed.restype = C.c_int32
     lib.vt_add_volume_indexed.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
+    lib.vt_render_into.restype = C.c_int32
+    lib.vt_render_into.argtypes = [C.POINTER(RenderTickInfo), C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64]
     lib.vt_enable_gbuffer.restype = C.c_int32
     lib.vt_enable_gbuffer.argtypes = [C.c_uint32]
     for name in ("vt_read_hits", "vt_read_radiance", "vt_read_bgra8", "vt_read_gbuffer"):

@@ -21,10 +21,22 @@
 #include "../../include/vtrace.h"
 #include "vt_internal.h"
 
+// Scratch of vt_render_into (the allocation-free frame entry point): the wavefront state,
+// queues, counters and request table of one frame, in a file-scope device array that the
+// code object brings with it (each device holds its own copy until the process ends).  Host
+// code never passes `scratch` itself to a kernel (that name is a host-side stand-in): entry()
+// resolves its device address with hipGetSymbolAddress and every use goes through that.
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace {
 
 constexpr uint32_t kMaxTextures = 65536;  // lib/common.h:35
 constexpr uint32_t kMaxChunks = 4;         // sample chunks (streams) per frame
+
+// host staging of vt_render_into's request table (instead of the pinned ring the other
+// entry points allocate on first use); the call waits for its frame before returning
+alignas(64) unsigned char noalloc_tables[sizeof(ReqGPU) * VT_MAX_BATCH + sizeof(BatchGPU)];
 
 struct State {
     bool inited = false;
@@ -91,6 +103,7 @@ struct State {
     float* srgb_thr = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool have_frame = false;
+    void* scratch_dev = nullptr;   // device address of `scratch` (hipGetSymbolAddress at entry)
     FrameGPU last_frame{};
     float last_ms = 0.0f;
     uint64_t frames = 0;
@@ -622,6 +635,11 @@ uint64_t entry(void) {
             vt_trace_set_persistent_blocks(g.cus * 4u);
         }
     }
+    if ((e = hipGetSymbolAddress(&g.scratch_dev, HIP_SYMBOL(scratch))) != hipSuccess) {
+        set_err("init: scratch address", e);
+        g.scratch_dev = nullptr;
+        return ((uint64_t)(uint32_t)e << 32) | 1u;
+    }
     default_params(g.params);
     g.inited = true;
     if (grow_instances(0) != 0) return ((uint64_t)hipErrorOutOfMemory << 32) | 1u;
@@ -655,6 +673,49 @@ int32_t render_tick(int32_t* window_width, int32_t* window_height, const render_
     ++g.frames;
     if (window_width) *window_width = (int32_t)g.params.width;
     if (window_height) *window_height = (int32_t)g.params.height;
+    return 0;
+}
+
+int32_t vt_render_into(const render_tick_info* info, void* hits_dev, uint64_t hits_bytes, float* radiance_dev,
+                       uint64_t radiance_bytes) {
+    Guard G;
+    if (!g.inited || !g.scratch_dev || !info || !info->perspective || !info->camera || !hits_dev || !radiance_dev) {
+        set_err("vt_render_into: not initialised, null info or null output");
+        return -1;
+    }
+    if (vt_trace_table_bytes() != sizeof(noalloc_tables)) {
+        set_err("vt_render_into: request table size mismatch");
+        return -1;
+    }
+    FrameGPU F;
+    if (!build_frame(static_cast<const float*>(info->perspective), static_cast<const float*>(info->camera), F)) {
+        set_err("vt_render_into: singular projection or camera");
+        return -1;
+    }
+    F.stats = 0;
+    const size_t npix = (size_t)F.width * F.height;
+    if (hits_bytes < npix * sizeof(HitRecord) || radiance_bytes < npix * sizeof(float4)) {
+        set_err("vt_render_into: output buffer too small");
+        return -1;
+    }
+    const size_t paths = F.mode == 0 ? 1 : padded_paths(F.width, F.height, F.spp);
+    if (paths > 0xFFFFFFFFull / 2 || vt_wave_bytes((uint32_t)paths) > kNoAllocScratchBytes) {
+        set_err("vt_render_into: the frame's wavefront state exceeds the 128 MiB scratch");
+        return -1;
+    }
+    WaveBuffers wb;
+    vt_wave_bind(wb, g.scratch_dev, (uint32_t)paths);
+    wb.gsamp = nullptr;
+    uint8_t kinds[VT_MAX_STAGES];
+    uint32_t ns = 0;
+    int rc = vt_trace_launch(F, scene_gpu(), static_cast<HitRecord*>(hits_dev), reinterpret_cast<float4*>(radiance_dev),
+                             nullptr, wb, nullptr, nullptr, kinds, &ns, g.stream, noalloc_tables);
+    // the table staging and the scratch are reused by the next call: the frame completes here
+    const hipError_t e = hipStreamSynchronize(g.stream);
+    if (rc != 0 || e != hipSuccess) {
+        set_err("vt_render_into: trace launch failed", e != hipSuccess ? e : hipGetLastError());
+        return -1;
+    }
     return 0;
 }
 

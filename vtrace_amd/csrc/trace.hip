@@ -1682,15 +1682,22 @@ static int launch_batch(const BatchGPU& Bh, const WaveBuffers& wb, unsigned long
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// Uploads the request table (stream-ordered) and launches the batched pipeline.
+// Uploads the request table (stream-ordered) and launches the batched pipeline.  host_tables
+// (optional, kTableBytes): stage the table there instead of the pinned ring, which then is
+// never allocated; the caller keeps it unchanged until the stream has passed the launch.
 int vt_trace_batch(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, unsigned long long* stats, hipEvent_t* ev,
-                   uint8_t* kinds, uint32_t* nstages, hipStream_t s, hipEvent_t acc_after, uint32_t blocks) {
+                   uint8_t* kinds, uint32_t* nstages, hipStream_t s, hipEvent_t acc_after, uint32_t blocks,
+                   void* host_tables) {
     if (!g_srgb_dec || n == 0 || n > VT_MAX_BATCH || !wb.tables) return -1;
-    if (ring_init() != 0) return -1;
-    const int slot = g_ring.next;
-    g_ring.next = (g_ring.next + 1) % TableRing::kSlots;
-    hipEventSynchronize(g_ring.ev[slot]);
-    char* h = static_cast<char*>(g_ring.host[slot]);
+    int slot = -1;
+    char* h = static_cast<char*>(host_tables);
+    if (!h) {
+        if (ring_init() != 0) return -1;
+        slot = g_ring.next;
+        g_ring.next = (g_ring.next + 1) % TableRing::kSlots;
+        hipEventSynchronize(g_ring.ev[slot]);
+        h = static_cast<char*>(g_ring.host[slot]);
+    }
     ReqGPU* rq = reinterpret_cast<ReqGPU*>(h);
     BatchGPU* Bh = reinterpret_cast<BatchGPU*>(h + sizeof(ReqGPU) * VT_MAX_BATCH);
     std::memcpy(rq, reqs, sizeof(ReqGPU) * n);
@@ -1705,7 +1712,7 @@ int vt_trace_batch(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, unsign
     }
     st = st && stats != nullptr;
     if (hipMemcpyAsync(wb.tables, h, kTableBytes, hipMemcpyHostToDevice, s) != hipSuccess) return -1;
-    hipEventRecord(g_ring.ev[slot], s);
+    if (slot >= 0) hipEventRecord(g_ring.ev[slot], s);
     if (deep) return st ? launch_batch<true, true>(*Bh, wb, stats, ev, kinds, nstages, s, acc_after, true, blocks)
                         : launch_batch<false, true>(*Bh, wb, stats, ev, kinds, nstages, s, acc_after, true, blocks);
     return st ? launch_batch<true, false>(*Bh, wb, stats, ev, kinds, nstages, s, acc_after, true, blocks)
@@ -1769,9 +1776,11 @@ static int launch_render0(const FrameGPU& F, const SceneGPU& S, HitRecord* hits,
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+size_t vt_trace_table_bytes() { return kTableBytes; }
+
 int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, float4* gbuf,
                     const WaveBuffers& wb, unsigned long long* stats, hipEvent_t* ev, uint8_t* kinds, uint32_t* nstages,
-                    hipStream_t s) {
+                    hipStream_t s, void* host_tables) {
     if (!g_srgb_dec || F.mode > 2) return -1;
     if (F.mode == 0) {
         const bool st = F.stats != 0 && stats != nullptr;
@@ -1794,7 +1803,7 @@ int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float
     R.s1 = F.spp ? F.spp : 1u;
     R.final_chunk = 1;
     R.tile_w = 8;
-    return vt_trace_batch(&R, 1, wb, stats, ev, kinds, nstages, s);
+    return vt_trace_batch(&R, 1, wb, stats, ev, kinds, nstages, s, nullptr, 0, host_tables);
 }
 
 int vt_encode_srgb8(const float4* rad, uint32_t* bgra, uint32_t n, hipStream_t s) {

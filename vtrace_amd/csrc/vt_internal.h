@@ -74,9 +74,13 @@ void vt_trace_set_persistent_blocks(uint32_t blocks);
 // Launches one frame.  stats: VT_MAX_STAGES x VT_STAT_COUNT counters (stage k at k*VT_STAT_COUNT).
 // ev (optional): 2 events per stage recorded around each launch.  kinds/nstages describe the
 // stages launched.
+// host_tables (optional, vt_trace_table_bytes()): host staging of the request table instead of
+// the library's pinned ring (then nothing is allocated); unchanged until the launch has run.
 int vt_trace_launch(const FrameGPU& F, const SceneGPU& S, HitRecord* hits, float4* rad, float4* gbuf,
                     const WaveBuffers& wb,
-                    unsigned long long* stats, hipEvent_t* ev, uint8_t* kinds, uint32_t* nstages, hipStream_t s);
+                    unsigned long long* stats, hipEvent_t* ev, uint8_t* kinds, uint32_t* nstages, hipStream_t s,
+                    void* host_tables = nullptr);
+size_t vt_trace_table_bytes();
 int vt_encode_srgb8(const float4* rad, uint32_t* bgra, uint32_t n, hipStream_t s);
 // Batched path tracing (modes 1/2): n requests, their sample chunks traced in one pipeline.
 bool vt_batch_layout(const ReqGPU* reqs, uint32_t n, uint32_t npaths, BatchGPU& B);
@@ -85,7 +89,7 @@ bool vt_batch_layout(const ReqGPU* reqs, uint32_t n, uint32_t npaths, BatchGPU& 
 // blocks (optional): persistent workgroups per launch (0 = the library's default, 4 per CU)
 int vt_trace_batch(const ReqGPU* reqs, uint32_t n, const WaveBuffers& wb, unsigned long long* stats, hipEvent_t* ev,
                    uint8_t* kinds, uint32_t* nstages, hipStream_t s, hipEvent_t acc_after = nullptr,
-                   uint32_t blocks = 0);
+                   uint32_t blocks = 0, void* host_tables = nullptr);
 // with_acc = false: the graph stops before the accumulation (vt_trace_accumulate launches it)
 int vt_trace_capture(uint32_t max_bounces, bool deep, const WaveBuffers& wb, hipStream_t s, hipGraphExec_t* out,
                      bool with_acc = true, uint32_t blocks = 0);
