@@ -30,7 +30,9 @@
 //   orc_lbvh_build          pinned: sort == std::stable_sort, topology invariants, AABB
 //                           containment, traversal result == brute force
 //   orc_brick               pinned: dense <-> brick round trip, popcount == dense count,
-//                           empty radius == brute-force L-inf distance (tests/test_oracle.py)
+//                           empty radius == brute-force L-inf distance, octant cube sides ==
+//                           brute-force largest anchored empty cube (tests/test_oracle.py)
+//   orc_brick_blocks        pinned: 2^3-block bits == brute-force block occupancy
 //   orc_render (primary)    pinned: brute-force nearest-voxel test on tiny volumes,
 //                           axis-aligned closed forms, mirror symmetry, max_steps cap
 //   orc_render (path)       pinned: white-furnace / single-plane closed forms, FP32-vs-FP64
@@ -101,8 +103,10 @@ typedef struct orc_stats {
 namespace {
 
 constexpr uint32_t kMiss = 0xFFFFFFFFu;
-// brick map: non-empty bricks hold their id, empty ones kEmptyBrick | radius
-constexpr uint32_t kEmptyBrick = 0xFFFFFF00u;
+// brick map: non-empty bricks hold their id; empty ones kEmptyBrick | radius << 24 |
+// (cube side - 1) << 3o for the 8 octants o
+constexpr uint32_t kEmptyBrick = 0xE0000000u;
+constexpr uint32_t kCubeCap = 8;
 constexpr float kInf = std::numeric_limits<float>::infinity();
 
 // ---- GLSL built-ins with the semantics this restatement fixes ---------------------
@@ -232,6 +236,8 @@ struct Vol {
     std::vector<uint16_t> remap[3];  // texel index for DDA voxel v per axis
     std::vector<uint8_t> brick_nonempty;
     std::vector<uint8_t> brick_radius;  // empty bricks: radius of the cube of empty bricks around them
+    std::vector<uint8_t> brick_cube[8]; // empty bricks: side of the empty cube anchored there, per octant
+    std::vector<uint64_t> brick_blocks; // non-empty bricks: bit bx + 4by + 16bz per occupied 2^3 block
     uint32_t bdim[3];
     const SrgbTables* s;
 
@@ -299,6 +305,34 @@ void brick_radius(Vol& V) {
         V.brick_radius[b] = V.brick_nonempty[b] ? 0 : (uint8_t)(std::min<uint32_t>(d[b], kRadiusCap + 1) - 1);
 }
 
+// Per empty brick and octant o (bit a of o set: towards -axis a), the side K_o, in bricks, of
+// the largest cube of empty bricks that has the brick as its corner and extends towards o,
+// capped at kCubeCap (bricks outside the volume count as empty): K = 1 + the smallest side
+// of the 7 neighbours towards o (outside: kCubeCap), a sweep from the far corner of the octant.
+void brick_cubes(Vol& V) {
+    const int bx = (int)V.bdim[0], by = (int)V.bdim[1], bz = (int)V.bdim[2];
+    for (int o = 0; o < 8; ++o) {
+        const int sx = (o & 1) ? -1 : 1, sy = (o & 2) ? -1 : 1, sz = (o & 4) ? -1 : 1;
+        std::vector<uint8_t>& K = V.brick_cube[o];
+        K.assign((size_t)bx * by * bz, 0);
+        auto at = [&](int x, int y, int z) -> int {
+            if (x < 0 || y < 0 || z < 0 || x >= bx || y >= by || z >= bz) return (int)kCubeCap;
+            return K[x + (size_t)bx * (y + (size_t)by * z)];
+        };
+        for (int zi = 0; zi < bz; ++zi)
+            for (int yi = 0; yi < by; ++yi)
+                for (int xi = 0; xi < bx; ++xi) {
+                    const int x = sx > 0 ? bx - 1 - xi : xi, y = sy > 0 ? by - 1 - yi : yi, z = sz > 0 ? bz - 1 - zi : zi;
+                    const size_t b = x + (size_t)bx * (y + (size_t)by * z);
+                    if (V.brick_nonempty[b]) continue;
+                    int m = (int)kCubeCap;
+                    for (int nb = 1; nb < 8; ++nb)
+                        m = std::min(m, at(x + ((nb & 1) ? sx : 0), y + ((nb & 2) ? sy : 0), z + ((nb & 4) ? sz : 0)));
+                    K[b] = (uint8_t)std::min<int>((int)kCubeCap, 1 + m);
+                }
+    }
+}
+
 void init_vol(Vol& V, const orc_volume* v) {
     V.v = v;
     V.s = &srgb();
@@ -310,12 +344,17 @@ void init_vol(Vol& V, const orc_volume* v) {
         V.bdim[a] = (v->dim[a] + 7) / 8;
     }
     V.brick_nonempty.assign((size_t)V.bdim[0] * V.bdim[1] * V.bdim[2], 0);
+    V.brick_blocks.assign(V.brick_nonempty.size(), 0);
     for (uint32_t z = 0; z < V.dim[2]; ++z)
         for (uint32_t y = 0; y < V.dim[1]; ++y)
             for (uint32_t x = 0; x < V.dim[0]; ++x)
-                if (V.occupied_texel(V.texel_index(x, y, z)))
-                    V.brick_nonempty[(x >> 3) + V.bdim[0] * ((y >> 3) + V.bdim[1] * (z >> 3))] = 1;
+                if (V.occupied_texel(V.texel_index(x, y, z))) {
+                    const size_t b = (x >> 3) + V.bdim[0] * ((y >> 3) + V.bdim[1] * (z >> 3));
+                    V.brick_nonempty[b] = 1;
+                    V.brick_blocks[b] |= 1ull << (((x & 7) >> 1) + 4 * ((y & 7) >> 1) + 16 * ((z & 7) >> 1));
+                }
     brick_radius(V);
+    brick_cubes(V);
 }
 
 // world AABB padding: keeps the LBVH conservative with respect to the model-space slab
@@ -596,129 +635,111 @@ DdaResult dda(const Vol& V, const float mpos[3], const int v0[3], const float md
     return r;
 }
 
-// Two-level DDA for secondary rays (no reference counterpart; this project's rule).
-// An empty 8^3 brick of radius r (brick_radius) is crossed together with the cube of
-// (2r+1)^3 empty bricks around it in one step: the ray moves from its current position q to
-// the nearest exit plane of that cube (ties: every axis at that distance), the exit axes
-// step into the voxel beyond it and the other axes take floor(q) clamped to the cube.
-// Inside a non-empty brick a voxel walk from q (trace.frag's side/delta formulas, steps
-// with selects so zero direction components never produce NaN) tests occupancy until it
-// hits or leaves the brick; the position where it leaves is q + t * unit dir with the
-// exit axes snapped to the voxel boundary.  Steps count voxel steps and brick steps.
+// Secondary-ray walk (no reference counterpart; this project's rule).  From position q in voxel v
+// the ray crosses, one box per step, the largest box known to be empty that holds v: in an empty
+// brick the cube of radius r centred on it or the cube of side K_o anchored at it towards the
+// ray's octant o, whichever exit lies further along the ray (the centred one on ties); in a
+// non-empty brick the 4^3 octant holding v if all its 2^3 blocks are empty, else the 2^3 block
+// holding v if empty, else v itself unless v is occupied (the hit).  A step moves q to the
+// box's nearest exit plane along the ray (ties: every axis at that distance): exit axes step
+// into the voxel beyond the plane, the other axes take floor(q) clamped to the box.  Steps
+// count boxes crossed; a brick is counted when the walk enters it.
 DdaResult dda2(const Vol& V, const float p[3], const int v0[3], const float m[3], Counters& C) {
     DdaResult r{};
     const float len = std::sqrt((m[0] * m[0] + m[1] * m[1]) + m[2] * m[2]);
-    float sg[3], u[3], dl[3], rinv[3];
+    float u[3], rinv[3];
     int st[3];
     for (int a = 0; a < 3; ++a) {
-        sg[a] = gsign(m[a]);
-        st[a] = (int)sg[a];
+        st[a] = m[a] > 0.0f ? 1 : (m[a] < 0.0f ? -1 : 0);
         u[a] = m[a] / len;
-        dl[a] = std::fabs(len / m[a]);
         rinv[a] = len / m[a];
     }
+    const int oct = (st[0] < 0 ? 1 : 0) | (st[1] < 0 ? 2 : 0) | (st[2] < 0 ? 4 : 0);
     int v[3] = {v0[0], v0[1], v0[2]};
     float q[3] = {p[0], p[1], p[2]};
     const int n[3] = {(int)V.dim[0], (int)V.dim[1], (int)V.dim[2]};
     uint32_t steps = 0, mask = 0;
-    const uint32_t guard = 2 * (V.bdim[0] + V.bdim[1] + V.bdim[2]) + 8;
+    int64_t cur = -1;
+    const uint32_t guard = 2 * (V.dim[0] + V.dim[1] + V.dim[2]) + 64;
     for (uint32_t it = 0; it < guard; ++it) {
         if (!(v[0] >= 0 && v[1] >= 0 && v[2] >= 0 && v[0] < n[0] && v[1] < n[1] && v[2] < n[2])) break;
         const int b8[3] = {v[0] & ~7, v[1] & ~7, v[2] & ~7};
         const uint32_t bi = (uint32_t)(b8[0] >> 3) + V.bdim[0] * ((uint32_t)(b8[1] >> 3) + V.bdim[1] * (uint32_t)(b8[2] >> 3));
-        const bool nonempty = V.brick_nonempty[bi] != 0;
-        C.brick(V, v[0], v[1], v[2]);
-        if (!nonempty) {
-            // the cube of empty bricks of the brick's radius is crossed in one step
+        if ((int64_t)bi != cur) {
+            cur = bi;
+            C.brick(V, v[0], v[1], v[2]);
+        }
+        int lo[3], hi[3];
+        if (!V.brick_nonempty[bi]) {
             const int rad = 8 * (int)V.brick_radius[bi];
-            float t[3];
-            float tmin = kInf;
+            const int k8 = 8 * (int)V.brick_cube[oct][bi];
+            int alo[3], ahi[3];
+            float tc = kInf, ta = kInf;
             for (int a = 0; a < 3; ++a) {
-                if (st[a] == 0) {
-                    t[a] = kInf;
-                } else {
-                    const float plane = (float)(st[a] > 0 ? b8[a] + 8 + rad : b8[a] - rad);
-                    t[a] = (plane - q[a]) * rinv[a];
-                }
-                tmin = gmin(tmin, t[a]);
-            }
-            uint32_t mk = 0;
-            float nq[3];
-            int nv[3];
-            for (int a = 0; a < 3; ++a) {
-                const int lo = b8[a] - rad, hi = b8[a] + 8 + rad;
-                if (st[a] != 0 && t[a] <= tmin) {
-                    mk |= 1u << a;
-                    nq[a] = (float)(st[a] > 0 ? hi : lo);
-                    nv[a] = st[a] > 0 ? hi : lo - 1;
-                } else {
-                    const float x = q[a] + tmin * u[a];
-                    nq[a] = x;
-                    int f = (int)std::floor(x);
-                    f = f < lo ? lo : (f > hi - 1 ? hi - 1 : f);
-                    nv[a] = f;
+                lo[a] = b8[a] - rad;
+                hi[a] = b8[a] + 8 + rad;
+                alo[a] = st[a] < 0 ? b8[a] + 8 - k8 : b8[a];
+                ahi[a] = st[a] < 0 ? b8[a] + 8 : b8[a] + k8;
+                if (st[a] != 0) {
+                    tc = gmin(tc, ((float)(st[a] > 0 ? hi[a] : lo[a]) - q[a]) * rinv[a]);
+                    ta = gmin(ta, ((float)(st[a] > 0 ? ahi[a] : alo[a]) - q[a]) * rinv[a]);
                 }
             }
-            for (int a = 0; a < 3; ++a) { q[a] = nq[a]; v[a] = nv[a]; }
-            mask = mk;
-            ++steps;
-            continue;
-        }
-        // voxel walk inside the non-empty brick, from q
-        float side[3];
-        for (int a = 0; a < 3; ++a) side[a] = ((sg[a] * ((float)v[a] - q[a]) + sg[a] * 0.5f) + 0.5f) * dl[a];
-        float t_last = 0.0f;
-        bool left = false;
-        for (;;) {
-            uint32_t ti = V.texel_index(v[0], v[1], v[2]);
-            uint8_t pal;
-            uint32_t c = V.rgba(ti, &pal);
-            if ((c >> 24) != 0u) {
-                r.hit = true;
-                r.rgba = c;
-                r.pal = pal;
-                r.t_last = t_last;
-                for (int a = 0; a < 3; ++a) r.q[a] = q[a];
-                break;
-            }
-            const bool mx = side[0] <= gmin(side[1], side[2]);
-            const bool my = side[1] <= gmin(side[2], side[0]);
-            const bool mz = side[2] <= gmin(side[0], side[1]);
-            t_last = mx ? side[0] : (my ? side[1] : side[2]);
-            side[0] = mx ? side[0] + dl[0] : side[0];
-            side[1] = my ? side[1] + dl[1] : side[1];
-            side[2] = mz ? side[2] + dl[2] : side[2];
-            v[0] += mx ? st[0] : 0;
-            v[1] += my ? st[1] : 0;
-            v[2] += mz ? st[2] : 0;
-            mask = (mx ? 1u : 0u) | (my ? 2u : 0u) | (mz ? 4u : 0u);
-            ++steps;
-            if ((v[0] & ~7) != b8[0] || (v[1] & ~7) != b8[1] || (v[2] & ~7) != b8[2] || v[0] < 0 || v[1] < 0 ||
-                v[2] < 0 || v[0] >= n[0] || v[1] >= n[1] || v[2] >= n[2]) {
-                left = true;
-                break;
-            }
-        }
-        if (r.hit) break;
-        if (left) {
-            float nq[3];
-            for (int a = 0; a < 3; ++a) {
-                if (mask & (1u << a)) {
-                    nq[a] = (float)(st[a] > 0 ? v[a] : v[a] + 1);
-                } else {
-                    const float x = q[a] + t_last * u[a];
-                    const float lo = (float)v[a], hi = (float)(v[a] + 1);
-                    nq[a] = x < lo ? lo : (x > hi ? hi : x);
+            if (ta > tc)
+                for (int a = 0; a < 3; ++a) { lo[a] = alo[a]; hi[a] = ahi[a]; }
+        } else {
+            const uint64_t blk = V.brick_blocks[bi];
+            const int lx = v[0] - b8[0], ly = v[1] - b8[1], lz = v[2] - b8[2];
+            int sz;
+            if (((blk >> (2 * (lx >> 2) + 8 * (ly >> 2) + 32 * (lz >> 2))) & 0x330033ull) == 0ull) {
+                sz = 4;
+            } else if (((blk >> ((lx >> 1) + 4 * (ly >> 1) + 16 * (lz >> 1))) & 1ull) == 0ull) {
+                sz = 2;
+            } else {
+                uint8_t pal;
+                const uint32_t c = V.rgba(V.texel_index(v[0], v[1], v[2]), &pal);
+                if ((c >> 24) != 0u) {
+                    r.hit = true;
+                    r.rgba = c;
+                    r.pal = pal;
+                    break;
                 }
+                sz = 1;
             }
-            for (int a = 0; a < 3; ++a) q[a] = nq[a];
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = v[a] & -sz;
+                hi[a] = lo[a] + sz;
+            }
         }
+        float t[3];
+        float tmin = kInf;
+        for (int a = 0; a < 3; ++a) {
+            t[a] = st[a] != 0 ? ((float)(st[a] > 0 ? hi[a] : lo[a]) - q[a]) * rinv[a] : kInf;
+            tmin = gmin(tmin, t[a]);
+        }
+        uint32_t mk = 0;
+        for (int a = 0; a < 3; ++a) {
+            if (st[a] != 0 && t[a] <= tmin) {
+                mk |= 1u << a;
+                q[a] = (float)(st[a] > 0 ? hi[a] : lo[a]);
+                v[a] = st[a] > 0 ? hi[a] : lo[a] - 1;
+            } else {
+                const float x = q[a] + tmin * u[a];
+                q[a] = x;
+                int f = (int)std::floor(x);
+                f = f < lo[a] ? lo[a] : (f > hi[a] - 1 ? hi[a] - 1 : f);
+                v[a] = f;
+            }
+        }
+        mask = mk;
+        ++steps;
     }
     if (C.st) C.st->steps += steps;
     r.v[0] = v[0]; r.v[1] = v[1]; r.v[2] = v[2];
     r.steps = steps;
     r.mask = (r.hit && steps > 0) ? mask : 0;
-    if (!r.hit) r.t_last = 0.0f;
+    r.t_last = 0.0f;
+    for (int a = 0; a < 3; ++a) r.q[a] = q[a];
     return r;
 }
 
@@ -1104,8 +1125,8 @@ int orc_lbvh_build(uint32_t n, const float* records, uint32_t nvol, void* nodes_
 }
 
 // Dense -> 8^3 bricks in the product's layout: brick map (u32 per brick, x-fastest brick
-// order, 0xFFFFFF00 | radius = empty, others numbered in brick order), occupancy masks (64 B per
-// non-empty brick, bit x + 8y + 64z of the brick), payload (u8 index or RGBA8 per voxel,
+// order; empty = 0xE0000000 | radius << 24 | (octant cube side - 1) << 3o, others numbered in
+// brick order), occupancy masks (64 B per non-empty brick, bit x + 8y + 64z of the brick), payload (u8 index or RGBA8 per voxel,
 // voxel x + 8y + 64z).  Voxel v of the brick volume holds the texel the reference samples
 // for v (nearest_texel remap).  Returns the non-empty brick count.
 uint32_t orc_brick(const orc_volume* v, uint32_t* map_out, uint64_t* masks_out, uint8_t* payload_out) {
@@ -1117,7 +1138,11 @@ uint32_t orc_brick(const orc_volume* v, uint32_t* map_out, uint64_t* masks_out, 
     for (uint32_t b = 0; b < nb; ++b) {
         uint32_t bx = b % V.bdim[0], by = (b / V.bdim[0]) % V.bdim[1], bz = b / (V.bdim[0] * V.bdim[1]);
         if (!V.brick_nonempty[b]) {
-            if (map_out) map_out[b] = kEmptyBrick | V.brick_radius[b];
+            if (map_out) {
+                uint32_t e = kEmptyBrick | ((uint32_t)V.brick_radius[b] << 24);
+                for (int o = 0; o < 8; ++o) e |= (uint32_t)(V.brick_cube[o][b] - 1) << (3 * o);
+                map_out[b] = e;
+            }
             continue;
         }
         uint32_t id = next++;
@@ -1142,6 +1167,13 @@ uint32_t orc_brick(const orc_volume* v, uint32_t* map_out, uint64_t* masks_out, 
         if (masks_out) std::memcpy(masks_out + 8 * (size_t)id, m, 64);
     }
     return next;
+}
+
+// 2^3-block masks of every brick (0 for empty bricks), brick order.
+void orc_brick_blocks(const orc_volume* v, uint64_t* out) {
+    Vol V;
+    init_vol(V, v);
+    std::memcpy(out, V.brick_blocks.data(), V.brick_blocks.size() * 8);
 }
 
 // Persistent oracle scene: volumes prepared once (remaps, brick occupancy), instances, LBVH.

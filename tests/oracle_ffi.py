@@ -69,6 +69,8 @@ def lib():
         L.orc_srgb_encode.argtypes = [C.c_float]
         L.orc_nearest_texel.restype = C.c_uint32
         L.orc_nearest_texel.argtypes = [C.c_int, C.c_uint32]
+        L.orc_brick_blocks.restype = None
+        L.orc_brick_blocks.argtypes = [C.POINTER(OrcVolume), C.c_void_p]
         L.orc_cosine_dir_samples.restype = None
         L.orc_cosine_dir_samples.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]
         L.orc_instance_prep.restype = None
@@ -260,4 +262,14 @@ def cosine_dir_samples(seed: int, normal, n: int) -> np.ndarray:
     nrm = np.ascontiguousarray(normal, dtype=np.float32)
     out = np.empty((n, 3), dtype=np.float32)
     lib().orc_cosine_dir_samples(seed, nrm.ctypes.data, n, out.ctypes.data)
+    return out
+
+
+def brick_blocks(volume) -> np.ndarray:
+    """2^3-block masks of every brick of a volume (brick order, 0 for empty bricks)."""
+    vols, keep = _vol_structs([volume])
+    bd = [(d + 7) // 8 for d in volume.dims]
+    out = np.zeros(bd[0] * bd[1] * bd[2], dtype=np.uint64)
+    lib().orc_brick_blocks(vols, out.ctypes.data)
+    _ = keep
     return out

@@ -159,6 +159,7 @@ def test_lbvh_structure(n):
 
 
 # ---- bricking ---------------------------------------------------------------------------------
+EMPTY = 0xE0000000   # brick map entries at or above this are empty bricks
 @pytest.mark.parametrize("dims,fmt", [((32, 32, 32), 0), ((50, 40, 30), 0), ((19, 8, 11), 1)])
 def test_brick_roundtrip_and_popcount(dims, fmt):
     idx = scenes.gen_volume(dims, 31, 0.2)
@@ -176,7 +177,7 @@ def test_brick_roundtrip_and_popcount(dims, fmt):
     recon = np.zeros((bdz * 8, bdy * 8, bdx * 8), dtype=vol.data.dtype)
     for b, e in enumerate(bmap):
         bx, by, bz = b % bdx, (b // bdx) % bdy, b // (bdx * bdy)
-        if e >= 0xFFFFFF00:
+        if e >= EMPTY:
             continue
         m = masks[e]
         for z in range(8):
@@ -193,7 +194,7 @@ def test_brick_roundtrip_and_popcount(dims, fmt):
     full = np.zeros((bdz * 8, bdy * 8, bdx * 8), dtype=bool)
     full[:dz, :dy, :dx] = occ
     blocks = full.reshape(bdz, 8, bdy, 8, bdx, 8).any(axis=(1, 3, 5)).reshape(-1)
-    assert np.array_equal(bmap < 0xFFFFFF00, blocks)
+    assert np.array_equal(bmap < EMPTY, blocks)
     _ = bpv
 
 
@@ -217,12 +218,74 @@ def test_brick_radius_matches_brute_force(dims, fill, boxes):
     bmap, _, _ = O.brick(vol)
     bd = [(d + 7) // 8 for d in dims]
     m = bmap.reshape(bd[2], bd[1], bd[0])
-    nonempty = m < 0xFFFFFF00
-    rad = np.where(nonempty, 0, (m & 0xFF).astype(np.int64))
+    nonempty = m < EMPTY
+    rad = np.where(nonempty, 0, ((m >> 24) & 31).astype(np.int64))
     want = _radius_brute(nonempty)
     assert np.array_equal(rad, want)
     if fill == 0.03:
         assert rad.max() >= 2  # sparse volumes exercise cubes of more than one brick
+
+
+def _cube_brute(nonempty, octant, cap=8):
+    """Side of the largest cube of empty bricks with brick b as its corner, extending towards the
+    octant (bit a set: towards -axis a), capped; bricks outside the grid count as empty."""
+    bz, by, bx = nonempty.shape
+    s = [-1 if (octant >> a) & 1 else 1 for a in range(3)]   # x, y, z signs
+    out = np.zeros(nonempty.shape, dtype=np.int64)
+    for z, y, x in np.argwhere(~nonempty):
+        k = 0
+        while k < cap:
+            # the cube of side k + 1 is empty iff its bricks inside the grid are all empty
+            xs = sorted((x, x + s[0] * k)); ys = sorted((y, y + s[1] * k)); zs = sorted((z, z + s[2] * k))
+            sub = nonempty[max(zs[0], 0):min(zs[1], bz - 1) + 1, max(ys[0], 0):min(ys[1], by - 1) + 1,
+                           max(xs[0], 0):min(xs[1], bx - 1) + 1]
+            if sub.any():
+                break
+            k += 1
+        out[z, y, x] = k
+    return out
+
+
+@pytest.mark.parametrize("dims,fill,boxes", [((64, 64, 64), 0.03, None), ((96, 40, 72), 0.10, None),
+                                             ((48, 48, 48), 0.0, [[20, 21, 22, 21, 22, 23]]),
+                                             ((40, 24, 16), 0.0, None)])
+def test_brick_octant_cubes_match_brute_force(dims, fill, boxes):
+    idx = scenes.gen_volume(dims, 78, fill, boxes=boxes) if fill > 0 or boxes else np.zeros(dims[::-1], np.uint8)
+    vol = scenes.Volume(idx, scenes.gen_palette(78))
+    bmap, _, _ = O.brick(vol)
+    bd = [(d + 7) // 8 for d in dims]
+    m = bmap.reshape(bd[2], bd[1], bd[0]).astype(np.int64)
+    nonempty = m < EMPTY
+    seen = set()
+    for o in range(8):
+        got = np.where(nonempty, 0, ((m >> (3 * o)) & 7) + 1)
+        want = _cube_brute(nonempty, o)
+        assert np.array_equal(got, want), o
+        seen.update(np.unique(got).tolist())
+    if fill == 0.03:
+        assert max(seen) == 8 and len(seen) >= 4   # cubes of several sizes, up to the cap
+
+
+@pytest.mark.parametrize("dims,fill", [((32, 32, 32), 0.2), ((50, 40, 30), 0.35), ((19, 8, 11), 0.05)])
+def test_block_masks_match_brute_force(dims, fill):
+    idx = scenes.gen_volume(dims, 33, fill)
+    vol = scenes.Volume(idx, scenes.gen_palette(33))
+    blocks = O.brick_blocks(vol)
+    dx, dy, dz = dims
+    bd = [(d + 7) // 8 for d in dims]
+    occ = np.zeros((bd[2] * 8, bd[1] * 8, bd[0] * 8), dtype=bool)
+    rm = [np.array([O.lib().orc_nearest_texel(i, d) for i in range(d)]) for d in (dx, dy, dz)]
+    occ[:dz, :dy, :dx] = vol.data[np.ix_(rm[2], rm[1], rm[0])] != 0
+    blk = occ.reshape(bd[2], 4, 2, bd[1], 4, 2, bd[0], 4, 2).any(axis=(2, 5, 8))   # (bz, z4, by, y4, bx, x4)
+    for b, word in enumerate(blocks):
+        bx, by, bz = b % bd[0], (b // bd[0]) % bd[1], b // (bd[0] * bd[1])
+        want = 0
+        for z in range(4):
+            for y in range(4):
+                for x in range(4):
+                    if blk[bz, z, by, y, bx, x]:
+                        want |= 1 << (x + 4 * y + 16 * z)
+        assert int(word) == want, b
 
 
 # ---- DDA against brute force -----------------------------------------------------------------------
@@ -304,7 +367,7 @@ def test_secondary_dda_crosses_empty_cubes_like_brute_force():
         idx = (occ * rng.integers(1, 256, size=occ.shape)).astype(np.uint8)
         vol = scenes.Volume(idx, scenes.gen_palette(2))
         bmap, _, _ = O.brick(vol)
-        big += int(((bmap >= 0xFFFFFF00) & ((bmap & 0xFF) >= 2)).any())
+        big += int(((bmap >= EMPTY) & (((bmap >> 24) & 31) >= 2)).any())
         targets = np.argwhere(occ)
         for _ in range(40):
             p = rng.uniform(0.0, n, size=3)
@@ -538,3 +601,37 @@ def test_srgb_encode_matches_the_published_oetf():
     assert np.array_equal(got[clear], want[clear])
     assert np.all(np.diff(got[np.argsort(xs, kind="stable")]) >= 0)   # monotone
     assert L.orc_srgb_encode(-0.25) == 0 and L.orc_srgb_encode(3.0) == 255
+
+
+@pytest.mark.parametrize("n,fill", [(32, 0.1), (40, 0.3), (24, 0.45)])
+def test_secondary_walk_on_noise_volumes_matches_brute_force(n, fill):
+    """Noise volumes like the workloads' (bricks with empty 4^3 octants and 2^3 blocks next to
+    occupied ones): the secondary walk's first hit equals the nearest occupied voxel."""
+    rng = np.random.default_rng(int(fill * 100) + n)
+    idx = scenes.gen_volume((n, n, n), 5 + n, fill)
+    vol = scenes.Volume(idx, scenes.gen_palette(3))
+    occ = idx != 0
+    free = np.argwhere(~occ)
+    checked = hits = 0
+    for _ in range(400):
+        z, y, x = free[rng.integers(len(free))]
+        p = np.array([x, y, z], dtype=np.float64) + rng.uniform(0.0, 1.0, size=3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        v0 = np.floor(p).astype(int)
+        p32 = p.astype(np.float32)
+        if occ[tuple(np.floor(p32.astype(np.float64)).astype(int)[::-1])] or np.any(np.floor(p32) != v0):
+            continue
+        hit, v, steps, _, _, q = O.dda(vol, 1, p32, v0, d.astype(np.float32))
+        want, gap = _first_voxel_brute(occ, p32.astype(np.float64), d.astype(np.float32).astype(np.float64))
+        if gap < 1e-4:
+            continue
+        if want is None:
+            assert not hit
+        else:
+            assert hit and v == tuple(int(c) for c in want), (p, d, v, want)
+            # the walk ends on the hit voxel's boundary
+            assert np.all(np.asarray(q) >= np.asarray(v) - 1e-3) and np.all(np.asarray(q) <= np.asarray(v) + 1 + 1e-3)
+            hits += 1
+        checked += 1
+    assert checked > 300 and hits > 100

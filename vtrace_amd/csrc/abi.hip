@@ -391,6 +391,7 @@ int upload_volume(const void* data, size_t bytes, uint32_t fmt, const uint32_t d
         d.masks = v.masks;
         d.payload = v.payload;
         d.palette = v.palette;
+        d.blocks = v.blocks;
     }
     const int32_t id = (int32_t)g.vols.size();
     hipMemcpyAsync(&g.vol_table[id], descs.data(), sizeof(VolumeDesc) * count, hipMemcpyHostToDevice, g.stream);
@@ -736,7 +737,7 @@ void cleanup(void) {
     vt_dn_release();
     for (auto& v : g.vols) {
         if (!v.owner) continue;  // shares the pools of the first volume of its batch
-        hipFree(v.brick_map); hipFree(v.masks); hipFree(v.payload); hipFree(v.palette);
+        hipFree(v.brick_map); hipFree(v.masks); hipFree(v.payload); hipFree(v.palette); hipFree(v.blocks);
     }
     g.vols.clear();
     vt_lbvh_free(g.bvh);
@@ -1069,6 +1070,7 @@ int32_t vt_read_volume(int32_t id, void* dst, uint64_t bytes) {
     d.masks = v.masks;
     d.payload = v.payload;
     d.palette = v.palette;
+    d.blocks = v.blocks;
     int rc = vt_debrick(d, tmp, g.stream);
     if (rc == 0) hipMemcpyAsync(dst, tmp, need, hipMemcpyDeviceToHost, g.stream);
     if (hipStreamSynchronize(g.stream) != hipSuccess) rc = -1;

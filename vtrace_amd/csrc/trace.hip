@@ -24,9 +24,10 @@
 // an atomic counter, so terminated paths never occupy lanes.  Each path's arithmetic is
 // independent of queue order, so results are deterministic.
 //
-// DDA: empty 8^3 bricks (brick map entry VT_EMPTY_BRICK | radius) are crossed by a tight loop without
-// memory access; in non-empty bricks the 64 B occupancy mask is staged once into the
-// thread's LDS slot and tested per step from LDS.  The LBVH stack lives in LDS as well.
+// DDA: primary rays walk the reference's voxel DDA (empty 8^3 bricks crossed by a tight loop
+// without memory access); secondary rays cross boxes known to be empty (brick cubes, 4^3
+// octants, 2^3 blocks) one per step.  In non-empty bricks the 64 B occupancy mask is staged
+// once into the thread's LDS slot and tested from LDS.  The LBVH stack lives in LDS as well.
 //
 // All FP32 expressions are compiled with -ffp-contract=off and evaluated in the order
 // written (correctly rounded division and square root), which keeps the integer outputs
@@ -427,205 +428,134 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
     r.hit = w.hit;
 }
 
-// Two-level DDA for secondary rays (no reference counterpart; this project's rule).
-// An empty 8^3 brick of radius r is crossed together with the cube of (2r+1)^3 empty bricks
-// around it in one step: the ray moves from its current position q to the nearest exit plane
-// of that cube (ties: every axis at that distance), the exit axes step into the voxel beyond
-// it and the other axes take floor(q) clamped to the cube.
-// Inside a non-empty brick a voxel walk from q (trace.frag's side/delta formulas, steps
-// with selects so zero direction components never produce NaN) tests occupancy from the
-// LDS-staged mask until it hits or leaves the brick; the position where it leaves is
-// q + t * unit dir with the exit axes snapped to the voxel boundary.
-// FAST: every delta is finite, so no side distance is NaN and "side <= gmin(a, b)" is
-// "side <= min3(sx, sy, sz)" (otherwise the selects keep GLSL's min semantics for NaN sides).
-template <bool ST, bool FAST>
-__device__ __forceinline__ void dda2_walk(const VolumeDesc& V, const float p[3], const int v0[3], V3 m, Dda& r,
-                                          Ctr<ST>& ctr, const Lds& mslot) {
-    const float len = sqrtf((m.x * m.x + m.y * m.y) + m.z * m.z);
-    const float mm[3] = {m.x, m.y, m.z};
-    float sg[3], u[3], dl[3], rinv[3];
-    int st[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        sg[a] = gsign(mm[a]);
-        st[a] = (int)sg[a];
-        u[a] = mm[a] / len;
-        dl[a] = fabsf(len / mm[a]);
-        rinv[a] = len / mm[a];
-    }
-    int v[3] = {v0[0], v0[1], v0[2]};
-    float q[3] = {p[0], p[1], p[2]};
-    const int n[3] = {(int)V.dim[0], (int)V.dim[1], (int)V.dim[2]};
-    const uint32_t bdx = V.bdim[0], bdy = V.bdim[1];
-    uint32_t steps = 0, mask = 0, entry = VT_EMPTY, li = 0;
-    float t_last = 0.0f;
-    bool hit = false;
-    const uint32_t guard = 2u * (V.bdim[0] + V.bdim[1] + V.bdim[2]) + 8u;
-    for (uint32_t it = 0; it < guard; ++it) {
-        if (!((uint32_t)v[0] < (uint32_t)n[0] && (uint32_t)v[1] < (uint32_t)n[1] && (uint32_t)v[2] < (uint32_t)n[2])) break;
-        int b8[3] = {v[0] & ~7, v[1] & ~7, v[2] & ~7};
-        entry = gld(V.brick_map + (uint32_t)(b8[0] >> 3) + bdx * ((uint32_t)(b8[1] >> 3) + bdy * (uint32_t)(b8[2] >> 3)));
-        ctr.brick(!vt_brick_empty(entry));
-        // Consecutive empty cubes are crossed in this inner loop (one jump per iteration), so
-        // lanes crossing empty space keep jumping together before the non-empty bricks are
-        // handled.
-        bool outside = false;
-        while (vt_brick_empty(entry)) {
-            // the cube of empty bricks of this brick's radius is crossed in one step
-            const int rad = 8 * (int)(entry & 0xFFu);
-            float t[3];
-            float tmin = INFINITY;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                if (st[a] == 0) {
-                    t[a] = INFINITY;
-                } else {
-                    const float plane = (float)(st[a] > 0 ? b8[a] + 8 + rad : b8[a] - rad);
-                    t[a] = (plane - q[a]) * rinv[a];
-                }
-                tmin = gmin(tmin, t[a]);
-            }
-            uint32_t mk = 0;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const int lo = b8[a] - rad, hi = b8[a] + 8 + rad;
-                if (st[a] != 0 && t[a] <= tmin) {
-                    mk |= 1u << a;
-                    q[a] = (float)(st[a] > 0 ? hi : lo);
-                    v[a] = st[a] > 0 ? hi : lo - 1;
-                } else {
-                    const float x = q[a] + tmin * u[a];
-                    q[a] = x;
-                    int f = (int)floorf(x);
-                    f = f < lo ? lo : (f > hi - 1 ? hi - 1 : f);
-                    v[a] = f;
-                }
-            }
-            mask = mk;
-            ++steps;
-            if (++it >= guard || !((uint32_t)v[0] < (uint32_t)n[0] && (uint32_t)v[1] < (uint32_t)n[1] && (uint32_t)v[2] < (uint32_t)n[2])) {
-                outside = true;
-                break;
-            }
-            b8[0] = v[0] & ~7; b8[1] = v[1] & ~7; b8[2] = v[2] & ~7;
-            entry = gld(V.brick_map + (uint32_t)(b8[0] >> 3) + bdx * ((uint32_t)(b8[1] >> 3) + bdy * (uint32_t)(b8[2] >> 3)));
-            ctr.brick(!vt_brick_empty(entry));
-        }
-        if (outside) break;
-        stage_mask(V.masks, entry, mslot);
-        int cur_z = -1;
-        uint64_t slice = 0;
-        // the voxel through which each stepping axis leaves this brick (or the volume)
-        int out[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) out[a] = st[a] > 0 ? min(b8[a] + 8, n[a]) : (st[a] < 0 ? b8[a] - 1 : INT_MIN);
-        float sx = ((sg[0] * ((float)v[0] - q[0]) + sg[0] * 0.5f) + 0.5f) * dl[0];
-        float sy = ((sg[1] * ((float)v[1] - q[1]) + sg[1] * 0.5f) + 0.5f) * dl[1];
-        float sz = ((sg[2] * ((float)v[2] - q[2]) + sg[2] * 0.5f) + 0.5f) * dl[2];
-        float tl = 0.0f;
-        if (FAST) {
-            // Every axis steps: remaining-step counts per axis as in the primary walk, the voxel's
-            // brick-local coordinates are out - b8 - st * r, and the voxel a step moves into is
-            // tested before the step's side update, so the sides left when the loop ends give the
-            // last step's axes and side.
-            uint32_t rx = (uint32_t)abs(out[0] - v[0]), ry = (uint32_t)abs(out[1] - v[1]), rz = (uint32_t)abs(out[2] - v[2]);
-            const int bit0 = (out[0] - b8[0]) + 8 * (out[1] - b8[1]), az = out[2] - b8[2];
-            int lz = v[2] - b8[2];
-            uint64_t sl = mask_slice(mslot, lz);
-            uint32_t bit = (uint32_t)(v[0] - b8[0]) + 8u * (uint32_t)(v[1] - b8[1]);
-            if (((uint32_t)(sl >> bit) & 1u) == 0u) {
-                float t;
-                for (;;) {
-                    t = fminf(fminf(sx, sy), sz);
-                    const bool mx = sx <= t, my = sy <= t, mz = sz <= t;
-                    rx -= mx ? 1u : 0u;
-                    ry -= my ? 1u : 0u;
-                    rz -= mz ? 1u : 0u;
-                    if (min(rx, min(ry, rz)) == 0u) break;
-                    ++steps;
-                    const int nlz = az - __mul24(st[2], (int)rz);
-                    if (nlz != lz) {
-                        lz = nlz;
-                        sl = mask_slice(mslot, lz);
-                    }
-                    bit = (uint32_t)(bit0 - __mul24(st[0], (int)rx) - __mul24(8 * st[1], (int)ry));
-                    if (((uint32_t)(sl >> bit) & 1u) != 0u) break;
-                    sx = mx ? sx + dl[0] : sx;
-                    sy = my ? sy + dl[1] : sy;
-                    sz = mz ? sz + dl[2] : sz;
-                }
-                const bool lmx = sx <= t, lmy = sy <= t, lmz = sz <= t;
-                mask = (lmx ? 1u : 0u) | (lmy ? 2u : 0u) | (lmz ? 4u : 0u);
-                tl = lmx ? sx : (lmy ? sy : sz);
-                if (min(rx, min(ry, rz)) == 0u) ++steps;   // the step that leaves the brick
-            }
-            v[0] = out[0] - __mul24(st[0], (int)rx);
-            v[1] = out[1] - __mul24(st[1], (int)ry);
-            v[2] = out[2] - __mul24(st[2], (int)rz);
-            if (min(rx, min(ry, rz)) != 0u) {
-                li = bit + 64u * (uint32_t)lz;
-                hit = true;
-                t_last = tl;
-                break;
-            }
-        } else {
-            for (;;) {
-                const int lz = v[2] - b8[2];
-                if (lz != cur_z) {
-                    cur_z = lz;
-                    slice = mask_slice(mslot, lz);
-                }
-                const uint32_t bit = (uint32_t)(v[0] - b8[0]) + 8u * (uint32_t)(v[1] - b8[1]);
-                if ((slice >> bit) & 1ull) {
-                    li = bit + 64u * (uint32_t)lz;
-                    hit = true;
-                    t_last = tl;
-                    break;
-                }
-                const bool mx = sx <= gmin(sy, sz), my = sy <= gmin(sz, sx), mz = sz <= gmin(sx, sy);
-                tl = mx ? sx : (my ? sy : sz);
-                sx = mx ? sx + dl[0] : sx;
-                sy = my ? sy + dl[1] : sy;
-                sz = mz ? sz + dl[2] : sz;
-                v[0] += mx ? st[0] : 0;
-                v[1] += my ? st[1] : 0;
-                v[2] += mz ? st[2] : 0;
-                mask = (mx ? 1u : 0u) | (my ? 2u : 0u) | (mz ? 4u : 0u);
-                ++steps;
-                if (v[0] == out[0] || v[1] == out[1] || v[2] == out[2]) break;
-            }
-        }
-        if (hit) break;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (mask & (1u << a)) {
-                q[a] = (float)(st[a] > 0 ? v[a] : v[a] + 1);
-            } else {
-                const float x = q[a] + tl * u[a];
-                const float lo = (float)v[a], hi = (float)(v[a] + 1);
-                q[a] = x < lo ? lo : (x > hi ? hi : x);
-            }
-        }
-    }
-    ctr.steps(steps);
-    r.v[0] = v[0]; r.v[1] = v[1]; r.v[2] = v[2];
-    r.steps = steps;
-    r.mask = hit && steps > 0 ? mask : 0u;
-    r.t_last = hit ? t_last : 0.0f;
-    r.q[0] = q[0]; r.q[1] = q[1]; r.q[2] = q[2];
-    r.entry = entry;
-    r.li = li;
-    r.hit = hit;
+// Secondary-ray walk (no reference counterpart; this project's rule).  From position q in
+// voxel v the ray crosses, one box per step, the largest box known to be empty that holds v:
+//   * in an empty brick, of the cube of radius r centred on the brick and the cube of side K_o
+//     anchored at it towards the ray's octant o (both from the brick map entry), the one whose
+//     exit lies further along the ray (the centred one on ties);
+//   * in a non-empty brick, the brick's 4^3 octant holding v if its 8 blocks are empty, else the
+//     2^3 block holding v if it is empty (block mask), else v itself -- unless v is occupied
+//     (the LDS-staged occupancy mask), which ends the walk with a hit.
+// A step moves q to the box's nearest exit plane along the ray (ties: every axis at that
+// distance): exit axes step into the voxel beyond the plane, the other axes take floor(q)
+// clamped to the box.  Every step runs the same instructions whatever the box, so lanes of a
+// wave walking empty space and lanes walking voxels share each iteration.
+__device__ __forceinline__ uint64_t gld64(const uint64_t* p) {
+    return *(const __attribute__((address_space(1))) uint64_t*)(p);
 }
 
 template <bool ST>
 __device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], const int v0[3], V3 m, Dda& r,
                                          Ctr<ST>& ctr, const Lds& mslot) {
     const float len = sqrtf((m.x * m.x + m.y * m.y) + m.z * m.z);
-    if (isfinite(fabsf(len / m.x)) && isfinite(fabsf(len / m.y)) && isfinite(fabsf(len / m.z)))
-        dda2_walk<ST, true>(V, p, v0, m, r, ctr, mslot);
-    else
-        dda2_walk<ST, false>(V, p, v0, m, r, ctr, mslot);
+    const float mm[3] = {m.x, m.y, m.z};
+    float u[3], rinv[3];
+    int st[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        st[a] = mm[a] > 0.0f ? 1 : (mm[a] < 0.0f ? -1 : 0);
+        u[a] = mm[a] / len;
+        rinv[a] = len / mm[a];
+    }
+    const uint32_t oct = (st[0] < 0 ? 1u : 0u) | (st[1] < 0 ? 2u : 0u) | (st[2] < 0 ? 4u : 0u);
+    int v[3] = {v0[0], v0[1], v0[2]};
+    float q[3] = {p[0], p[1], p[2]};
+    const int n[3] = {(int)V.dim[0], (int)V.dim[1], (int)V.dim[2]};
+    const uint32_t bdx = V.bdim[0], bdy = V.bdim[1];
+    uint32_t steps = 0, mask = 0, entry = VT_EMPTY, li = 0, cur = VT_EMPTY;
+    uint64_t blk = 0;
+    bool hit = false;
+    const uint32_t guard = 2u * (V.dim[0] + V.dim[1] + V.dim[2]) + 64u;
+    for (uint32_t it = 0; it < guard; ++it) {
+        if (!((uint32_t)v[0] < (uint32_t)n[0] && (uint32_t)v[1] < (uint32_t)n[1] && (uint32_t)v[2] < (uint32_t)n[2])) break;
+        const int b8[3] = {v[0] & ~7, v[1] & ~7, v[2] & ~7};
+        const uint32_t bi = (uint32_t)(b8[0] >> 3) + bdx * ((uint32_t)(b8[1] >> 3) + bdy * (uint32_t)(b8[2] >> 3));
+        if (bi != cur) {
+            cur = bi;
+            entry = gld(V.brick_map + bi);
+            ctr.brick(!vt_brick_empty(entry));
+            if (!vt_brick_empty(entry)) {
+                stage_mask(V.masks, entry, mslot);
+                blk = gld64(V.blocks + entry);
+            }
+        }
+        int lo[3], hi[3];
+        if (vt_brick_empty(entry)) {
+            const int rad = 8 * (int)vt_brick_radius(entry);
+            const int k8 = 8 * (int)vt_brick_cube(entry, oct);
+            int alo[3], ahi[3];
+            float tc = INFINITY, ta = INFINITY;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = b8[a] - rad;
+                hi[a] = b8[a] + 8 + rad;
+                alo[a] = st[a] < 0 ? b8[a] + 8 - k8 : b8[a];
+                ahi[a] = st[a] < 0 ? b8[a] + 8 : b8[a] + k8;
+                if (st[a] != 0) {
+                    tc = gmin(tc, ((float)(st[a] > 0 ? hi[a] : lo[a]) - q[a]) * rinv[a]);
+                    ta = gmin(ta, ((float)(st[a] > 0 ? ahi[a] : alo[a]) - q[a]) * rinv[a]);
+                }
+            }
+            if (ta > tc) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) { lo[a] = alo[a]; hi[a] = ahi[a]; }
+            }
+        } else {
+            const int lx = v[0] - b8[0], ly = v[1] - b8[1], lz = v[2] - b8[2];
+            int sz;   // side of the box: 4, 2 or 1
+            if (((blk >> (uint32_t)(2 * (lx >> 2) + 8 * (ly >> 2) + 32 * (lz >> 2))) & 0x330033ull) == 0ull) {
+                sz = 4;
+            } else if (((blk >> (uint32_t)((lx >> 1) + 4 * (ly >> 1) + 16 * (lz >> 1))) & 1ull) == 0ull) {
+                sz = 2;
+            } else {
+                const uint32_t bit = (uint32_t)lx + 8u * (uint32_t)ly;
+                if (((uint32_t)(mask_slice(mslot, lz) >> bit) & 1u) != 0u) {
+                    li = bit + 64u * (uint32_t)lz;
+                    hit = true;
+                    break;
+                }
+                sz = 1;
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = v[a] & -sz;
+                hi[a] = lo[a] + sz;
+            }
+        }
+        float t[3];
+        float tmin = INFINITY;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            t[a] = st[a] != 0 ? ((float)(st[a] > 0 ? hi[a] : lo[a]) - q[a]) * rinv[a] : INFINITY;
+            tmin = gmin(tmin, t[a]);
+        }
+        uint32_t mk = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (st[a] != 0 && t[a] <= tmin) {
+                mk |= 1u << a;
+                q[a] = (float)(st[a] > 0 ? hi[a] : lo[a]);
+                v[a] = st[a] > 0 ? hi[a] : lo[a] - 1;
+            } else {
+                const float x = q[a] + tmin * u[a];
+                q[a] = x;
+                int f = (int)floorf(x);
+                f = f < lo[a] ? lo[a] : (f > hi[a] - 1 ? hi[a] - 1 : f);
+                v[a] = f;
+            }
+        }
+        mask = mk;
+        ++steps;
+    }
+    ctr.steps(steps);
+    r.v[0] = v[0]; r.v[1] = v[1]; r.v[2] = v[2];
+    r.steps = steps;
+    r.mask = hit && steps > 0 ? mask : 0u;
+    r.t_last = 0.0f;
+    r.q[0] = q[0]; r.q[1] = q[1]; r.q[2] = q[2];
+    r.entry = entry;
+    r.li = li;
+    r.hit = hit;
 }
 
 struct HitG {

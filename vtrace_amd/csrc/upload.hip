@@ -13,13 +13,18 @@
 //
 // Kernels:
 //   k_brick_classify  one 512-thread block per brick: occupancy bit per voxel
-//                     (alpha > 0, trace.frag:78), one ballot per z-slice -> 8 x u64 mask;
+//                     (alpha > 0, trace.frag:78), one ballot per z-slice -> 8 x u64 mask,
+//                     and the 2^3-block mask (one bit per 2x2x2 block holding a voxel);
 //   k_scan_*          exclusive scan of the non-empty flags -> brick ids in brick order;
 //   k_radius_*        empty-space radius of every empty brick: breadth-first layers of
 //                     26-neighbour dilation from the non-empty bricks (in place, layer k marks
 //                     the still-unreached bricks next to layer k-1);
-//   k_brick_scatter   brick map (id, or VT_EMPTY_BRICK | radius), masks and payload of
-//                     non-empty bricks into the pools.
+//   k_cube_*          per empty brick and octant, the side of the largest empty cube with the
+//                     brick as its corner: layer k grows the side from k to k + 1 where the
+//                     7 neighbours towards the octant all have sides >= k (in place: sides only
+//                     grow, so a neighbour updated during the layer decides the same way);
+//   k_brick_scatter   brick map (id, or VT_EMPTY_BRICK with radius and cube sides), masks,
+//                     block masks and payload of non-empty bricks into the pools.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -61,7 +66,19 @@ __device__ inline size_t brick_coords(const BrickArgs& a, uint32_t b, uint32_t* 
     return (size_t)k * a.vol_texels;
 }
 
-__global__ __launch_bounds__(512) void k_brick_classify(BrickArgs a, uint64_t* masks_all, uint32_t* flags) {
+// 2x2 footprints of a z-slice's occupancy (bit x + 8y) -> 16 bits, bit bx + 4by for the
+// 2x2 column (2bx.., 2by..) of the slice
+__device__ inline uint32_t slice_blocks(uint64_t m) {
+    uint64_t t = m | (m >> 1);
+    t |= t >> 8;
+    uint32_t r = 0;
+    for (uint32_t by = 0; by < 4; ++by)
+        for (uint32_t bx = 0; bx < 4; ++bx) r |= (uint32_t)((t >> (2u * bx + 16u * by)) & 1u) << (bx + 4u * by);
+    return r;
+}
+
+__global__ __launch_bounds__(512) void k_brick_classify(BrickArgs a, uint64_t* masks_all, uint64_t* blocks_all,
+                                                        uint32_t* flags) {
     const uint32_t b = blockIdx.x;
     uint32_t bx, by, bz;
     const size_t vbase = brick_coords(a, b, &bx, &by, &bz);
@@ -76,14 +93,54 @@ __global__ __launch_bounds__(512) void k_brick_classify(BrickArgs a, uint64_t* m
     }
     uint64_t m = __ballot(occ);  // wave t>>6 holds z-slice t>>6
     __shared__ uint32_t any;
-    if (t == 0) any = 0;
+    __shared__ uint32_t blk[2];   // 2^3-block mask: slices 2bz, 2bz+1 -> bits 16bz.. (bz = 0, 1 | 2, 3)
+    if (t == 0) { any = 0; blk[0] = 0; blk[1] = 0; }
     __syncthreads();
     if ((t & 63) == 0) {
         masks_all[(size_t)b * 8 + (t >> 6)] = m;
-        if (m) atomicOr(&any, 1u);
+        if (m) {
+            atomicOr(&any, 1u);
+            const uint32_t bz = (t >> 6) >> 1;
+            atomicOr(&blk[bz >> 1], slice_blocks(m) << (16u * (bz & 1u)));
+        }
     }
     __syncthreads();
-    if (t == 0) flags[b] = any;
+    if (t == 0) {
+        flags[b] = any;
+        blocks_all[b] = (uint64_t)blk[0] | ((uint64_t)blk[1] << 32);
+    }
+}
+
+// cube sides of all 8 octants packed 4 bits each (0 for a non-empty brick, 1 for an empty one)
+__global__ void k_cube_init(const uint32_t* flags, uint32_t* cube, uint32_t nb) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb) cube[b] = flags[b] ? 0u : 0x11111111u;
+}
+
+// layer k over nvol volumes of bx*by*bz bricks each (neighbours within the same volume; outside
+// the volume counts as a side of VT_BRICK_CUBE_CAP)
+__global__ void k_cube_step(uint32_t* cube, uint32_t bx, uint32_t by, uint32_t bz, uint32_t nvol, uint32_t k) {
+    const uint32_t gb = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nbv = bx * by * bz;
+    if (gb >= nbv * nvol) return;
+    const uint32_t cur = cube[gb];
+    if (cur == 0u) return;
+    const uint32_t base = (gb / nbv) * nbv, b = gb - base;
+    const int x = (int)(b % bx), y = (int)((b / bx) % by), z = (int)(b / (bx * by));
+    uint32_t out = cur;
+    for (uint32_t o = 0; o < 8; ++o) {
+        if (((cur >> (4u * o)) & 15u) != k) continue;
+        const int dx = (o & 1u) ? -1 : 1, dy = (o & 2u) ? -1 : 1, dz = (o & 4u) ? -1 : 1;
+        bool grow = true;
+        for (int n = 1; n < 8 && grow; ++n) {
+            const int X = x + ((n & 1) ? dx : 0), Y = y + ((n & 2) ? dy : 0), Z = z + ((n & 4) ? dz : 0);
+            if (X < 0 || Y < 0 || Z < 0 || X >= (int)bx || Y >= (int)by || Z >= (int)bz) continue;
+            const uint32_t nk = (cube[base + (uint32_t)X + bx * ((uint32_t)Y + by * (uint32_t)Z)] >> (4u * o)) & 15u;
+            grow = nk >= k;
+        }
+        if (grow) out += 1u << (4u * o);
+    }
+    if (out != cur) cube[gb] = out;
 }
 
 __global__ void k_radius_init(const uint32_t* flags, uint32_t* dist, uint32_t nb) {
@@ -110,18 +167,27 @@ __global__ void k_radius_step(uint32_t* dist, uint32_t bx, uint32_t by, uint32_t
             }
 }
 
-__global__ __launch_bounds__(512) void k_brick_scatter(BrickArgs a, const uint64_t* masks_all, const uint32_t* ids,
-                                                       const uint32_t* flags, const uint32_t* dist,
-                                                       uint32_t* brick_map, uint64_t* masks, uint8_t* payload) {
+__global__ __launch_bounds__(512) void k_brick_scatter(BrickArgs a, const uint64_t* masks_all,
+                                                       const uint64_t* blocks_all, const uint32_t* ids,
+                                                       const uint32_t* flags, const uint32_t* dist, const uint32_t* cube,
+                                                       uint32_t* brick_map, uint64_t* masks, uint64_t* blocks,
+                                                       uint8_t* payload) {
     const uint32_t b = blockIdx.x;
     const uint32_t t = threadIdx.x;
     const bool nonempty = flags[b] != 0;
     const uint32_t id = ids[b];                            // pool slot
     const uint32_t first = ids[(b / a.nbv) * a.nbv];        // pool slot of the volume's first brick
-    if (t == 0)
-        brick_map[b] = nonempty ? id - first : (VT_EMPTY_BRICK | (min(dist[b], VT_BRICK_RADIUS_CAP + 1u) - 1u));
+    if (t == 0) {
+        uint32_t e = id - first;
+        if (!nonempty) {
+            e = VT_EMPTY_BRICK | ((min(dist[b], VT_BRICK_RADIUS_CAP + 1u) - 1u) << 24);
+            for (uint32_t o = 0; o < 8; ++o) e |= (((cube[b] >> (4u * o)) & 15u) - 1u) << (3u * o);
+        }
+        brick_map[b] = e;
+    }
     if (!nonempty) return;
     if (t < 8) masks[(size_t)id * 8 + t] = masks_all[(size_t)b * 8 + t];
+    if (t == 8) blocks[id] = blocks_all[b];
     uint32_t bx, by, bz;
     const size_t vbase = brick_coords(a, b, &bx, &by, &bz);
     const uint32_t gx = bx * 8 + (t & 7), gy = by * 8 + ((t >> 3) & 7), gz = bz * 8 + (t >> 6);
@@ -230,23 +296,31 @@ int vt_brick_volume(const uint8_t* dense_dev, uint32_t fmt, const uint32_t* pale
     const uint64_t nb64 = (uint64_t)a.nbv * nvol;
     if (nb64 >= 0x7FFFFFFFull) return -1;
     const uint32_t nb = (uint32_t)nb64;
-    uint64_t* masks_all = nullptr;
-    uint32_t *flags = nullptr, *ids = nullptr, *blocks = nullptr, *total = nullptr, *dist = nullptr;
+    uint64_t *masks_all = nullptr, *blocks_all = nullptr;
+    uint32_t *flags = nullptr, *ids = nullptr, *blocks = nullptr, *total = nullptr, *dist = nullptr, *cube = nullptr;
     const uint32_t nblk = (nb + kScanBlock - 1) / kScanBlock + 1;
-    if (hipMalloc(&masks_all, (size_t)nb * 64) != hipSuccess) return -1;
-    if (hipMalloc(&flags, (size_t)nb * 4) != hipSuccess || hipMalloc(&ids, (size_t)nb * 4) != hipSuccess ||
+    auto free_temps = [&]() {
+        hipFree(masks_all); hipFree(blocks_all); hipFree(flags); hipFree(ids); hipFree(blocks); hipFree(total);
+        hipFree(dist); hipFree(cube);
+    };
+    if (hipMalloc(&masks_all, (size_t)nb * 64) != hipSuccess || hipMalloc(&blocks_all, (size_t)nb * 8) != hipSuccess ||
+        hipMalloc(&flags, (size_t)nb * 4) != hipSuccess || hipMalloc(&ids, (size_t)nb * 4) != hipSuccess ||
         hipMalloc(&blocks, (size_t)nblk * 4) != hipSuccess || hipMalloc(&total, 4) != hipSuccess ||
-        hipMalloc(&dist, (size_t)nb * 4) != hipSuccess) {
-        hipFree(masks_all); hipFree(flags); hipFree(ids); hipFree(blocks); hipFree(total); hipFree(dist);
+        hipMalloc(&dist, (size_t)nb * 4) != hipSuccess || hipMalloc(&cube, (size_t)nb * 4) != hipSuccess) {
+        free_temps();
         return -1;
     }
-    hipLaunchKernelGGL(k_brick_classify, dim3(nb), dim3(512), 0, s, a, masks_all, flags);
+    hipLaunchKernelGGL(k_brick_classify, dim3(nb), dim3(512), 0, s, a, masks_all, blocks_all, flags);
     vt_scan_flags(flags, nb, ids, blocks, total, s);
     hipLaunchKernelGGL(k_radius_init, dim3((nb + 255) / 256), dim3(256), 0, s, flags, dist, nb);
     const uint32_t layers = std::min<uint32_t>(VT_BRICK_RADIUS_CAP + 1u,
                                                std::max(a.bdim[0], std::max(a.bdim[1], a.bdim[2])));
     for (uint32_t k = 1; k <= layers; ++k)
         hipLaunchKernelGGL(k_radius_step, dim3((nb + 255) / 256), dim3(256), 0, s, dist, a.bdim[0], a.bdim[1],
+                           a.bdim[2], nvol, k);
+    hipLaunchKernelGGL(k_cube_init, dim3((nb + 255) / 256), dim3(256), 0, s, flags, cube, nb);
+    for (uint32_t k = 1; k < VT_BRICK_CUBE_CAP; ++k)
+        hipLaunchKernelGGL(k_cube_step, dim3((nb + 255) / 256), dim3(256), 0, s, cube, a.bdim[0], a.bdim[1],
                            a.bdim[2], nvol, k);
     // pool slot of each volume's first brick (ids[k * nbv]) and the total
     std::vector<uint32_t> first(nvol + 1, 0);
@@ -259,21 +333,22 @@ int vt_brick_volume(const uint8_t* dense_dev, uint32_t fmt, const uint32_t* pale
     const size_t bpv = fmt == 0 ? 1 : 4;
     uint32_t* pool_map = nullptr;
     uint32_t* pool_masks = nullptr;
+    uint64_t* pool_blocks = nullptr;
     uint8_t* pool_payload = nullptr;
     if (rc == 0) {
         const size_t cnt = count ? count : 1;
         if (hipMalloc(&pool_map, (size_t)nb * 4) != hipSuccess || hipMalloc(&pool_masks, cnt * 64) != hipSuccess ||
-            hipMalloc(&pool_payload, cnt * 512 * bpv) != hipSuccess) {
-            hipFree(pool_map); hipFree(pool_masks); hipFree(pool_payload);
+            hipMalloc(&pool_blocks, cnt * 8) != hipSuccess || hipMalloc(&pool_payload, cnt * 512 * bpv) != hipSuccess) {
+            hipFree(pool_map); hipFree(pool_masks); hipFree(pool_blocks); hipFree(pool_payload);
             rc = -1;
         }
     }
     if (rc == 0) {
-        hipLaunchKernelGGL(k_brick_scatter, dim3(nb), dim3(512), 0, s, a, masks_all, ids, flags, dist, pool_map,
-                           reinterpret_cast<uint64_t*>(pool_masks), pool_payload);
+        hipLaunchKernelGGL(k_brick_scatter, dim3(nb), dim3(512), 0, s, a, masks_all, blocks_all, ids, flags, dist, cube,
+                           pool_map, reinterpret_cast<uint64_t*>(pool_masks), pool_blocks, pool_payload);
         if (hipGetLastError() != hipSuccess) rc = -1;
         if (hipStreamSynchronize(s) != hipSuccess) rc = -1;
-        if (rc != 0) { hipFree(pool_map); hipFree(pool_masks); hipFree(pool_payload); }
+        if (rc != 0) { hipFree(pool_map); hipFree(pool_masks); hipFree(pool_blocks); hipFree(pool_payload); }
     }
     if (rc == 0) {
         for (uint32_t k = 0; k < nvol; ++k) {
@@ -285,11 +360,12 @@ int vt_brick_volume(const uint8_t* dense_dev, uint32_t fmt, const uint32_t* pale
             o.brick_map = pool_map + (size_t)k * a.nbv;
             o.masks = pool_masks + (size_t)first[k] * 16;
             o.payload = pool_payload + (size_t)first[k] * 512 * bpv;
+            o.blocks = pool_blocks + first[k];
             o.palette = const_cast<uint32_t*>(palette_dev);
-            o.bytes = (size_t)a.nbv * 4 + (size_t)o.nbricks * (64 + 512 * bpv);
+            o.bytes = (size_t)a.nbv * 4 + (size_t)o.nbricks * (64 + 8 + 512 * bpv);
         }
     }
-    hipFree(masks_all); hipFree(flags); hipFree(ids); hipFree(blocks); hipFree(total); hipFree(dist);
+    free_temps();
     return rc;
 }
 

@@ -20,6 +20,7 @@ struct VolumeStore {
     uint32_t* masks = nullptr;
     uint8_t* payload = nullptr;
     uint32_t* palette = nullptr;
+    uint64_t* blocks = nullptr;
     size_t bytes = 0;
     bool owner = true;          // frees brick_map/masks/payload/palette (volumes bricked together share them)
 };

@@ -1,12 +1,14 @@
 // vt_types.h — data layout shared by the HIP kernels and the host runtime of librender_hip.
 //
 // HBM layout (one GPU):
-//   * volumes: per volume a brick map (u32 per 8^3 brick, x-fastest brick order,
-//     VT_EMPTY_BRICK | radius = empty), an occupancy-mask pool (64 B = 16 u32 per non-empty brick,
-//     bit x + 8y + 64z of the brick) and a payload pool (512 B u8 palette indices or
-//     2048 B RGBA8 per non-empty brick, voxel x + 8y + 64z), plus a 256-entry palette.
+//   * volumes: per volume a brick map (u32 per 8^3 brick, x-fastest brick order; empty bricks
+//     hold VT_EMPTY_BRICK with their empty-space sizes, below), an occupancy-mask pool (64 B =
+//     16 u32 per non-empty brick, bit x + 8y + 64z of the brick), a block-mask pool (u64 per
+//     non-empty brick, bit bx + 4by + 16bz set when its 2^3 block (2bx.., 2by.., 2bz..) holds an
+//     occupied voxel) and a payload pool (512 B u8 palette indices or 2048 B RGBA8 per non-empty
+//     brick, voxel x + 8y + 64z), plus a 256-entry palette.
 //     Replaces the reference's R8G8B8A8_SRGB 3-D images (lib/memory.c:286-385).
-//   * a volume-descriptor table (64 B per volume) replacing the bindless sampler3D array
+//   * a volume-descriptor table (72 B per volume) replacing the bindless sampler3D array
 //     (lib/descriptor.c:38-95, shaders/trace.frag:31).
 //   * instances: the 64 B ABI records (src/render.rs:31-35) expanded to 128 B GPU records
 //     holding M and inverse(M) (rows 0..2), the texture id and the volume size.
@@ -18,17 +20,29 @@
 #include <stdint.h>
 
 #define VT_EMPTY 0xFFFFFFFFu
-// Brick map entries: a non-empty brick holds its id; an empty brick holds VT_EMPTY_BRICK | r,
-// r = radius of the cube of empty bricks around it (L-inf brick distance to the nearest
-// non-empty brick - 1, at most VT_BRICK_RADIUS_CAP; bricks outside the volume count as empty).
-#define VT_EMPTY_BRICK 0xFFFFFF00u
+// Brick map entries: a non-empty brick holds its id (< VT_EMPTY_BRICK); an empty brick holds
+//   VT_EMPTY_BRICK | r << 24 | (K_o - 1) << 3o   (o = 0..7)
+// r = radius of the cube of empty bricks centred on it (L-inf brick distance to the nearest
+// non-empty brick - 1, at most VT_BRICK_RADIUS_CAP); K_o = side, in bricks, of the largest cube
+// of empty bricks that has this brick as its corner and extends towards octant o (bit a of o
+// set: towards -axis a), at most VT_BRICK_CUBE_CAP.  Bricks outside the volume count as empty.
+#define VT_EMPTY_BRICK 0xE0000000u
 #define VT_BRICK_RADIUS_CAP 31u
+#define VT_BRICK_CUBE_CAP 8u
 #ifdef __HIPCC__
 __host__ __device__
 #endif
 inline bool vt_brick_empty(uint32_t e) { return e >= VT_EMPTY_BRICK; }
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline uint32_t vt_brick_radius(uint32_t e) { return (e >> 24) & 31u; }
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline uint32_t vt_brick_cube(uint32_t e, uint32_t octant) { return ((e >> (3u * octant)) & 7u) + 1u; }
 
-struct VolumeDesc {             // 64 B
+struct VolumeDesc {             // 72 B
     uint32_t dim[3];
     uint32_t bdim[3];
     uint32_t fmt;               // 0 = u8 index + palette, 1 = RGBA8
@@ -37,6 +51,7 @@ struct VolumeDesc {             // 64 B
     const uint32_t* masks;
     const uint8_t* payload;
     const uint32_t* palette;
+    const uint64_t* blocks;     // 2^3-block masks of the non-empty bricks
 };
 
 struct InstanceGPU {            // 128 B
