@@ -56,6 +56,9 @@ constexpr uint32_t kQueueCounters = 64;
 constexpr uint32_t kParts = 8;
 constexpr uint32_t kPartStride = 64;   // 256 B between the counters of one slot
 constexpr uint32_t kCounters = kSlots * kParts * kPartStride;
+// Counter slot of the sun list of the queue whose counters are in slot qi: [1] for the queue of
+// the primary vertices, qi + 32 for the queue built at depth d (slot 64 + d + 1).
+__host__ __device__ constexpr uint32_t sun_slot(uint32_t qi) { return qi == 0 ? 1u : qi + 32u; }
 __host__ __device__ constexpr uint32_t seg_cap(uint32_t npaths) { return ((npaths + kParts - 1) / kParts + 63u) & ~63u; }
 
 // GLSL min/max as "y if y < x else x" (also fixes the NaN cases the DDA can produce)
@@ -466,6 +469,9 @@ __device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], 
     uint64_t blk = 0;
     bool hit = false;
     const uint32_t guard = 2u * (V.dim[0] + V.dim[1] + V.dim[2]) + 64u;
+    // One iteration is one step for every lane, whatever its box: both candidate boxes of an
+    // empty brick are evaluated, a non-empty brick's box fills both candidates, and the exit
+    // of each axis is a select, so lanes in empty and in occupied bricks run the same code.
     for (uint32_t it = 0; it < guard; ++it) {
         if (!((uint32_t)v[0] < (uint32_t)n[0] && (uint32_t)v[1] < (uint32_t)n[1] && (uint32_t)v[2] < (uint32_t)n[2])) break;
         const int b8[3] = {v[0] & ~7, v[1] & ~7, v[2] & ~7};
@@ -475,74 +481,55 @@ __device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], 
             entry = gld(V.brick_map + bi);
             ctr.brick(!vt_brick_empty(entry));
             if (!vt_brick_empty(entry)) {
+                blk = gld64(V.blocks + entry);   // issued with the mask loads: one round trip after the entry
                 stage_mask(V.masks, entry, mslot);
-                blk = gld64(V.blocks + entry);
             }
         }
-        int lo[3], hi[3];
-        if (vt_brick_empty(entry)) {
-            const int rad = 8 * (int)vt_brick_radius(entry);
-            const int k8 = 8 * (int)vt_brick_cube(entry, oct);
-            int alo[3], ahi[3];
-            float tc = INFINITY, ta = INFINITY;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                lo[a] = b8[a] - rad;
-                hi[a] = b8[a] + 8 + rad;
-                alo[a] = st[a] < 0 ? b8[a] + 8 - k8 : b8[a];
-                ahi[a] = st[a] < 0 ? b8[a] + 8 : b8[a] + k8;
-                if (st[a] != 0) {
-                    tc = gmin(tc, ((float)(st[a] > 0 ? hi[a] : lo[a]) - q[a]) * rinv[a]);
-                    ta = gmin(ta, ((float)(st[a] > 0 ? ahi[a] : alo[a]) - q[a]) * rinv[a]);
-                }
-            }
-            if (ta > tc) {
-#pragma unroll
-                for (int a = 0; a < 3; ++a) { lo[a] = alo[a]; hi[a] = ahi[a]; }
-            }
-        } else {
-            const int lx = v[0] - b8[0], ly = v[1] - b8[1], lz = v[2] - b8[2];
-            int sz;   // side of the box: 4, 2 or 1
-            if (((blk >> (uint32_t)(2 * (lx >> 2) + 8 * (ly >> 2) + 32 * (lz >> 2))) & 0x330033ull) == 0ull) {
-                sz = 4;
-            } else if (((blk >> (uint32_t)((lx >> 1) + 4 * (ly >> 1) + 16 * (lz >> 1))) & 1ull) == 0ull) {
-                sz = 2;
-            } else {
-                const uint32_t bit = (uint32_t)lx + 8u * (uint32_t)ly;
-                if (((uint32_t)(mask_slice(mslot, lz) >> bit) & 1u) != 0u) {
-                    li = bit + 64u * (uint32_t)lz;
-                    hit = true;
-                    break;
-                }
-                sz = 1;
-            }
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                lo[a] = v[a] & -sz;
-                hi[a] = lo[a] + sz;
+        const bool emp = vt_brick_empty(entry);
+        const int lx = v[0] - b8[0], ly = v[1] - b8[1], lz = v[2] - b8[2];
+        int sz = 1;
+        if (((blk >> (uint32_t)(2 * (lx >> 2) + 8 * (ly >> 2) + 32 * (lz >> 2))) & 0x330033ull) == 0ull) sz = 4;
+        else if (((blk >> (uint32_t)((lx >> 1) + 4 * (ly >> 1) + 16 * (lz >> 1))) & 1ull) == 0ull) sz = 2;
+        if (!emp && sz == 1) {
+            const uint32_t bit = (uint32_t)lx + 8u * (uint32_t)ly;
+            if (((uint32_t)(mask_slice(mslot, lz) >> bit) & 1u) != 0u) {
+                li = bit + 64u * (uint32_t)lz;
+                hit = true;
+                break;
             }
         }
-        float t[3];
-        float tmin = INFINITY;
+        // candidates: A = the centred cube (empty brick) or the block box, B = the anchored cube
+        // (empty brick) or the block box again
+        const int rad = 8 * (int)vt_brick_radius(entry);
+        const int k8 = 8 * (int)vt_brick_cube(entry, oct);
+        int lo[3], hi[3], loB[3], hiB[3];
+        float tA[3], tB[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            t[a] = st[a] != 0 ? ((float)(st[a] > 0 ? hi[a] : lo[a]) - q[a]) * rinv[a] : INFINITY;
-            tmin = gmin(tmin, t[a]);
+            const int bl = v[a] & -sz;
+            lo[a] = emp ? b8[a] - rad : bl;
+            hi[a] = emp ? b8[a] + 8 + rad : bl + sz;
+            loB[a] = emp ? (st[a] < 0 ? b8[a] + 8 - k8 : b8[a]) : lo[a];
+            hiB[a] = emp ? (st[a] < 0 ? b8[a] + 8 : b8[a] + k8) : hi[a];
+            tA[a] = st[a] != 0 ? ((float)(st[a] > 0 ? hi[a] : lo[a]) - q[a]) * rinv[a] : INFINITY;
+            tB[a] = st[a] != 0 ? ((float)(st[a] > 0 ? hiB[a] : loB[a]) - q[a]) * rinv[a] : INFINITY;
         }
+        const float tcA = gmin(gmin(gmin(INFINITY, tA[0]), tA[1]), tA[2]);
+        const float tcB = gmin(gmin(gmin(INFINITY, tB[0]), tB[1]), tB[2]);
+        const bool useB = tcB > tcA;
+        const float tmin = useB ? tcB : tcA;
         uint32_t mk = 0;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            if (st[a] != 0 && t[a] <= tmin) {
-                mk |= 1u << a;
-                q[a] = (float)(st[a] > 0 ? hi[a] : lo[a]);
-                v[a] = st[a] > 0 ? hi[a] : lo[a] - 1;
-            } else {
-                const float x = q[a] + tmin * u[a];
-                q[a] = x;
-                int f = (int)floorf(x);
-                f = f < lo[a] ? lo[a] : (f > hi[a] - 1 ? hi[a] - 1 : f);
-                v[a] = f;
-            }
+            const int l = useB ? loB[a] : lo[a], h = useB ? hiB[a] : hi[a];
+            const float t = useB ? tB[a] : tA[a];
+            const bool ex = st[a] != 0 && t <= tmin;
+            const float x = q[a] + tmin * u[a];
+            int f = (int)floorf(x);
+            f = f < l ? l : (f > h - 1 ? h - 1 : f);
+            q[a] = ex ? (float)(st[a] > 0 ? h : l) : x;
+            v[a] = ex ? (st[a] > 0 ? h : l - 1) : f;
+            mk |= ex ? 1u << a : 0u;
         }
         mask = mk;
         ++steps;
@@ -1046,6 +1033,7 @@ struct WF {
     SurfaceRec* surf[2];  // per queue slot
     float4* contrib[2];   // per slot: direct-light contribution if the sun ray is clear (w = 1: trace it)
     float4* dir[2];       // per slot: bounce direction (w = 1: trace it)
+    uint32_t* sun[2];     // sun list: the queue slots whose surfaces face the sun (one sun ray each)
     uint32_t* counters;   // kSlots x kParts counters (queue sizes and fetch counters)
     uint32_t seg_cap;     // slots per queue segment
     float4* gsamp;        // optional: per path (albedo rgb, distance), (normal, hit)
@@ -1214,6 +1202,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
             wf.surf[0][slot] = sf;
             wf.contrib[0][slot] = contrib;
             wf.dir[0][slot] = dirw;
+            if (!no_sun) wf.sun[0][append_slot(slot_counters(wf.counters, sun_slot(0)), wf.seg_cap)] = slot;
         }
         wf.L[w] = L;
         wf.Ls[w] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1225,8 +1214,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
 }
 
 // The secondary rays of one depth in one launch.  Part p covers segment p of this depth's queue
-// (count slots): items [0, nb) are the bounce rays of its slots 0..nb-1 (nb = count when paths
-// continue, else 0), items [nb, nb + count) the sun rays of slots 0..count-1.  The two kinds
+// (count slots) and of its sun list (ns entries): items [0, nb) are the bounce rays of its slots
+// 0..nb-1 (nb = count when paths continue, else 0), items [nb, nb + ns) the sun rays of the
+// slots its sun list names (surfaces facing away from the sun have none, so no lane idles).  The two kinds
 // write different per-path state (bounce rays: L, beta, rng and the next queue; sun rays: Ls and
 // the shadow flag), so they run side by side and fill each other's tail.  in_idx / out_idx:
 // counter slots of this depth's queue and the next one.  ONE: the batch holds one request (its
@@ -1245,19 +1235,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     __syncthreads();
     Ctr<ST> ctr;
     uint32_t* const qin = slot_counters(wf.counters, in_idx);
+    uint32_t* const qsun = slot_counters(wf.counters, sun_slot(in_idx));
     const uint32_t cap = wf.seg_cap;
     const uint32_t nq = 1u - q;
     const uint32_t nreq = B->nreq;
     for_each_item(slot_counters(wf.counters, fetch_idx), &gone,
                   [&](uint32_t p, uint32_t* first, uint32_t* n) {
-                      const uint32_t c = queue_count(qin, p, cap);
                       *first = 0u;
-                      *n = bounce ? 2u * c : c;
+                      *n = (bounce ? queue_count(qin, p, cap) : 0u) + queue_count(qsun, p, cap);
                   },
                   [&](uint32_t p, uint32_t item) {
         const uint32_t nb = bounce ? queue_count(qin, p, cap) : 0u;
         const bool is_bounce = item < nb;
-        const uint32_t slot = p * cap + (is_bounce ? item : item - nb);
+        const uint32_t slot = is_bounce ? p * cap + item : wf.sun[q][p * cap + (item - nb)];
         const float4 dw = is_bounce ? wf.dir[q][slot] : wf.contrib[q][slot];  // w = 1: trace it
         if (dw.w == 0.0f) return;
         const SurfaceRec sf = wf.surf[q][slot];
@@ -1303,6 +1293,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         wf.surf[nq][ns_slot] = ns;
         wf.contrib[nq][ns_slot] = contrib;
         wf.dir[nq][ns_slot] = dirw;
+        if (!no_sun) wf.sun[nq][append_slot(slot_counters(wf.counters, sun_slot(out_idx)), cap)] = ns_slot;
         wf.beta[path] = make_float4(beta[0], beta[1], beta[2], 0.0f);
         wf.rng[path] = rng.s;
     });
@@ -1466,7 +1457,7 @@ void vt_trace_set_persistent_blocks(uint32_t blocks) { g_persistent_blocks = blo
 
 size_t vt_wave_bytes(uint32_t npaths) {
     const size_t qcap = (size_t)kParts * seg_cap(npaths);
-    return (size_t)npaths * (16 + 16 + 16 + 4 + 1) + 2 * qcap * (64 + 16 + 16) + (size_t)kCounters * 4 + 16 * 256 +
+    return (size_t)npaths * (16 + 16 + 16 + 4 + 1) + 2 * qcap * (64 + 16 + 16 + 4) + (size_t)kCounters * 4 + 18 * 256 +
            sizeof(ReqGPU) * VT_MAX_BATCH + sizeof(BatchGPU);
 }
 
@@ -1488,6 +1479,7 @@ int vt_wave_bind(WaveBuffers& wb, void* mem, uint32_t npaths) {
         wb.surf[q] = take(qcap * 64);
         wb.contrib[q] = take(qcap * 16);
         wb.dir[q] = take(qcap * 16);
+        wb.sun[q] = take(qcap * 4);
     }
     wb.counters = reinterpret_cast<uint32_t*>(take((size_t)kCounters * 4));
     wb.tables = take(sizeof(ReqGPU) * VT_MAX_BATCH + sizeof(BatchGPU));
@@ -1505,6 +1497,7 @@ static WF to_wf(const WaveBuffers& wb) {
         w.surf[q] = reinterpret_cast<SurfaceRec*>(wb.surf[q]);
         w.contrib[q] = reinterpret_cast<float4*>(wb.contrib[q]);
         w.dir[q] = reinterpret_cast<float4*>(wb.dir[q]);
+        w.sun[q] = reinterpret_cast<uint32_t*>(wb.sun[q]);
     }
     w.counters = wb.counters;
     w.seg_cap = seg_cap(wb.npaths);

@@ -63,6 +63,7 @@ struct WaveBuffers {
     void* surf[2] = {nullptr, nullptr};     // 64 B surface records per queue slot
     void* contrib[2] = {nullptr, nullptr};  // float4 per slot
     void* dir[2] = {nullptr, nullptr};      // float4 per slot
+    void* sun[2] = {nullptr, nullptr};      // u32 per slot: sun list (queue slots facing the sun)
     uint32_t* counters = nullptr;           // 128 u32: queue sizes and fetch counters
     void* tables = nullptr;                 // ReqGPU[VT_MAX_BATCH] then BatchGPU
     void* gsamp = nullptr;                  // optional, owned by the caller: 2 x float4 per path
