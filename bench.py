@@ -157,23 +157,57 @@ def _kernel_key(name: str):
     return m.group(1), args[0].strip() == "true"
 
 
-def summarize_counters(dirs) -> dict:
-    """Mean per dispatch of every counter of every kernel in rocprofv3 --pmc outputs (CSV),
-    counting-frame instantiations left out."""
+def _counter_rows(d):
+    """(kernel name, counter, value, dispatch id) of every counter sample under a rocprofv3
+    output directory: the rocpd database (run_results.db) or counter_collection CSV files."""
     import csv
     import glob
+    import sqlite3
+    for path in glob.glob(os.path.join(d, "**", "*.db"), recursive=True):
+        db = sqlite3.connect(path)
+        try:
+            for name, counter, value, disp in db.execute(
+                    "select kernel_name, counter_name, value, dispatch_id from counters_collection"):
+                yield name, counter, float(value), disp
+        finally:
+            db.close()
+    for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        for row in csv.DictReader(open(path)):
+            yield (row.get("Kernel_Name", ""), row["Counter_Name"], float(row["Counter_Value"]),
+                   row.get("Dispatch_Id") or row.get("Correlation_Id"))
+
+
+def summarize_counters(dirs) -> dict:
+    """Mean per dispatch of every counter of every kernel in rocprofv3 --pmc outputs,
+    counting-frame instantiations left out."""
     sums, disp = {}, {}
     for d in dirs:
-        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
-            for row in csv.DictReader(open(path)):
-                k, counting = _kernel_key(row.get("Kernel_Name", ""))
-                if k is None or counting:
-                    continue
-                c = row["Counter_Name"]
-                sums.setdefault(k, {}).setdefault(c, 0.0)
-                sums[k][c] += float(row["Counter_Value"])
-                disp.setdefault(k, {}).setdefault(c, set()).add(row.get("Dispatch_Id") or row.get("Correlation_Id"))
+        for name, c, value, dispatch in _counter_rows(d):
+            k, counting = _kernel_key(name)
+            if k is None or counting:
+                continue
+            sums.setdefault(k, {}).setdefault(c, 0.0)
+            sums[k][c] += value
+            disp.setdefault(k, {}).setdefault(c, set()).add(dispatch)
     return {k: {c: v / max(1, len(disp[k][c])) for c, v in cs.items()} for k, cs in sums.items()}
+
+
+def summarize_trace(d) -> list:
+    """[(kernel, calls, average us, min us, max us, total ms)] from a rocprofv3 --kernel-trace
+    database, slowest total first."""
+    import glob
+    import sqlite3
+    out = {}
+    for path in glob.glob(os.path.join(d, "**", "*.db"), recursive=True):
+        db = sqlite3.connect(path)
+        try:
+            for name, dur in db.execute("select name, duration from kernels"):
+                k = name.split("(")[0].replace("void ", "").replace("(anonymous namespace)::", "")
+                out.setdefault(k, []).append(float(dur) * 1e-3)
+        finally:
+            db.close()
+    rows = [(k, len(v), sum(v) / len(v), min(v), max(v), sum(v) * 1e-3) for k, v in out.items()]
+    return sorted(rows, key=lambda r: -r[5])
 
 
 def summarize_profiles(args) -> None:
@@ -199,6 +233,10 @@ def summarize_profiles(args) -> None:
     if args.pmc:
         out = {"workload": args.workload, "command": cmd, "kernels": summarize_counters(args.pmc)}
         json.dump(out, open(os.path.join(ROOT, "profiles", "pmc.json"), "w"), indent=1)
+    if args.trace:
+        print("| kernel | calls | avg us | min us | max us | total ms |\n|---|---|---|---|---|---|")
+        for k, n, avg, mn, mx, tot in summarize_trace(args.trace):
+            print(f"| `{k}` | {n} | {avg:.1f} | {mn:.1f} | {mx:.1f} | {tot:.3f} |")
 
 
 def load_traffic(kernel: str):
@@ -272,6 +310,7 @@ def main():
     ap.add_argument("--cal-fetch")
     ap.add_argument("--cal-write")
     ap.add_argument("--pmc", nargs="*")
+    ap.add_argument("--trace", help="print a markdown table of a rocprofv3 --kernel-trace output directory")
     args = ap.parse_args()
     if args.summarize_profiles:
         summarize_profiles(args)

@@ -751,7 +751,7 @@ __device__ HitG query(const SceneGPU& S, const FrameGPU& F, V3 O, V3 D, Ctr<ST>&
     return best;
 }
 
-// Surface of a hit: what shading and the next query need (64 B, queue element).
+// Surface of a hit, as shading needs it (registers only).
 struct SurfaceRec {
     float p[3];        // hit point, voxel units of the hit instance
     uint32_t inst;
@@ -762,7 +762,52 @@ struct SurfaceRec {
     int vstart[3];     // empty neighbour voxel on the face side
     uint32_t face;
 };
-static_assert(sizeof(SurfaceRec) == 64, "surface record is 64 bytes");
+
+// Queue element: a shaded surface with its two secondary rays (64 B, written once by the launch
+// that found the surface, read by the next depth's bounce ray and sun ray).
+struct QueueRec {
+    float p[3];        // hit point, voxel units of the hit instance
+    float X[3];        // world position
+    float w[3];        // bounce direction (world, unit)
+    float c[3];        // direct sun light added if the sun ray is clear
+    uint32_t inst;
+    uint32_t path;
+    uint16_t v[3];     // hit voxel (the start voxel is its neighbour across the face)
+    uint8_t face;
+    uint8_t pad;
+};
+static_assert(sizeof(QueueRec) == 64, "queue record is 64 bytes");
+
+__device__ __forceinline__ void put_rec(QueueRec* dst, const SurfaceRec& sf, const float4& dirw, const float4& contrib,
+                                        uint32_t path) {
+    const int fa = (int)(sf.face >> 1);
+    const int off = (sf.face & 1u) == 0u ? -1 : 1;
+    const uint32_t v0 = (uint32_t)(sf.vstart[0] - (fa == 0 ? off : 0));
+    const uint32_t v1 = (uint32_t)(sf.vstart[1] - (fa == 1 ? off : 0));
+    const uint32_t v2 = (uint32_t)(sf.vstart[2] - (fa == 2 ? off : 0));
+    uint4* d = reinterpret_cast<uint4*>(dst);
+    d[0] = make_uint4(__float_as_uint(sf.p[0]), __float_as_uint(sf.p[1]), __float_as_uint(sf.p[2]), __float_as_uint(sf.X[0]));
+    d[1] = make_uint4(__float_as_uint(sf.X[1]), __float_as_uint(sf.X[2]), __float_as_uint(dirw.x), __float_as_uint(dirw.y));
+    d[2] = make_uint4(__float_as_uint(dirw.z), __float_as_uint(contrib.x), __float_as_uint(contrib.y),
+                      __float_as_uint(contrib.z));
+    d[3] = make_uint4(sf.inst, path, (v0 & 0xFFFFu) | (v1 << 16), (v2 & 0xFFFFu) | (sf.face << 16));
+}
+
+__device__ __forceinline__ QueueRec get_rec(const QueueRec* src) {
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    const uint4 a = gld(s), b = gld(s + 1), c = gld(s + 2), d = gld(s + 3);
+    QueueRec r;
+    r.p[0] = __uint_as_float(a.x); r.p[1] = __uint_as_float(a.y); r.p[2] = __uint_as_float(a.z);
+    r.X[0] = __uint_as_float(a.w); r.X[1] = __uint_as_float(b.x); r.X[2] = __uint_as_float(b.y);
+    r.w[0] = __uint_as_float(b.z); r.w[1] = __uint_as_float(b.w); r.w[2] = __uint_as_float(c.x);
+    r.c[0] = __uint_as_float(c.y); r.c[1] = __uint_as_float(c.z); r.c[2] = __uint_as_float(c.w);
+    r.inst = d.x;
+    r.path = d.y;
+    r.v[0] = (uint16_t)(d.z & 0xFFFFu); r.v[1] = (uint16_t)(d.z >> 16); r.v[2] = (uint16_t)(d.w & 0xFFFFu);
+    r.face = (uint8_t)(d.w >> 16);
+    r.pad = 0;
+    return r;
+}
 
 __device__ void surface_of(const SceneGPU& S, const HitG& h, SurfaceRec& sf) {
     const InstanceGPU* I = &S.inst[h.inst];
@@ -800,7 +845,7 @@ __device__ void surface_of(const SceneGPU& S, const HitG& h, SurfaceRec& sf) {
 // walk per loop iteration), so lanes walking the origin and lanes walking another instance
 // share its instructions instead of running two copies of the walk one after the other.
 template <bool ST, bool DEEP>
-__device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const SurfaceRec& sf, V3 w, bool any_hit,
+__device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const QueueRec& sf, V3 w, bool any_hit,
                           Ctr<ST>& ctr, const Lds& stk_lds, const Lds& mslot, TreeLds tree) {
     ctr.ray();
     HitG best;
@@ -816,7 +861,9 @@ __device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const SurfaceRec
     // the instance to walk next and its entry (the origin: the surface point and start voxel)
     uint32_t inst = origin;
     float mpos[3] = {sf.p[0], sf.p[1], sf.p[2]};
-    int v0[3] = {sf.vstart[0], sf.vstart[1], sf.vstart[2]};
+    const int fa0 = (int)(sf.face >> 1), off0 = (sf.face & 1u) == 0u ? -1 : 1;
+    int v0[3] = {(int)sf.v[0] + (fa0 == 0 ? off0 : 0), (int)sf.v[1] + (fa0 == 1 ? off0 : 0),
+                 (int)sf.v[2] + (fa0 == 2 ? off0 : 0)};
     float tn = 0.0f;
     uint8_t face0 = (uint8_t)sf.face;   // face when the walk hits its start voxel
     V3 mdir;
@@ -1025,14 +1072,11 @@ __device__ __forceinline__ uint32_t append_slot(uint32_t* q, uint32_t cap) {
 
 // Wavefront buffers (device pointers), see WaveBuffers in vt_internal.h.
 struct WF {
-    float4* L;            // per path: accumulated radiance (sky and emission terms)
-    float4* Ls;           // per path: accumulated sun terms (written by sun rays only)
-    float4* beta;         // per path: throughput
-    uint32_t* rng;        // per path: RNG state
-    uint8_t* shadowed;    // per path: primary vertex shadowed
-    SurfaceRec* surf[2];  // per queue slot
-    float4* contrib[2];   // per slot: direct-light contribution if the sun ray is clear (w = 1: trace it)
-    float4* dir[2];       // per slot: bounce direction (w = 1: trace it)
+    float4* L;            // per path: sky terms; w = 1.0f bits (1u) if the primary ray hit
+    float4* Ls;           // per path with a primary hit: sun terms (sun rays only); w bits 1u if the
+                          // primary vertex is shadowed (facing away, or its sun ray blocked)
+    float4* br;           // per path with a primary hit: throughput (xyz), RNG state (w bits)
+    QueueRec* rec[2];     // per queue slot
     uint32_t* sun[2];     // sun list: the queue slots whose surfaces face the sun (one sun ray each)
     uint32_t* counters;   // kSlots x kParts counters (queue sizes and fetch counters)
     uint32_t seg_cap;     // slots per queue segment
@@ -1155,7 +1199,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         HitG h = query<ST, DEEP>(S, F, V3{F.O[0], F.O[1], F.O[2]}, D, ctr, stk, mslot, tree);
         float4 L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float beta[3] = {1.0f, 1.0f, 1.0f};
-        uint8_t shadowed = 0;
         if (!h.hit) {
             L = make_float4(1.0f * F.sky[0], 1.0f * F.sky[1], 1.0f * F.sky[2], 0.0f);
             if (smp == 0) {
@@ -1197,18 +1240,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
             float4 contrib, dirw;
             bool no_sun;
             shade(F, srgb_dec, sf, 0u, max_b, beta, rng, contrib, dirw, no_sun);
-            shadowed = no_sun ? 1 : 0;
             const uint32_t slot = append_slot(slot_counters(wf.counters, 0), wf.seg_cap);
-            wf.surf[0][slot] = sf;
-            wf.contrib[0][slot] = contrib;
-            wf.dir[0][slot] = dirw;
+            put_rec(&wf.rec[0][slot], sf, dirw, contrib, w);
             if (!no_sun) wf.sun[0][append_slot(slot_counters(wf.counters, sun_slot(0)), wf.seg_cap)] = slot;
+            // a hit path's state: sun terms (w: shadowed if facing away; its sun ray may set it),
+            // throughput and RNG state (only read by bounce rays)
+            L.w = __uint_as_float(1u);
+            wf.Ls[w] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(no_sun ? 1u : 0u));
+            if (max_b > 0) wf.br[w] = make_float4(beta[0], beta[1], beta[2], __uint_as_float(rng.s));
         }
         wf.L[w] = L;
-        wf.Ls[w] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        wf.beta[w] = make_float4(beta[0], beta[1], beta[2], 0.0f);
-        wf.rng[w] = rng.s;
-        wf.shadowed[w] = shadowed;
     });
     if (ST) ctr.flush(stats);
 }
@@ -1217,8 +1258,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
 // (count slots) and of its sun list (ns entries): items [0, nb) are the bounce rays of its slots
 // 0..nb-1 (nb = count when paths continue, else 0), items [nb, nb + ns) the sun rays of the
 // slots its sun list names (surfaces facing away from the sun have none, so no lane idles).  The two kinds
-// write different per-path state (bounce rays: L, beta, rng and the next queue; sun rays: Ls and
-// the shadow flag), so they run side by side and fill each other's tail.  in_idx / out_idx:
+// write different per-path state (bounce rays: L, throughput and RNG state, the next queue; sun
+// rays: Ls and the shadow flag), so they run side by side and fill each other's tail.  in_idx / out_idx:
 // counter slots of this depth's queue and the next one.  ONE: the batch holds one request (its
 // scene and frame are then wave-uniform, read with scalar loads).
 template <bool ST, bool DEEP, bool ONE>
@@ -1248,28 +1289,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         const uint32_t nb = bounce ? queue_count(qin, p, cap) : 0u;
         const bool is_bounce = item < nb;
         const uint32_t slot = is_bounce ? p * cap + item : wf.sun[q][p * cap + (item - nb)];
-        const float4 dw = is_bounce ? wf.dir[q][slot] : wf.contrib[q][slot];  // w = 1: trace it
-        if (dw.w == 0.0f) return;
-        const SurfaceRec sf = wf.surf[q][slot];
+        const QueueRec sf = get_rec(&wf.rec[q][slot]);
         const uint32_t path = sf.path;
         const ReqGPU& Rq = reqs[ONE ? 0u : find_req(B->work_base, nreq, path)];
         const SceneGPU S = Rq.S;
         const FrameGPU& F = Rq.F;
-        const V3 w = is_bounce ? V3{dw.x, dw.y, dw.z} : V3{F.sun[0], F.sun[1], F.sun[2]};
+        const V3 w = is_bounce ? V3{sf.w[0], sf.w[1], sf.w[2]} : V3{F.sun[0], F.sun[1], F.sun[2]};
         HitG h = secondary<ST, DEEP>(S, F, sf, w, !is_bounce, ctr, stk, mslot, tree);
         if (!is_bounce) {
             if (!h.hit) {
                 float4 L = wf.Ls[path];
-                L.x = L.x + dw.x;
-                L.y = L.y + dw.y;
-                L.z = L.z + dw.z;
+                L.x = L.x + sf.c[0];
+                L.y = L.y + sf.c[1];
+                L.z = L.z + sf.c[2];
                 wf.Ls[path] = L;
             } else if (depth == 0) {
-                wf.shadowed[path] = 1;
+                float4 L = wf.Ls[path];
+                L.w = __uint_as_float(1u);
+                wf.Ls[path] = L;
             }
             return;
         }
-        const float4 b4 = wf.beta[path];
+        const float4 b4 = wf.br[path];
         if (!h.hit) {
             float4 L = wf.L[path];
             L.x = L.x + b4.x * F.sky[0];
@@ -1282,20 +1323,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         uint8_t pal;
         ns.rgba = hit_rgba(S, h, &pal);
         surface_of(S, h, ns);
-        ns.path = path;
         float beta[3] = {b4.x, b4.y, b4.z};
-        Rng rng{wf.rng[path]};
+        Rng rng{__float_as_uint(b4.w)};
         float4 contrib, dirw;
         bool no_sun;
         const uint32_t max_b = F.mode == 2 ? F.max_bounces : 0u;
         shade(F, srgb_dec, ns, depth + 1u, max_b, beta, rng, contrib, dirw, no_sun);
         const uint32_t ns_slot = append_slot(slot_counters(wf.counters, out_idx), cap);
-        wf.surf[nq][ns_slot] = ns;
-        wf.contrib[nq][ns_slot] = contrib;
-        wf.dir[nq][ns_slot] = dirw;
+        put_rec(&wf.rec[nq][ns_slot], ns, dirw, contrib, path);
         if (!no_sun) wf.sun[nq][append_slot(slot_counters(wf.counters, sun_slot(out_idx)), cap)] = ns_slot;
-        wf.beta[path] = make_float4(beta[0], beta[1], beta[2], 0.0f);
-        wf.rng[path] = rng.s;
+        if (depth + 1u < max_b) wf.br[path] = make_float4(beta[0], beta[1], beta[2], __uint_as_float(rng.s));
     });
     if (ST) ctr.flush(stats);
 }
@@ -1318,7 +1355,9 @@ __device__ void accumulate_pixel(const ReqGPU* __restrict__ reqs, const BatchGPU
     }
     for (uint32_t s = Rq.s0; s < Rq.s1; ++s) {
         const size_t k = wb + tm.index(px, py, s, Rq.s0);
-        const float4 L = wf.L[k], Ls = wf.Ls[k];
+        const float4 L = wf.L[k];
+        // sun terms exist only for paths whose primary ray hit (L.w bits 1u)
+        const float4 Ls = __float_as_uint(L.w) != 0u ? wf.Ls[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         acc[0] += L.x + Ls.x;
         acc[1] += L.y + Ls.y;
         acc[2] += L.z + Ls.z;
@@ -1348,7 +1387,10 @@ __device__ void accumulate_pixel(const ReqGPU* __restrict__ reqs, const BatchGPU
             Rq.gacc[2 * p + 1] = make_float4(g[4], g[5], g[6], g[7]);
         }
     }
-    if (Rq.s0 == 0 && wf.shadowed[wb + tm.index(px, py, 0u, 0u)] && (Rq.hits[p].flags & 1u)) Rq.hits[p].flags |= 2u;
+    if (Rq.s0 == 0 && (Rq.hits[p].flags & 1u)) {
+        const size_t k0 = wb + tm.index(px, py, 0u, 0u);
+        if (__float_as_uint(wf.L[k0].w) != 0u && __float_as_uint(wf.Ls[k0].w) != 0u) Rq.hits[p].flags |= 2u;
+    }
 }
 
 __global__ void k_accumulate(const ReqGPU* __restrict__ reqs, const BatchGPU* __restrict__ B, WF wf) {
@@ -1457,7 +1499,7 @@ void vt_trace_set_persistent_blocks(uint32_t blocks) { g_persistent_blocks = blo
 
 size_t vt_wave_bytes(uint32_t npaths) {
     const size_t qcap = (size_t)kParts * seg_cap(npaths);
-    return (size_t)npaths * (16 + 16 + 16 + 4 + 1) + 2 * qcap * (64 + 16 + 16 + 4) + (size_t)kCounters * 4 + 18 * 256 +
+    return (size_t)npaths * (16 + 16 + 16) + 2 * qcap * (64 + 4) + (size_t)kCounters * 4 + 16 * 256 +
            sizeof(ReqGPU) * VT_MAX_BATCH + sizeof(BatchGPU);
 }
 
@@ -1471,14 +1513,10 @@ int vt_wave_bind(WaveBuffers& wb, void* mem, uint32_t npaths) {
     wb.npaths = npaths;
     wb.L = take((size_t)npaths * 16);
     wb.Ls = take((size_t)npaths * 16);
-    wb.beta = take((size_t)npaths * 16);
-    wb.rng = take((size_t)npaths * 4);
-    wb.shadowed = take((size_t)npaths);
+    wb.br = take((size_t)npaths * 16);
     const size_t qcap = (size_t)kParts * seg_cap(npaths);
     for (int q = 0; q < 2; ++q) {
-        wb.surf[q] = take(qcap * 64);
-        wb.contrib[q] = take(qcap * 16);
-        wb.dir[q] = take(qcap * 16);
+        wb.rec[q] = take(qcap * 64);
         wb.sun[q] = take(qcap * 4);
     }
     wb.counters = reinterpret_cast<uint32_t*>(take((size_t)kCounters * 4));
@@ -1490,13 +1528,9 @@ static WF to_wf(const WaveBuffers& wb) {
     WF w;
     w.L = reinterpret_cast<float4*>(wb.L);
     w.Ls = reinterpret_cast<float4*>(wb.Ls);
-    w.beta = reinterpret_cast<float4*>(wb.beta);
-    w.rng = reinterpret_cast<uint32_t*>(wb.rng);
-    w.shadowed = reinterpret_cast<uint8_t*>(wb.shadowed);
+    w.br = reinterpret_cast<float4*>(wb.br);
     for (int q = 0; q < 2; ++q) {
-        w.surf[q] = reinterpret_cast<SurfaceRec*>(wb.surf[q]);
-        w.contrib[q] = reinterpret_cast<float4*>(wb.contrib[q]);
-        w.dir[q] = reinterpret_cast<float4*>(wb.dir[q]);
+        w.rec[q] = reinterpret_cast<QueueRec*>(wb.rec[q]);
         w.sun[q] = reinterpret_cast<uint32_t*>(wb.sun[q]);
     }
     w.counters = wb.counters;

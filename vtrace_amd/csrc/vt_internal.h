@@ -55,14 +55,10 @@ int vt_lbvh_refit(LbvhWork& w, const float* records_dev, uint32_t n, const Volum
 // trace.hip: wavefront path state and queues (one block of device memory, see vt_wave_bind)
 struct WaveBuffers {
     uint32_t npaths = 0;        // capacity in pixel samples
-    void* L = nullptr;          // float4 per path
-    void* Ls = nullptr;         // float4 per path (sun terms)
-    void* beta = nullptr;       // float4 per path
-    void* rng = nullptr;        // u32 per path
-    void* shadowed = nullptr;   // u8 per path
-    void* surf[2] = {nullptr, nullptr};     // 64 B surface records per queue slot
-    void* contrib[2] = {nullptr, nullptr};  // float4 per slot
-    void* dir[2] = {nullptr, nullptr};      // float4 per slot
+    void* L = nullptr;          // float4 per path: sky terms, w = primary-hit flag
+    void* Ls = nullptr;         // float4 per path with a primary hit: sun terms, w = shadow flag
+    void* br = nullptr;         // float4 per path with a primary hit: throughput, RNG state
+    void* rec[2] = {nullptr, nullptr};      // 64 B queue records per queue slot
     void* sun[2] = {nullptr, nullptr};      // u32 per slot: sun list (queue slots facing the sun)
     uint32_t* counters = nullptr;           // 128 u32: queue sizes and fetch counters
     void* tables = nullptr;                 // ReqGPU[VT_MAX_BATCH] then BatchGPU
