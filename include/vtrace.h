@@ -194,6 +194,9 @@ typedef struct vt_serve_config {
     uint32_t streams;           /* an iteration of 2^18+ pixel samples is spread over up to this many
                                    streams (1..4, 0 = 1): sub-batches of about equal work, a request's
                                    samples in consecutive pieces accumulated in order */
+    uint32_t poll;              /* 1: vt_serve_step never waits for the GPU: it retires the iterations
+                                   that have finished and launches when one may start, so requests
+                                   submitted between calls join the next iteration */
 } vt_serve_config;
 
 /* round robin: every admitted request traces its next spp_chunk samples per iteration;

@@ -51,17 +51,18 @@ def _plain(r, recs, q):
     return r.radiance(), r.hits()
 
 
-@pytest.mark.parametrize("policy,pipeline", [(serving.ROUND_ROBIN, 1), (serving.HEAD_RIDERS, 2),
-                                             (serving.SHORTEST_FIRST, 2)])
-def test_mixed_batch_equals_plain_render(policy, pipeline):
+@pytest.mark.parametrize("policy,pipeline,poll", [(serving.ROUND_ROBIN, 1, False), (serving.HEAD_RIDERS, 2, False),
+                                                  (serving.SHORTEST_FIRST, 2, False), (serving.HEAD_RIDERS, 2, True),
+                                                  (serving.ROUND_ROBIN, 1, True)])
+def test_mixed_batch_equals_plain_render(policy, pipeline, poll):
     r = R.reset_renderer()
     _, recs = _setup(r)
     reqs = _requests()
     loop = serving.ServingLoop(r, max_slots=8, spp_chunk=2, memory_budget=64 << 20, max_work=1 << 15, tile_w=8,
-                               max_bounces=3, use_graph=False, policy=policy, pipeline=pipeline)
+                               max_bounces=3, use_graph=False, policy=policy, pipeline=pipeline, poll=poll)
     for rc in recs:
         loop.add_scene(rc)
-    _say(f"case: serving mixed batch of 5 vs plain render, policy {policy} pipeline {pipeline}")
+    _say(f"case: serving mixed batch of 5 vs plain render, policy {policy} pipeline {pipeline} poll {poll}")
     got = loop.run_all(reqs)
     loop.close()
     for i, q in enumerate(reqs):

@@ -3,9 +3,9 @@
 * Serving: requests of the serving workload's distribution (resolution from {256x256, 640x480,
   1280x720, 1920x1080}, spp 1..16, up to 4 bounces, over the 32 resident noise volumes of
   64^3..512^3) run through the loop as the benchmark runs it (hipGraph replay, four sub-batch
-  streams, head + riders, pipeline 2).  Every result equals the plain render_tick render bit
-  for bit, and bands of rows of a 1920x1080 request match the oracle (hits bit-exact,
-  radiance within 1e-5 relative).
+  streams, head + riders, pipeline 2, polling steps).  Every result equals the plain
+  render_tick render bit for bit, and bands of rows of a 1920x1080 request match the oracle
+  (hits bit-exact, radiance within 1e-5 relative).
 * Denoiser: vt_dn_denoise_frame on a 1920x1080 frame of the path-tracing benchmark scene
   (padded to 1088 rows inside the library) against the reference network, which rounds to
   BF16 at the same points, on tiles around blocks of the frame (corner, bottom edge with the
@@ -52,7 +52,8 @@ def test_serving_stream_at_its_workload():
     big = [i for i, q in enumerate(reqs) if (q.width, q.height) == (1920, 1080)]
     pick = next((i for i in big if dims[reqs[i].scene] == 512), big[0])
     loop = serving.ServingLoop(r, max_slots=32, spp_chunk=4, memory_budget=2 << 30, max_work=16 << 20, tile_w=4,
-                               max_bounces=4, use_graph=True, policy=serving.HEAD_RIDERS, pipeline=2, streams=4)
+                               max_bounces=4, use_graph=True, policy=serving.HEAD_RIDERS, pipeline=2, streams=4,
+                               poll=True)
     for rc in recs:
         loop.add_scene(rc)
     _say(f"case: serving 24 requests of the serving workload (largest {max(q.width * q.height for q in reqs)} px, "

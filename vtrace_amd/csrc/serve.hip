@@ -498,10 +498,18 @@ int32_t vt_serve_step(int64_t* completed, uint32_t max_completed) {
         ++sv.fl_count;
     }
     // 4. retire: keep at most pipeline - 1 iterations in flight after a launch; with nothing
-    // to launch, wait for the oldest one
+    // to launch, wait for the oldest one.  Polling: retire only what has finished, never wait.
     uint32_t n_done = 0;
-    if (sv.fl_count > 0 && (sv.fl_count >= c.pipeline || chunks.empty()))
+    if (c.poll) {
+        while (sv.fl_count > 0 && rc == 0) {
+            const hipError_t q = hipEventQuery(sv.fl[sv.fl_head].done);
+            if (q == hipErrorNotReady) break;
+            if (q != hipSuccess) rc = -1;
+            else n_done = retire_oldest(completed, max_completed, n_done, &rc);
+        }
+    } else if (sv.fl_count > 0 && (sv.fl_count >= c.pipeline || chunks.empty())) {
         n_done = retire_oldest(completed, max_completed, n_done, &rc);
+    }
     if (rc != 0) {
         vt_rt_set_error("vt_serve_step: iteration failed");
         return -1;

@@ -36,7 +36,8 @@ POLICY_NAMES = {ROUND_ROBIN: "round_robin", HEAD_RIDERS: "head_riders", SHORTEST
 class VtServeConfig(C.Structure):
     _fields_ = [("max_slots", C.c_uint32), ("spp_chunk", C.c_uint32), ("memory_budget", C.c_uint64),
                 ("max_work", C.c_uint32), ("tile_w", C.c_uint32), ("max_bounces", C.c_uint32),
-                ("use_graph", C.c_uint32), ("policy", C.c_uint32), ("pipeline", C.c_uint32), ("streams", C.c_uint32)]
+                ("use_graph", C.c_uint32), ("policy", C.c_uint32), ("pipeline", C.c_uint32), ("streams", C.c_uint32),
+                ("poll", C.c_uint32)]
 
 
 class VtRequest(C.Structure):
@@ -108,12 +109,12 @@ class Request:
 
 class ServingLoop:
     def __init__(self, renderer, max_slots=32, spp_chunk=4, memory_budget=2 << 30, max_work=16 << 20, tile_w=8,
-                 max_bounces=4, use_graph=True, policy=HEAD_RIDERS, pipeline=2, streams=4):
+                 max_bounces=4, use_graph=True, policy=HEAD_RIDERS, pipeline=2, streams=4, poll=False):
         self.r = renderer
         self.lib = renderer.lib
         declare(self.lib)
         cfg = VtServeConfig(max_slots, spp_chunk, memory_budget, max_work, tile_w, max_bounces, 1 if use_graph else 0,
-                            policy, pipeline, streams)
+                            policy, pipeline, streams, 1 if poll else 0)
         if self.lib.vt_serve_init(C.byref(cfg)) != 0:
             raise RendererError(f"vt_serve_init failed: {renderer.error()}")
         self.cfg = cfg
@@ -313,12 +314,13 @@ def default_tune_cache() -> str:
 def benchmark(renderer, nreq: int = 2000, util: float = 0.75, seed: int = 11, probe: int = 48, max_slots: int = 32,
               spp_chunk: int = 4, memory_budget: int = 2 << 30, max_work: int = 16 << 20, max_bounces: int = 4,
               use_graph: bool = True, policy: int = HEAD_RIDERS, pipeline: int = 2, streams: int = 4,
-              tune_cache: str | None = None,
+              poll: bool = True, over: float = 1.2, tune_cache: str | None = None,
               plain: bool = True, log=None) -> dict:
     """The serving workload: 32 resident noise volumes (64^3..512^3, fill 5-40%) and seeded Poisson
     streams of nreq requests.  Capacities (requests/s with a probe of requests all queued at
     once) are measured for the batching loop and for the plain path; each stream is then served
-    by both, at `util` of the loop's capacity and at `util` of the plain path's capacity.
+    by both, at `util` of the loop's capacity, at `util` of the plain path's capacity and at
+    `over` times the plain path's capacity (more than it can sustain).
     Latency = arrival -> radiance in host memory, wall clock."""
     say = log or (lambda *a: None)
     t0 = time.perf_counter()
@@ -334,7 +336,7 @@ def benchmark(renderer, nreq: int = 2000, util: float = 0.75, seed: int = 11, pr
     recs = scene_records(ids, dims)
     say(f"serving: 32 volumes generated in {t_gen:.1f} s, bricked in {t_up:.1f} s, {resident / 2**20:.0f} MiB resident")
     cfg = dict(max_slots=max_slots, spp_chunk=spp_chunk, memory_budget=memory_budget, max_work=max_work,
-               max_bounces=max_bounces, policy=policy, pipeline=pipeline, streams=streams)
+               max_bounces=max_bounces, policy=policy, pipeline=pipeline, streams=streams, poll=poll)
     probe_reqs = request_stream(probe, len(recs), dims, rate_hz=1.0, seed=seed + 1, max_bounces=max_bounces)
     tune = autotune_tile(renderer, probe_reqs[:12], recs, cache_path=tune_cache or default_tune_cache(),
                          key=f"serve32:{max_bounces}b:{POLICY_NAMES[policy]}:{max_work}:s{streams}", **cfg)
@@ -357,9 +359,9 @@ def benchmark(renderer, nreq: int = 2000, util: float = 0.75, seed: int = 11, pr
            "tile": tune["tile"], "resident_volume_mib": round(resident / 2**20, 1),
            "config": {"max_slots": max_slots, "memory_budget_gib": memory_budget / 2**30, "max_work": max_work,
                       "graph": bool(use_graph), "max_bounces": max_bounces, "policy": POLICY_NAMES[policy],
-                      "pipeline": pipeline, "sub_batch_streams": streams}, "streams": {}}
-    for tag, cap in (("at_batched_capacity", cap_loop), ("at_plain_capacity", cap_plain)):
-        rate = util * cap
+                      "pipeline": pipeline, "sub_batch_streams": streams, "poll": bool(poll)}, "streams": {}}
+    for tag, rate in (("at_batched_capacity", util * cap_loop), ("at_plain_capacity", util * cap_plain),
+                      ("above_plain_capacity", over * cap_plain)):
         reqs = request_stream(nreq, len(recs), dims, rate_hz=rate, seed=seed, max_bounces=max_bounces)
         st0 = loop.status()
         res = loop.run_stream(reqs)
