@@ -202,7 +202,7 @@ def summarize_trace(d) -> list:
         db = sqlite3.connect(path)
         try:
             for name, dur in db.execute("select name, duration from kernels"):
-                k = name.split("(")[0].replace("void ", "").replace("(anonymous namespace)::", "")
+                k = name.replace("(anonymous namespace)::", "").replace("void ", "", 1).split("(")[0]
                 out.setdefault(k, []).append(float(dur) * 1e-3)
         finally:
             db.close()

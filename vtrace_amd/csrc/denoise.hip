@@ -18,7 +18,9 @@
 //                    ReLU mask of the layer that produced X (X > 0).  Each wave computes 32
 //                    pixels x all output channels; A fragments (8 channels of one pixel at
 //                    one tap) are 16-byte loads from NHWC tensors, B fragments from the
-//                    [co][tap][ci] weight rows.
+//                    [co][tap][ci] weight rows.  k_conv3x3_tile, the same computation for
+//                    inputs of up to 96 channels, stages a 16 x 8 pixel tile with its halo
+//                    in LDS so the nine taps read LDS.
 //   k_conv3x3_wgrad  dW[co][tap][ci] = sum_p dY[p][co] X[p+tap][ci] over a chunk of pixels:
 //                    32-pixel tiles of dY and the shifted X are staged transposed in LDS so
 //                    both MFMA operands read 8 consecutive pixels; dbias in the same pass.
@@ -138,6 +140,130 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
                 else reinterpret_cast<uint16_t*>(a.out)[(size_t)p * a.out_stride + a.out_off + co] = f2bf(v);
             }
         }
+}
+
+// Tiled form of k_conv3x3 for inputs of at most kTileMaxCin channels: a workgroup (4 waves)
+// computes a kTileX x kTileY pixel tile of one image.  The input tile with a one-pixel halo is
+// staged in LDS once (zeros outside the image), so the nine taps of each pixel read LDS
+// instead of fetching the same 16-byte chunks again from L1/L2.  Each wave owns two rows of 16
+// pixels (two 16-row MFMA tiles) x all output channels and runs the same K chunks in the same
+// order as k_conv3x3 with the same operands, so its results are identical.
+constexpr int kTileX = 16, kTileY = 8;
+constexpr uint32_t kTileMaxCin = 96;
+
+// LDS row pitch (elements) of a staged 32-wide K slice of the weights: 80 B rows, so the 16
+// rows one MFMA operand reads start in different banks
+constexpr uint32_t kBPitch = 40;
+
+template <int NT, bool RELU, bool OUTF, bool MASK>
+__global__ __launch_bounds__(256) void k_conv3x3_tile(ConvArgs a) {
+    extern __shared__ __align__(16) uint16_t tile[];  // (kTileY + 2) x (kTileX + 2) pixels x cs, then
+                                                      // 2 x NT*16 rows x kBPitch weight slices
+    const uint32_t cs = a.Cin + 8u;                   // LDS pixel stride (16 B of padding)
+    uint16_t* const bsl = tile + (size_t)((kTileY + 2) * (kTileX + 2)) * cs;
+    const uint32_t n = blockIdx.z;
+    const int tx0 = (int)blockIdx.x * kTileX, ty0 = (int)blockIdx.y * kTileY;
+    const size_t img = (size_t)n * a.H * a.W;
+    const uint32_t cpp = a.Cin / 8u;
+    const uint32_t total = (uint32_t)((kTileY + 2) * (kTileX + 2)) * cpp;
+    for (uint32_t i = threadIdx.x; i < total; i += 256u) {
+        const uint32_t pix = i / cpp, c8 = i - pix * cpp;
+        const int ly = (int)(pix / (uint32_t)(kTileX + 2)), lx = (int)(pix % (uint32_t)(kTileX + 2));
+        const int y = ty0 + ly - 1, x = tx0 + lx - 1;
+        u32x4 v = u32x4{0u, 0u, 0u, 0u};
+        if (y >= 0 && y < (int)a.H && x >= 0 && x < (int)a.W)
+            v = *reinterpret_cast<const u32x4*>(a.in + (img + (size_t)y * a.W + (size_t)x) * a.in_stride + 8u * c8);
+        *reinterpret_cast<u32x4*>(tile + (size_t)pix * cs + 8u * c8) = v;
+    }
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t g = lane >> 4, r = lane & 15u;
+    f32x4 acc[2][NT];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[m][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const uint32_t K = 9u * a.Cin;
+    // weight slices: slice s holds rows co < NT*16, k = 32s .. 32s+31 (zeros past Cout or K);
+    // piece q of a slice is 16 B: row q / 4, k offset 8 (q % 4)
+    constexpr uint32_t kPieces = (uint32_t)NT * 64u;
+    auto fetch = [&](uint32_t k0, uint32_t q) -> u32x4 {
+        const uint32_t co = q >> 2, kk = k0 + 8u * (q & 3u);
+        return (co < a.Cout && kk < K) ? *reinterpret_cast<const u32x4*>(a.w + (size_t)co * K + kk)
+                                       : u32x4{0u, 0u, 0u, 0u};
+    };
+    auto store = [&](uint32_t buf, uint32_t q, const u32x4& v) {
+        *reinterpret_cast<u32x4*>(bsl + (size_t)buf * (NT * 16u * kBPitch) + (q >> 2) * kBPitch + 8u * (q & 3u)) = v;
+    };
+    for (uint32_t q = threadIdx.x; q < kPieces; q += 256u) store(0, q, fetch(0, q));
+    __syncthreads();
+    uint32_t buf = 0;
+    for (uint32_t k0 = 0; k0 < K; k0 += 32u) {
+        // prefetch the next slice into registers while this one is used
+        u32x4 nx[(kPieces + 255u) / 256u];
+        const bool more = k0 + 32u < K;
+#pragma unroll
+        for (uint32_t t = 0; t < (kPieces + 255u) / 256u; ++t) {
+            const uint32_t q = threadIdx.x + 256u * t;
+            nx[t] = (more && q < kPieces) ? fetch(k0 + 32u, q) : u32x4{0u, 0u, 0u, 0u};
+        }
+        const uint32_t k = k0 + 8u * g;
+        const bool kv = k < K;
+        uint32_t tap = 0, ci = 0;
+        if (kv) {
+            tap = k / a.Cin;
+            ci = k - tap * a.Cin;
+        }
+        const int dy = (int)(tap / 3u) - 1, dx = (int)(tap % 3u) - 1;
+        bf16x8 A[2];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const int ly = (int)(2u * wave) + m + 1 + dy, lx = (int)r + 1 + dx;
+            A[m] = kv ? __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
+                            tile + (size_t)(ly * (kTileX + 2) + lx) * cs + ci))
+                      : zero8();
+        }
+        const uint16_t* bs = bsl + (size_t)buf * (NT * 16u * kBPitch);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const bf16x8 B = __builtin_bit_cast(
+                bf16x8, *reinterpret_cast<const u32x4*>(bs + ((uint32_t)j * 16u + r) * kBPitch + 8u * g));
+            acc[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B, acc[0][j], 0, 0, 0);
+            acc[1][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B, acc[1][j], 0, 0, 0);
+        }
+        if (more) {
+#pragma unroll
+            for (uint32_t t = 0; t < (kPieces + 255u) / 256u; ++t) {
+                const uint32_t q = threadIdx.x + 256u * t;
+                if (q < kPieces) store(buf ^ 1u, q, nx[t]);
+            }
+        }
+        __syncthreads();
+        buf ^= 1u;
+    }
+    // D layout: lane holds column r (output channel) of rows 4g..4g+3 (pixels 4g.. of its row)
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const int y = ty0 + (int)(2u * wave) + m;
+        if (y >= (int)a.H) continue;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const uint32_t co = (uint32_t)j * 16u + r;
+            if (co >= a.Cout) continue;
+            const float b = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int x = tx0 + (int)(4u * g) + i;
+                if (x >= (int)a.W) continue;
+                const size_t p = img + (size_t)y * a.W + (size_t)x;
+                float v = acc[m][j][i] + b;
+                if (RELU) v = v > 0.0f ? v : 0.0f;
+                if (MASK && !(bf2f(a.mask[p * a.mask_stride + co]) > 0.0f)) v = 0.0f;
+                if (OUTF) reinterpret_cast<float*>(a.out)[p * a.out_stride + a.out_off + co] = v;
+                else reinterpret_cast<uint16_t*>(a.out)[p * a.out_stride + a.out_off + co] = f2bf(v);
+            }
+        }
+    }
 }
 
 // ---- weight gradient ----------------------------------------------------------------------
@@ -696,6 +822,19 @@ int ensure_grad(size_t P0) {
 // ---- launches ----------------------------------------------------------------------------
 template <bool RELU, bool OUTF, bool MASK>
 int launch_conv_nt(uint32_t nt, const ConvArgs& a, hipStream_t s) {
+    if (a.Cin <= kTileMaxCin && a.Cin % 8u == 0 && a.in_stride % 8u == 0 && nt <= 6) {
+        const dim3 tg((a.W + kTileX - 1) / kTileX, (a.H + kTileY - 1) / kTileY, a.N), tb(256);
+        const size_t lds = ((size_t)(kTileY + 2) * (kTileX + 2) * (a.Cin + 8u) + 2u * nt * 16u * kBPitch) * 2u;
+        switch (nt) {
+            case 1: hipLaunchKernelGGL((k_conv3x3_tile<1, RELU, OUTF, MASK>), tg, tb, lds, s, a); return 0;
+            case 2: hipLaunchKernelGGL((k_conv3x3_tile<2, RELU, OUTF, MASK>), tg, tb, lds, s, a); return 0;
+            case 3: hipLaunchKernelGGL((k_conv3x3_tile<3, RELU, OUTF, MASK>), tg, tb, lds, s, a); return 0;
+            case 4: hipLaunchKernelGGL((k_conv3x3_tile<4, RELU, OUTF, MASK>), tg, tb, lds, s, a); return 0;
+            case 5: hipLaunchKernelGGL((k_conv3x3_tile<5, RELU, OUTF, MASK>), tg, tb, lds, s, a); return 0;
+            case 6: hipLaunchKernelGGL((k_conv3x3_tile<6, RELU, OUTF, MASK>), tg, tb, lds, s, a); return 0;
+            default: break;
+        }
+    }
     const dim3 grid(blocks((size_t)a.N * a.H * a.W, kConvPix)), blk(256);
     switch (nt) {
         case 1: hipLaunchKernelGGL((k_conv3x3<1, RELU, OUTF, MASK>), grid, blk, 0, s, a); break;
