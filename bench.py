@@ -146,6 +146,27 @@ def calibrate_copy(nbytes: int = 256 << 20):
     print(json.dumps({"calibration_copy_bytes": nbytes}), flush=True)
 
 
+def denoise_frames(args):
+    """Only the denoiser: W untimed, then K timed vt_dn_denoise_frame passes (device time each)
+    over one frame of the path-tracing scene with its G-buffer; random-init network weights."""
+    from vtrace_amd import denoise
+    from vtrace_amd import renderer as R
+    from vtrace_amd import scenes
+    sc = scenes.config2(args.width, args.height, spp=1)
+    r = R.get_renderer()
+    R.load_scene(r, sc)
+    d = denoise.Denoiser(r, seed=5)
+    r.enable_gbuffer(True)
+    r.render(sc.P, sc.C)
+    for _ in range(args.warmup):
+        d.denoise_frame()
+    ms = [d.denoise_frame()[1] for _ in range(args.denoise_frames)]
+    d.close()
+    print(json.dumps({"denoise_frame_ms": sorted(ms)[len(ms) // 2], "all_ms": [round(m, 4) for m in ms],
+                      "width": sc.params.width, "height": sc.params.height, "params": d.real_param_count}),
+          flush=True)
+
+
 def _kernel_key(name: str):
     """'void (anonymous namespace)::k_secondary<false, false, true>(...)' -> ('k_secondary', False);
     the flag is True for the counting instantiations (first template argument true)."""
@@ -300,6 +321,9 @@ def main():
     ap.add_argument("--denoiser-iters", type=int, default=24000,
                     help="training iterations of the denoiser (0 skips the denoiser measurement)")
     ap.add_argument("--streaming", type=int, default=1, help="1: streamed terrain measurement, 0: skip it")
+    ap.add_argument("--denoise-frames", type=int, default=0,
+                    help="run only the denoiser: this many 1920x1080 inference passes over a frame of the "
+                         "path-tracing scene (random-init network), for profiling")
     ap.add_argument("--single-stream", action="store_true",
                     help="render every frame on one stream (kernel traces then match the stage table)")
     ap.add_argument("--summarize-profiles", action="store_true",
@@ -321,6 +345,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.denoise_frames > 0:
+        denoise_frames(args)
         return
 
     from vtrace_amd import renderer as R

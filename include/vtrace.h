@@ -137,7 +137,12 @@ int32_t vt_render_into(const render_tick_info* info, void* hits_dev, uint64_t hi
                        uint64_t radiance_bytes);
 
 /* Read-back of the last frame (synchronises).  bytes must be >= width*height*16 (hits and
- * radiance) or width*height*4 (B8G8R8A8 sRGB, the reference's swapchain format). */
+ * radiance) or width*height*4 (B8G8R8A8 sRGB, the reference's swapchain format).
+ * Hit record (16 B per pixel, sample 0): u32 instance (0xFFFFFFFF: sky); u32 bits 0-9 of the
+ * hit voxel's x | y << 10 | z << 20; u16 DDA steps, saturated at 65535; u8 face (0..5 =
+ * -X,+X,-Y,+Y,-Z,+Z outward normal, 0xFF miss); u8 palette index; u8 last step's axis mask;
+ * u8 flags (bit 0 hit, bit 1 shadowed, bit 2 a voxel coordinate >= 32768 of which only the
+ * low 15 bits are held, bit 3 steps saturated); u16 bits 10-14 of x | y << 5 | z << 10. */
 int32_t vt_read_hits(void* dst, uint64_t bytes);
 int32_t vt_read_radiance(float* dst, uint64_t bytes);
 int32_t vt_read_bgra8(uint32_t* dst, uint64_t bytes);

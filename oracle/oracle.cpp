@@ -79,13 +79,14 @@ typedef struct orc_params {
 
 typedef struct orc_hit {
     uint32_t inst;    // instance index (gl_InstanceIndex), 0xFFFFFFFF for sky
-    uint32_t voxel;   // x | y << 10 | z << 20
-    uint16_t steps;   // DDA steps taken in the winning instance
+    uint32_t voxel;   // bits 0-9 of x | y << 10 | z << 20
+    uint16_t steps;   // DDA steps taken in the winning instance, saturated at 65535 (flags bit 3)
     uint8_t face;     // 0..5 = -X,+X,-Y,+Y,-Z,+Z outward normal of the hit face; 0xFF miss
     uint8_t pal;      // palette index of the hit texel (fmt 0), 0 for RGBA8 volumes
     uint8_t mask;     // last DDA step mask (bit0 x, bit1 y, bit2 z), 0 if steps == 0
-    uint8_t flags;    // bit0 hit, bit1 primary vertex shadowed (mode >= 1)
-    uint16_t pad;
+    uint8_t flags;    // bit0 hit, bit1 primary vertex shadowed (mode >= 1), bit2 a voxel
+                      // coordinate >= 32768 (low 15 bits held), bit3 steps saturated
+    uint16_t vhi;     // bits 10-14 of x | y << 5 | z << 10
 } orc_hit;
 
 typedef struct orc_stats {
@@ -1037,13 +1038,18 @@ void write_hit(const Hit& h, orc_hit& out) {
         out.face = 0xFF;
         return;
     }
+    // 15-bit coordinates split 10 + 5 over voxel and vhi; steps saturate instead of wrapping
+    const uint32_t x = h.d.v[0], y = h.d.v[1], z = h.d.v[2];
     out.inst = h.inst;
-    out.voxel = (uint32_t)h.d.v[0] | ((uint32_t)h.d.v[1] << 10) | ((uint32_t)h.d.v[2] << 20);
-    out.steps = (uint16_t)h.d.steps;
+    out.voxel = (x % 1024u) + (y % 1024u) * 1024u + (z % 1024u) * 1048576u;
+    out.vhi = (uint16_t)((x / 1024u) % 32u + ((y / 1024u) % 32u) * 32u + ((z / 1024u) % 32u) * 1024u);
+    out.steps = (uint16_t)std::min<uint32_t>(h.d.steps, 65535u);
     out.face = h.face;
     out.pal = h.d.pal;
     out.mask = (uint8_t)h.d.mask;
     out.flags = 1;
+    if (x >= 32768u || y >= 32768u || z >= 32768u) out.flags |= 4;
+    if (h.d.steps > 65535u) out.flags |= 8;
 }
 
 }  // namespace

@@ -34,7 +34,7 @@ class OrcStats(C.Structure):
 
 
 HIT_DTYPE = np.dtype([("inst", "<u4"), ("voxel", "<u4"), ("steps", "<u2"), ("face", "u1"), ("pal", "u1"),
-                      ("mask", "u1"), ("flags", "u1"), ("pad", "<u2")])
+                      ("mask", "u1"), ("flags", "u1"), ("vhi", "<u2")])
 assert HIT_DTYPE.itemsize == 16
 
 _lib = None

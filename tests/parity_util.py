@@ -8,7 +8,7 @@ import numpy as np
 import oracle_ffi as O
 from vtrace_amd import renderer as R
 
-HIT_FIELDS = ("inst", "voxel", "steps", "face", "pal", "mask", "flags")
+HIT_FIELDS = ("inst", "voxel", "steps", "face", "pal", "mask", "flags", "vhi")
 
 
 def gpu_render(scene, stats=False, params=None):

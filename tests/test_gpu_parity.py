@@ -185,7 +185,7 @@ def test_gbuffer_consistent_with_hits():
         inst = int(h["inst"])
         rec = np.asarray(sc.records[inst], np.float32).copy()
         vol = sc.volumes[int(rec.view(np.uint32)[15])]
-        vx, vy, vz = int(h["voxel"]) & 1023, (int(h["voxel"]) >> 10) & 1023, (int(h["voxel"]) >> 20) & 1023
+        vx, vy, vz = (int(c) for c in R.hit_voxels(h))
         rgba = int(vol.palette[int(h["pal"])])
         assert np.array_equal(gb[y, x, :3], dec[[rgba & 255, (rgba >> 8) & 255, (rgba >> 16) & 255]])
         face = int(h["face"])

@@ -1204,7 +1204,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
             if (smp == 0) {
                 HitRecord rec;
                 rec.inst = VT_EMPTY; rec.voxel = VT_EMPTY; rec.steps = 0; rec.face = 0xFF;
-                rec.pal = 0; rec.mask = 0; rec.flags = 0; rec.pad = 0;
+                rec.pal = 0; rec.mask = 0; rec.flags = 0; rec.vhi = 0;
                 Rq.hits[pix] = rec;
             }
             if (Rq.gbuf && wf.gsamp) {
@@ -1219,13 +1219,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
             if (smp == 0) {
                 HitRecord rec;
                 rec.inst = h.inst;
-                rec.voxel = (uint32_t)h.d.v[0] | ((uint32_t)h.d.v[1] << 10) | ((uint32_t)h.d.v[2] << 20);
-                rec.steps = (uint16_t)h.d.steps;
+                const uint32_t fl = vt_pack_hit_voxel(rec, h.d.v[0], h.d.v[1], h.d.v[2], h.d.steps);
                 rec.face = h.face;
                 rec.pal = pal;
                 rec.mask = (uint8_t)h.d.mask;
-                rec.flags = 1;
-                rec.pad = 0;
+                rec.flags = (uint8_t)(1u | fl);
                 Rq.hits[pix] = rec;
             }
             surface_of(S, h, sf);
@@ -1426,7 +1424,7 @@ __global__ __launch_bounds__(kBlock) void k_render0(const FrameGPU F, const Scen
         float acca = 0.0f;
         HitRecord rec;
         rec.inst = VT_EMPTY; rec.voxel = VT_EMPTY; rec.steps = 0; rec.face = 0xFF;
-        rec.pal = 0; rec.mask = 0; rec.flags = 0; rec.pad = 0;
+        rec.pal = 0; rec.mask = 0; rec.flags = 0; rec.vhi = 0;
         for (uint32_t smp = 0; smp < spp; ++smp) {
             Rng rng{pixel_seed(F.seed, F.frame, pix, smp)};
             float jx = 0.5f, jy = 0.5f;
@@ -1443,12 +1441,11 @@ __global__ __launch_bounds__(kBlock) void k_render0(const FrameGPU F, const Scen
                 const uint32_t rgba = hit_rgba(S, h, &pal);
                 if (smp == 0) {
                     rec.inst = h.inst;
-                    rec.voxel = (uint32_t)h.d.v[0] | ((uint32_t)h.d.v[1] << 10) | ((uint32_t)h.d.v[2] << 20);
-                    rec.steps = (uint16_t)h.d.steps;
+                    const uint32_t fl = vt_pack_hit_voxel(rec, h.d.v[0], h.d.v[1], h.d.v[2], h.d.steps);
                     rec.face = h.face;
                     rec.pal = pal;
                     rec.mask = (uint8_t)h.d.mask;
-                    rec.flags = 1;
+                    rec.flags = (uint8_t)(1u | fl);
                 }
                 const float a = (float)(rgba >> 24) / 255.0f;
                 const float rgb[3] = {srgb_dec[rgba & 255u], srgb_dec[(rgba >> 8) & 255u], srgb_dec[(rgba >> 16) & 255u]};

@@ -73,14 +73,26 @@ struct BvhNode {                // 32 B
 
 struct HitRecord {              // 16 B, one per pixel
     uint32_t inst;              // instance index or 0xFFFFFFFF
-    uint32_t voxel;             // x | y << 10 | z << 20
-    uint16_t steps;
+    uint32_t voxel;             // bits 0-9 of x | y << 10 | z << 20
+    uint16_t steps;             // DDA steps, saturated at 65535 (flags bit 3)
     uint8_t face;               // 0..5 = -X,+X,-Y,+Y,-Z,+Z (outward normal), 0xFF miss
     uint8_t pal;                // palette index (u8 volumes)
     uint8_t mask;               // last DDA step mask
-    uint8_t flags;              // bit0 hit, bit1 primary vertex shadowed
-    uint16_t pad;
+    uint8_t flags;              // bit0 hit, bit1 primary vertex shadowed, bit2 a voxel coordinate
+                                // >= 32768 (only its low 15 bits held), bit3 steps saturated
+    uint16_t vhi;               // bits 10-14 of x | y << 5 | z << 10 (zero below 1024)
 };
+
+// Voxel and step fields of a hit record; returns the flag bits 2-3 they need.
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline uint32_t vt_pack_hit_voxel(HitRecord& r, uint32_t x, uint32_t y, uint32_t z, uint32_t steps) {
+    r.voxel = (x & 1023u) | ((y & 1023u) << 10) | ((z & 1023u) << 20);
+    r.vhi = (uint16_t)(((x >> 10) & 31u) | (((y >> 10) & 31u) << 5) | (((z >> 10) & 31u) << 10));
+    r.steps = (uint16_t)(steps < 65535u ? steps : 65535u);
+    return ((x | y | z) >= 32768u ? 4u : 0u) | (steps > 65535u ? 8u : 0u);
+}
 
 // Per-frame constants, passed by value to the render kernels.
 struct FrameGPU {

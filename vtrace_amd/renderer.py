@@ -24,7 +24,18 @@ from ._native import render_lib
 from .abi import RenderTickInfo, VtRenderParams
 
 HIT_DTYPE = np.dtype([("inst", "<u4"), ("voxel", "<u4"), ("steps", "<u2"), ("face", "u1"), ("pal", "u1"),
-                      ("mask", "u1"), ("flags", "u1"), ("pad", "<u2")])
+                      ("mask", "u1"), ("flags", "u1"), ("vhi", "<u2")])
+
+
+def hit_voxels(hits: np.ndarray) -> np.ndarray:
+    """(..., 3) voxel coordinates of hit records: bits 0-9 from `voxel`, bits 10-14 from `vhi`.
+    A coordinate of 32768 or more (flags bit 2) comes back as its low 15 bits."""
+    v = hits["voxel"].astype(np.int64)
+    hi = hits["vhi"].astype(np.int64)
+    return np.stack([(v & 1023) | ((hi & 31) << 10), ((v >> 10) & 1023) | (((hi >> 5) & 31) << 10),
+                     ((v >> 20) & 1023) | (((hi >> 10) & 31) << 10)], axis=-1)
+
+
 INSTANCE_DTYPE = np.dtype([("M", "<f4", 12), ("inv", "<f4", 12), ("tex", "<u4"), ("valid", "<u4"),
                            ("size", "<f4", 3), ("dim", "<u4", 3), ("pad", "<u4", 2)])
 NODE_DTYPE = np.dtype([("lo", "<f4", 3), ("hi", "<f4", 3), ("left", "<u4"), ("right", "<u4")])
