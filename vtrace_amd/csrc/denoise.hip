@@ -18,9 +18,15 @@
 //                    ReLU mask of the layer that produced X (X > 0).  Each wave computes 32
 //                    pixels x all output channels; A fragments (8 channels of one pixel at
 //                    one tap) are 16-byte loads from NHWC tensors, B fragments from the
-//                    [co][tap][ci] weight rows.  k_conv3x3_tile, the same computation for
-//                    inputs of up to 96 channels, stages a 16 x 8 pixel tile with its halo
-//                    in LDS so the nine taps read LDS.
+//                    [co][tap][ci] weight rows.  Weights are the first MFMA operand, so a
+//                    lane's outputs are 4 consecutive channels of one pixel.  For inputs of
+//                    up to 96 channels the same computation runs from LDS:
+//                    k_conv3x3_res (persistent workgroups of 8 waves with the layer's weights
+//                    resident in LDS, 16 x 16 pixel tiles with their halo, the next tile
+//                    fetched into registers during the current one, a software-pipelined K
+//                    loop with no barrier) where weights and a tile fit in 160 KiB, else
+//                    k_conv3x3_tile (16 x 8 tiles, weights streamed through LDS in 32-wide K
+//                    slices).
 //   k_conv3x3_wgrad  dW[co][tap][ci] = sum_p dY[p][co] X[p+tap][ci] over a chunk of pixels:
 //                    32-pixel tiles of dY and the shifted X are staged transposed in LDS so
 //                    both MFMA operands read 8 consecutive pixels; dbias in the same pass.
@@ -55,6 +61,13 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
     return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint32_t b) { return __uint_as_float((b & 0xFFFFu) << 16); }
+// two floats to packed BF16 (lo in bits 0-15), round to nearest even: f2bf's result for finite
+// values, in one v_cvt_pk_bf16_f32
+__device__ __forceinline__ uint32_t pk2bf(float lo, float hi) {
+    typedef float f32x2v __attribute__((ext_vector_type(2)));
+    typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2v{lo, hi}), bf16x2v));
+}
 __device__ __forceinline__ bf16x8 ld8(const uint16_t* p) {
     return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(p));
 }
@@ -117,150 +130,427 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
         for (int j = 0; j < NT; ++j) {
             const uint32_t co = (uint32_t)j * 16u + r;
             const bf16x8 B = (kv && co < a.Cout) ? ld8(a.w + (size_t)co * K + k) : zero8();
-            acc[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B, acc[0][j], 0, 0, 0);
-            acc[1][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B, acc[1][j], 0, 0, 0);
+            // weights as the first operand: D rows are output channels, columns pixels
+            acc[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B, A[0], acc[0][j], 0, 0, 0);
+            acc[1][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B, A[1], acc[1][j], 0, 0, 0);
         }
     }
-    // D layout: lane holds column r (output channel) of rows 4g..4g+3 (pixels)
+    // D layout: lane holds output channels 16j + 4g .. 16j + 4g + 3 of pixel r of tile m
 #pragma unroll
-    for (int m = 0; m < 2; ++m)
+    for (int m = 0; m < 2; ++m) {
+        const uint32_t p = pbase + (uint32_t)m * 16u + r;
+        if (p >= P) continue;
 #pragma unroll
-        for (int j = 0; j < NT; ++j) {
-            const uint32_t co = (uint32_t)j * 16u + r;
-            if (co >= a.Cout) continue;
-            const float b = a.bias ? a.bias[co] : 0.0f;
+        for (int j = 0; j < NT; ++j)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const uint32_t p = pbase + (uint32_t)m * 16u + 4u * g + (uint32_t)i;
-                if (p >= P) continue;
-                float v = acc[m][j][i] + b;
+                const uint32_t co = (uint32_t)j * 16u + 4u * g + (uint32_t)i;
+                if (co >= a.Cout) continue;
+                float v = acc[m][j][i] + (a.bias ? a.bias[co] : 0.0f);
                 if (RELU) v = v > 0.0f ? v : 0.0f;
                 if (MASK && !(bf2f(a.mask[(size_t)p * a.mask_stride + co]) > 0.0f)) v = 0.0f;
                 if (OUTF) reinterpret_cast<float*>(a.out)[(size_t)p * a.out_stride + a.out_off + co] = v;
                 else reinterpret_cast<uint16_t*>(a.out)[(size_t)p * a.out_stride + a.out_off + co] = f2bf(v);
             }
-        }
+    }
 }
 
 // Tiled form of k_conv3x3 for inputs of at most kTileMaxCin channels: a workgroup (4 waves)
-// computes a kTileX x kTileY pixel tile of one image.  The input tile with a one-pixel halo is
+// computes a kTileX x (4 MT) pixel tile of one image.  The input tile with a one-pixel halo is
 // staged in LDS once (zeros outside the image), so the nine taps of each pixel read LDS
-// instead of fetching the same 16-byte chunks again from L1/L2.  Each wave owns two rows of 16
-// pixels (two 16-row MFMA tiles) x all output channels and runs the same K chunks in the same
-// order as k_conv3x3 with the same operands, so its results are identical.
-constexpr int kTileX = 16, kTileY = 8;
+// instead of fetching the same 16-byte chunks again from L1/L2.  Each wave owns MT rows of 16
+// pixels (MT 16-row MFMA tiles) x all output channels, so every weight fragment it reads feeds
+// MT MFMAs.  The weights pass through LDS in 32-wide K slices, double-buffered: each slice is
+// fetched from global memory into registers while the previous one feeds the MFMAs.  K chunks,
+// operands and their order are those of k_conv3x3 (steps past 9 Cin add zero products), so
+// every output is the same sum in the same order.
+constexpr int kTileX = 16;
 constexpr uint32_t kTileMaxCin = 96;
 
 // LDS row pitch (elements) of a staged 32-wide K slice of the weights: 80 B rows, so the 16
 // rows one MFMA operand reads start in different banks
 constexpr uint32_t kBPitch = 40;
 
-template <int NT, bool RELU, bool OUTF, bool MASK>
+template <int NT>
+inline size_t conv_tile_lds(int mt, uint32_t cin) {
+    return ((size_t)(4 * mt + 2) * (kTileX + 2) * (cin + 8u) + 2u * NT * 16u * kBPitch) * 2u;
+}
+
+template <int MT, int NT, bool RELU, bool OUTF, bool MASK>
 __global__ __launch_bounds__(256) void k_conv3x3_tile(ConvArgs a) {
-    extern __shared__ __align__(16) uint16_t tile[];  // (kTileY + 2) x (kTileX + 2) pixels x cs, then
+    constexpr int TY = 4 * MT;
+    extern __shared__ __align__(16) uint16_t tile[];  // (TY + 2) x (kTileX + 2) pixels x cs, then
                                                       // 2 x NT*16 rows x kBPitch weight slices
     const uint32_t cs = a.Cin + 8u;                   // LDS pixel stride (16 B of padding)
-    uint16_t* const bsl = tile + (size_t)((kTileY + 2) * (kTileX + 2)) * cs;
+    uint16_t* const bsl = tile + (size_t)((TY + 2) * (kTileX + 2)) * cs;
     const uint32_t n = blockIdx.z;
-    const int tx0 = (int)blockIdx.x * kTileX, ty0 = (int)blockIdx.y * kTileY;
+    const int tx0 = (int)blockIdx.x * kTileX, ty0 = (int)blockIdx.y * TY;
     const size_t img = (size_t)n * a.H * a.W;
-    const uint32_t cpp = a.Cin / 8u;
-    const uint32_t total = (uint32_t)((kTileY + 2) * (kTileX + 2)) * cpp;
-    for (uint32_t i = threadIdx.x; i < total; i += 256u) {
-        const uint32_t pix = i / cpp, c8 = i - pix * cpp;
-        const int ly = (int)(pix / (uint32_t)(kTileX + 2)), lx = (int)(pix % (uint32_t)(kTileX + 2));
-        const int y = ty0 + ly - 1, x = tx0 + lx - 1;
-        u32x4 v = u32x4{0u, 0u, 0u, 0u};
-        if (y >= 0 && y < (int)a.H && x >= 0 && x < (int)a.W)
-            v = *reinterpret_cast<const u32x4*>(a.in + (img + (size_t)y * a.W + (size_t)x) * a.in_stride + 8u * c8);
-        *reinterpret_cast<u32x4*>(tile + (size_t)pix * cs + 8u * c8) = v;
+    // cpp + 1 chunks of 16 B per pixel: its channels, then its padding, set to zeros (K past
+    // the end reads the padding, so no select is needed)
+    const uint32_t cpp = a.Cin / 8u + 1u;
+    const uint32_t total = (uint32_t)((TY + 2) * (kTileX + 2)) * cpp;
+    // i / cpp as (i * inv) >> 20: exact for i < 2^20 / 13 (cpp <= 13, i < 4212)
+    const uint32_t inv = ((1u << 20) + cpp - 1u) / cpp;
+    // staging in batches: all loads of a batch are issued before its LDS stores, so a thread
+    // waits for one round trip per batch instead of one per 16-byte chunk
+    constexpr uint32_t kBatch = 8;
+    for (uint32_t i0 = 0; i0 < total; i0 += 256u * kBatch) {
+        u32x4 v[kBatch];
+#pragma unroll
+        for (uint32_t b = 0; b < kBatch; ++b) {
+            const uint32_t i = i0 + threadIdx.x + 256u * b;
+            const uint32_t pix = (i * inv) >> 20, c8 = i - pix * cpp;
+            const int ly = (int)(pix / (uint32_t)(kTileX + 2)), lx = (int)(pix % (uint32_t)(kTileX + 2));
+            const int y = ty0 + ly - 1, x = tx0 + lx - 1;
+            v[b] = u32x4{0u, 0u, 0u, 0u};
+            if (i < total && c8 + 1u < cpp && y >= 0 && y < (int)a.H && x >= 0 && x < (int)a.W)
+                v[b] = *reinterpret_cast<const u32x4*>(a.in + (img + (size_t)y * a.W + (size_t)x) * a.in_stride +
+                                                       8u * c8);
+        }
+#pragma unroll
+        for (uint32_t b = 0; b < kBatch; ++b) {
+            const uint32_t i = i0 + threadIdx.x + 256u * b;
+            const uint32_t pix = (i * inv) >> 20, c8 = i - pix * cpp;
+            if (i < total) *reinterpret_cast<u32x4*>(tile + (size_t)pix * cs + 8u * c8) = v[b];
+        }
     }
-    __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t g = lane >> 4, r = lane & 15u;
-    f32x4 acc[2][NT];
+    f32x4 acc[MT][NT];
 #pragma unroll
-    for (int m = 0; m < 2; ++m)
+    for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int j = 0; j < NT; ++j) acc[m][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const uint32_t K = 9u * a.Cin;
+    const uint32_t S = (K + 31u) / 32u;
     // weight slices: slice s holds rows co < NT*16, k = 32s .. 32s+31 (zeros past Cout or K);
     // piece q of a slice is 16 B: row q / 4, k offset 8 (q % 4)
     constexpr uint32_t kPieces = (uint32_t)NT * 64u;
-    auto fetch = [&](uint32_t k0, uint32_t q) -> u32x4 {
-        const uint32_t co = q >> 2, kk = k0 + 8u * (q & 3u);
-        return (co < a.Cout && kk < K) ? *reinterpret_cast<const u32x4*>(a.w + (size_t)co * K + kk)
-                                       : u32x4{0u, 0u, 0u, 0u};
+    constexpr uint32_t kPer = (kPieces + 255u) / 256u;
+    // loads are unconditional (a clamped address, zeros selected afterwards): conditional
+    // loads make the compiler wait for every outstanding load at the join
+    // every thread fetches and stores kPer pieces: past kPieces it repeats a piece (the same
+    // value to the same place), so nothing is conditional and the compiler keeps the fetch
+    // where it is issued instead of sinking it next to the store
+    auto piece = [&](uint32_t t) -> uint32_t {
+        const uint32_t q = threadIdx.x + 256u * t;
+        return q < kPieces ? q : q - kPieces * ((q - kPieces) / kPieces + 1u);
     };
-    auto store = [&](uint32_t buf, uint32_t q, const u32x4& v) {
-        *reinterpret_cast<u32x4*>(bsl + (size_t)buf * (NT * 16u * kBPitch) + (q >> 2) * kBPitch + 8u * (q & 3u)) = v;
+    // (pieces outside the weights are fetched from a clamped address and zeroed at the store,
+    // so the wait for the fetch comes at the store)
+    auto fetch = [&](uint32_t sl, u32x4 (&dst)[kPer], bool (&ok)[kPer]) {
+#pragma unroll
+        for (uint32_t t = 0; t < kPer; ++t) {
+            const uint32_t q = piece(t);
+            const uint32_t co = q >> 2, kk = 32u * sl + 8u * (q & 3u);
+            ok[t] = co < a.Cout && kk < K;
+            dst[t] = *reinterpret_cast<const u32x4*>(a.w + (ok[t] ? (size_t)co * K + kk : 0u));
+        }
     };
-    for (uint32_t q = threadIdx.x; q < kPieces; q += 256u) store(0, q, fetch(0, q));
+    auto store = [&](uint32_t buf, const u32x4 (&src)[kPer], const bool (&ok)[kPer]) {
+#pragma unroll
+        for (uint32_t t = 0; t < kPer; ++t) {
+            const uint32_t q = piece(t);
+            *reinterpret_cast<u32x4*>(bsl + (size_t)buf * (NT * 16u * kBPitch) + (q >> 2) * kBPitch + 8u * (q & 3u)) =
+                ok[t] ? src[t] : u32x4{0u, 0u, 0u, 0u};
+        }
+    };
+    u32x4 nx[kPer];
+    bool nok[kPer];
+    fetch(0, nx, nok);
+    store(0, nx, nok);
     __syncthreads();
-    uint32_t buf = 0;
-    for (uint32_t k0 = 0; k0 < K; k0 += 32u) {
-        // prefetch the next slice into registers while this one is used
-        u32x4 nx[(kPieces + 255u) / 256u];
-        const bool more = k0 + 32u < K;
+    // k = 32 s + 8 g as (tap, channel): tap = (k * invc) >> 20, exact for k < 9 * 96
+    const uint32_t invc = ((1u << 20) + a.Cin - 1u) / a.Cin;
+    const uint32_t rowbase = (uint32_t)MT * wave;
+    for (uint32_t sl = 0; sl < S; ++sl) {
+        const uint32_t cur = sl & 1u;
+        // the next slice (the last one again past the end: a buffer nobody reads)
+        fetch(min(sl + 1u, S - 1u), nx, nok);
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t k = 32u * sl + 8u * g;
+        const uint32_t tap = (k * invc) >> 20;
+        const bool kv = tap < 9u;
+        const uint32_t tp = kv ? tap : 4u, cc = kv ? k - tap * a.Cin : a.Cin;   // past K: the zero padding
+        const uint32_t ty = (tp >= 3u ? 1u : 0u) + (tp >= 6u ? 1u : 0u);
+        const int dy = (int)ty - 1, dx = (int)(tp - 3u * ty) - 1;
+        bf16x8 A[MT];
 #pragma unroll
-        for (uint32_t t = 0; t < (kPieces + 255u) / 256u; ++t) {
-            const uint32_t q = threadIdx.x + 256u * t;
-            nx[t] = (more && q < kPieces) ? fetch(k0 + 32u, q) : u32x4{0u, 0u, 0u, 0u};
+        for (int m = 0; m < MT; ++m) {
+            const int ly = (int)rowbase + m + 1 + dy, lx = (int)r + 1 + dx;
+            A[m] = __builtin_bit_cast(
+                bf16x8, *reinterpret_cast<const u32x4*>(tile + (size_t)(ly * (kTileX + 2) + lx) * cs + cc));
         }
-        const uint32_t k = k0 + 8u * g;
-        const bool kv = k < K;
-        uint32_t tap = 0, ci = 0;
-        if (kv) {
-            tap = k / a.Cin;
-            ci = k - tap * a.Cin;
-        }
-        const int dy = (int)(tap / 3u) - 1, dx = (int)(tap % 3u) - 1;
-        bf16x8 A[2];
+        const uint16_t* bs = bsl + (size_t)cur * (NT * 16u * kBPitch);
+        bf16x8 B[NT];
 #pragma unroll
-        for (int m = 0; m < 2; ++m) {
-            const int ly = (int)(2u * wave) + m + 1 + dy, lx = (int)r + 1 + dx;
-            A[m] = kv ? __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
-                            tile + (size_t)(ly * (kTileX + 2) + lx) * cs + ci))
-                      : zero8();
-        }
-        const uint16_t* bs = bsl + (size_t)buf * (NT * 16u * kBPitch);
+        for (int j = 0; j < NT; ++j)
+            B[j] = __builtin_bit_cast(
+                bf16x8, *reinterpret_cast<const u32x4*>(bs + ((uint32_t)j * 16u + r) * kBPitch + 8u * g));
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+                acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[j], A[m], acc[m][j], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        store(cur ^ 1u, nx, nok);
+        __syncthreads();
+    }
+    // D layout: lane holds output channels 16j + 4g .. 16j + 4g + 3 of pixel r of its row m.
+    // BF16 outputs at 4-channel aligned strides and offsets go out as one 8-byte store per
+    // four channels (mask read the same way); otherwise one store per channel
+    const int x = tx0 + (int)r;
+    if (x >= (int)a.W) return;
+    const bool vec = !OUTF && a.Cout % 4u == 0 && a.out_stride % 4u == 0 && a.out_off % 4u == 0 &&
+                     (!MASK || a.mask_stride % 4u == 0);
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+        const int y = ty0 + (int)rowbase + m;
+        if (y >= (int)a.H) continue;
+        const size_t p = img + (size_t)y * a.W + (size_t)x;
 #pragma unroll
         for (int j = 0; j < NT; ++j) {
-            const bf16x8 B = __builtin_bit_cast(
-                bf16x8, *reinterpret_cast<const u32x4*>(bs + ((uint32_t)j * 16u + r) * kBPitch + 8u * g));
-            acc[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B, acc[0][j], 0, 0, 0);
-            acc[1][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B, acc[1][j], 0, 0, 0);
-        }
-        if (more) {
+            const uint32_t co0 = (uint32_t)j * 16u + 4u * g;
+            if (co0 >= a.Cout) continue;
+            if (vec) {
+                float v[4];
 #pragma unroll
-            for (uint32_t t = 0; t < (kPieces + 255u) / 256u; ++t) {
-                const uint32_t q = threadIdx.x + 256u * t;
-                if (q < kPieces) store(buf ^ 1u, q, nx[t]);
+                for (int i = 0; i < 4; ++i) {
+                    v[i] = acc[m][j][i] + (a.bias ? a.bias[co0 + i] : 0.0f);
+                    if (RELU) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
+                }
+                if (MASK) {
+                    const uint2 mk = *reinterpret_cast<const uint2*>(a.mask + p * a.mask_stride + co0);
+                    if (!(bf2f(mk.x) > 0.0f)) v[0] = 0.0f;
+                    if (!(bf2f(mk.x >> 16) > 0.0f)) v[1] = 0.0f;
+                    if (!(bf2f(mk.y) > 0.0f)) v[2] = 0.0f;
+                    if (!(bf2f(mk.y >> 16) > 0.0f)) v[3] = 0.0f;
+                }
+                *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(a.out) + p * a.out_stride + a.out_off + co0) =
+                    make_uint2(pk2bf(v[0], v[1]), pk2bf(v[2], v[3]));
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint32_t co = co0 + (uint32_t)i;
+                    if (co >= a.Cout) continue;
+                    float v = acc[m][j][i] + (a.bias ? a.bias[co] : 0.0f);
+                    if (RELU) v = v > 0.0f ? v : 0.0f;
+                    if (MASK && !(bf2f(a.mask[p * a.mask_stride + co]) > 0.0f)) v = 0.0f;
+                    if (OUTF) reinterpret_cast<float*>(a.out)[p * a.out_stride + a.out_off + co] = v;
+                    else reinterpret_cast<uint16_t*>(a.out)[p * a.out_stride + a.out_off + co] = f2bf(v);
+                }
             }
         }
-        __syncthreads();
-        buf ^= 1u;
     }
-    // D layout: lane holds column r (output channel) of rows 4g..4g+3 (pixels 4g.. of its row)
+}
+
+// Resident-weight form for layers whose weights fit in LDS next to one input tile: persistent
+// workgroups of NW waves load the layer's weights into LDS once, as rows of KP = 9 Cin rounded
+// up to 64 (zeros past Cout and past 9 Cin) at a pitch of KP + 8 elements (the 16 rows one
+// operand reads then start in different banks), and walk the kTileX x (NW MT) pixel tiles of
+// the grid.  The next tile's input is fetched into registers while the current tile is
+// computed, so the K loop reads only LDS and has no barrier; each K step's tap offset comes
+// from a small LDS table.  K chunks, operands and their order are those of k_conv3x3 (steps
+// past 9 Cin add zero products), so every output is the same sum in the same order.
+template <int MT, int NW>
+constexpr uint32_t res_chunks() {   // 16-B chunks of a halo tile per thread, Cin <= 96 (12 + 1 pad)
+    return ((uint32_t)(NW * MT + 2) * (kTileX + 2) * 13u + 64u * NW - 1u) / (64u * NW);
+}
+inline uint32_t res_kp(uint32_t cin) { return (9u * cin + 63u) & ~63u; }
+template <int NT>
+inline size_t conv_res_lds(int rows, uint32_t cin) {
+    return ((size_t)NT * 16u * (res_kp(cin) + 8u) + (size_t)(rows + 2) * (kTileX + 2) * (cin + 8u)) * 2u +
+           4u * (res_kp(cin) / 32u + 2u) * 4u;
+}
+
+template <int MT, int NW, int NT, bool RELU, bool OUTF, bool MASK>
+__global__ __launch_bounds__(64 * NW) void k_conv3x3_res(ConvArgs a, uint32_t ntx, uint32_t nty, uint32_t ntiles) {
+    constexpr int TY = NW * MT;
+    constexpr uint32_t kThreads = 64u * NW;
+    constexpr uint32_t kPix = (uint32_t)(TY + 2) * (kTileX + 2);
+    constexpr uint32_t kCh = res_chunks<MT, NW>();
+    extern __shared__ __align__(16) uint16_t lds[];
+    const uint32_t K = 9u * a.Cin, KP = (K + 63u) & ~63u, WP = KP + 8u, KS = KP / 32u;
+    uint16_t* const wl = lds;                                  // NT*16 rows x WP
+    uint16_t* const tile = lds + (size_t)NT * 16u * WP;        // kPix pixels x cs
+    const uint32_t cs = a.Cin + 8u;
+    uint32_t* const offs = reinterpret_cast<uint32_t*>(tile + (size_t)kPix * cs);   // KS x 4 tap offsets
+    const uint32_t tid = threadIdx.x;
+    // weights -> LDS in batches of 8 pieces of 16 B per thread (row = piece / kq by a
+    // multiply-high with ceil(2^32 / kq), exact for these sizes)
+    {
+        const uint32_t kq = KP / 8u, np = (uint32_t)NT * 16u * kq;
+        const uint32_t magic = (uint32_t)(((1ull << 32) + kq - 1u) / kq);
+        for (uint32_t i0 = 0; i0 < np; i0 += kThreads * 8u) {
+            u32x4 v[8];
 #pragma unroll
-    for (int m = 0; m < 2; ++m) {
-        const int y = ty0 + (int)(2u * wave) + m;
-        if (y >= (int)a.H) continue;
+            for (uint32_t b = 0; b < 8u; ++b) {
+                const uint32_t i = i0 + tid + kThreads * b;
+                const uint32_t row = __umulhi(i, magic), kk = 8u * (i - row * kq);
+                v[b] = u32x4{0u, 0u, 0u, 0u};
+                if (i < np && row < a.Cout && kk < K) v[b] = *reinterpret_cast<const u32x4*>(a.w + (size_t)row * K + kk);
+            }
 #pragma unroll
-        for (int j = 0; j < NT; ++j) {
-            const uint32_t co = (uint32_t)j * 16u + r;
-            if (co >= a.Cout) continue;
-            const float b = a.bias ? a.bias[co] : 0.0f;
+            for (uint32_t b = 0; b < 8u; ++b) {
+                const uint32_t i = i0 + tid + kThreads * b;
+                const uint32_t row = __umulhi(i, magic), kk = 8u * (i - row * kq);
+                if (i < np) *reinterpret_cast<u32x4*>(wl + (size_t)row * WP + kk) = v[b];
+            }
+        }
+    }
+    // tap offsets: entry 4 s + g is the tile offset (elements) of k = 32 s + 8 g relative to the
+    // centre tap's pixel; past 9 Cin (and for the two steps after the last, which the pipelined
+    // loop reads ahead) it points at the centre pixel's zero padding
+    if (tid < 4u * (KS + 2u)) {
+        const uint32_t k = 8u * tid;
+        const uint32_t tap = k / a.Cin;
+        const bool kv = tap < 9u;
+        const int dy = kv ? (int)(tap / 3u) - 1 : 0, dx = kv ? (int)(tap % 3u) - 1 : 0;
+        offs[tid] = (uint32_t)((dy * (kTileX + 2) + dx) * (int)cs) + (kv ? k - tap * a.Cin : a.Cin);
+    }
+    // input tiles: cpp = Cin / 8 + 1 chunks per pixel (channels, then the zero padding); chunk
+    // i -> pixel (i * inv) >> 20, exact for i < 2^20 / 13
+    const uint32_t cpp = a.Cin / 8u + 1u, total = kPix * cpp;
+    const uint32_t inv = ((1u << 20) + cpp - 1u) / cpp;
+    u32x4 pre[kCh];
+    auto origin = [&](uint32_t t, uint32_t& n, int& tx0, int& ty0) {
+        const uint32_t tx = t % ntx, rest = t / ntx;
+        tx0 = (int)(tx * (uint32_t)kTileX);
+        ty0 = (int)((rest % nty) * (uint32_t)TY);
+        n = rest / nty;
+    };
+    auto fetch_tile = [&](uint32_t t) {
+        uint32_t n;
+        int tx0, ty0;
+        origin(t, n, tx0, ty0);
+        const size_t img = (size_t)n * a.H * a.W;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int x = tx0 + (int)(4u * g) + i;
-                if (x >= (int)a.W) continue;
-                const size_t p = img + (size_t)y * a.W + (size_t)x;
-                float v = acc[m][j][i] + b;
-                if (RELU) v = v > 0.0f ? v : 0.0f;
-                if (MASK && !(bf2f(a.mask[p * a.mask_stride + co]) > 0.0f)) v = 0.0f;
-                if (OUTF) reinterpret_cast<float*>(a.out)[p * a.out_stride + a.out_off + co] = v;
-                else reinterpret_cast<uint16_t*>(a.out)[p * a.out_stride + a.out_off + co] = f2bf(v);
+        for (uint32_t b = 0; b < kCh; ++b) {
+            const uint32_t i = tid + kThreads * b;
+            const uint32_t pix = (i * inv) >> 20, c8 = i - pix * cpp;
+            const int y = ty0 + (int)(pix / (uint32_t)(kTileX + 2)) - 1, x = tx0 + (int)(pix % (uint32_t)(kTileX + 2)) - 1;
+            pre[b] = u32x4{0u, 0u, 0u, 0u};
+            if (i < total && c8 + 1u < cpp && y >= 0 && y < (int)a.H && x >= 0 && x < (int)a.W)
+                pre[b] = *reinterpret_cast<const u32x4*>(a.in + (img + (size_t)y * a.W + (size_t)x) * a.in_stride + 8u * c8);
+        }
+    };
+    uint32_t t = blockIdx.x;
+    if (t < ntiles) fetch_tile(t);
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    const uint32_t g = lane >> 4, r = lane & 15u;
+    const uint32_t rowbase = (uint32_t)MT * wave;
+    // this lane's centre pixel of row 0 of its rows, and its weight row
+    const uint16_t* const abase = tile + (size_t)((rowbase + 1u) * (kTileX + 2) + r + 1u) * cs;
+    const uint32_t rstep = (uint32_t)(kTileX + 2) * cs;
+    const uint16_t* const wrow = wl + (size_t)r * WP + 8u * g;
+    const bool vec = !OUTF && a.Cout % 4u == 0 && a.out_stride % 4u == 0 && a.out_off % 4u == 0 &&
+                     (!MASK || a.mask_stride % 4u == 0);
+    // this lane's biases, loaded once: loads in the epilogue would also wait for the next
+    // tile's fetch (vector loads complete in order)
+    float bias[NT][4];
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t co = (uint32_t)j * 16u + 4u * g + (uint32_t)i;
+            bias[j][i] = (a.bias && co < a.Cout) ? a.bias[co] : 0.0f;
+        }
+    for (; t < ntiles; t += gridDim.x) {
+        __syncthreads();   // the previous tile is no longer read (and weights and offsets are written)
+#pragma unroll
+        for (uint32_t b = 0; b < kCh; ++b) {
+            const uint32_t i = tid + kThreads * b;
+            const uint32_t pix = (i * inv) >> 20, c8 = i - pix * cpp;
+            if (i < total) *reinterpret_cast<u32x4*>(tile + (size_t)pix * cs + 8u * c8) = pre[b];
+        }
+        __syncthreads();
+        uint32_t n;
+        int tx0, ty0;
+        origin(t, n, tx0, ty0);
+        if (t + gridDim.x < ntiles) fetch_tile(t + gridDim.x);
+        f32x4 acc[MT][NT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int j = 0; j < NT; ++j) acc[m][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        // software-pipelined by hand: the operands of step s + 1 (and the offset of step s + 2)
+        // are read while the MFMAs of step s run; KS is even, reads past the last step hit the
+        // zero padding (A) or whatever follows the weight row (B, unused)
+        bf16x8 A0[MT], B0[NT], A1[MT], B1[NT];
+        auto load = [&](uint32_t ks, uint32_t o, bf16x8 (&A)[MT], bf16x8 (&B)[NT]) {
+            const uint16_t* const ap = abase + (int32_t)o;
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+                A[m] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(ap + (size_t)m * rstep));
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+                B[j] = __builtin_bit_cast(bf16x8,
+                                          *reinterpret_cast<const u32x4*>(wrow + (size_t)j * 16u * WP + 32u * ks));
+        };
+        auto mma = [&](const bf16x8 (&A)[MT], const bf16x8 (&B)[NT]) {
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+                    acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[j], A[m], acc[m][j], 0, 0, 0);
+        };
+        load(0u, offs[g], A0, B0);
+        uint32_t o1 = offs[4u + g];
+        // (scheduling barriers keep each group of reads ahead of the MFMAs it overlaps)
+        for (uint32_t ks = 0; ks < KS; ks += 2u) {
+            load(ks + 1u, o1, A1, B1);
+            const uint32_t o2 = offs[4u * (ks + 2u) + g];
+            __builtin_amdgcn_sched_barrier(0);
+            mma(A0, B0);
+            __builtin_amdgcn_sched_barrier(0);
+            load(ks + 2u, o2, A0, B0);
+            o1 = offs[4u * (ks + 3u) + g];
+            __builtin_amdgcn_sched_barrier(0);
+            mma(A1, B1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // D layout: lane holds output channels 16j + 4g .. 16j + 4g + 3 of pixel r of row m
+        const int x = tx0 + (int)r;
+        if (x >= (int)a.W) continue;
+        const size_t img = (size_t)n * a.H * a.W;
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int y = ty0 + (int)rowbase + m;
+            if (y >= (int)a.H) continue;
+            const size_t p = img + (size_t)y * a.W + (size_t)x;
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                const uint32_t co0 = (uint32_t)j * 16u + 4u * g;
+                if (co0 >= a.Cout) continue;
+                if (vec) {
+                    float v[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        v[i] = acc[m][j][i] + bias[j][i];
+                        if (RELU) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
+                    }
+                    if (MASK) {
+                        const uint2 mk = *reinterpret_cast<const uint2*>(a.mask + p * a.mask_stride + co0);
+                        if (!(bf2f(mk.x) > 0.0f)) v[0] = 0.0f;
+                        if (!(bf2f(mk.x >> 16) > 0.0f)) v[1] = 0.0f;
+                        if (!(bf2f(mk.y) > 0.0f)) v[2] = 0.0f;
+                        if (!(bf2f(mk.y >> 16) > 0.0f)) v[3] = 0.0f;
+                    }
+                    *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(a.out) + p * a.out_stride + a.out_off + co0) =
+                        make_uint2(pk2bf(v[0], v[1]), pk2bf(v[2], v[3]));
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const uint32_t co = co0 + (uint32_t)i;
+                        if (co >= a.Cout) continue;
+                        float v = acc[m][j][i] + bias[j][i];
+                        if (RELU) v = v > 0.0f ? v : 0.0f;
+                        if (MASK && !(bf2f(a.mask[p * a.mask_stride + co]) > 0.0f)) v = 0.0f;
+                        if (OUTF) reinterpret_cast<float*>(a.out)[p * a.out_stride + a.out_off + co] = v;
+                        else reinterpret_cast<uint16_t*>(a.out)[p * a.out_stride + a.out_off + co] = f2bf(v);
+                    }
+                }
             }
         }
     }
@@ -820,18 +1110,77 @@ int ensure_grad(size_t P0) {
 }
 
 // ---- launches ----------------------------------------------------------------------------
+// launches one tiled instantiation; workgroups above 64 KiB of LDS are allowed first (once
+// per instantiation and size)
+template <int MT, int NT, bool RELU, bool OUTF, bool MASK>
+int launch_tile(const ConvArgs& a, dim3 tg, hipStream_t s) {
+    const size_t lds = conv_tile_lds<NT>(MT, a.Cin);
+    static size_t allowed = 65536;
+    if (lds > allowed) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3x3_tile<MT, NT, RELU, OUTF, MASK>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+            return -1;
+        allowed = lds;
+    }
+    hipLaunchKernelGGL((k_conv3x3_tile<MT, NT, RELU, OUTF, MASK>), tg, dim3(256), lds, s, a);
+    return 0;
+}
+
+// compute units of the current device (queried once)
+uint32_t device_cus() {
+    static int n = 0;
+    if (n == 0 && (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, vt_rt_device()) != hipSuccess || n <= 0))
+        n = 256;
+    return (uint32_t)n;
+}
+
+// resident-weight launch when the weights and a tile fit in LDS and every workgroup gets at
+// least two tiles; returns 1 when not applicable
+template <int MT, int NW, int NT, bool RELU, bool OUTF, bool MASK>
+int launch_res(const ConvArgs& a, hipStream_t s) {
+    constexpr uint32_t kMaxLds = 160u * 1024u;
+    const size_t lds = conv_res_lds<NT>(NW * MT, a.Cin);
+    if (lds > kMaxLds) return 1;
+    const uint32_t ntx = (a.W + kTileX - 1) / kTileX, nty = (a.H + NW * MT - 1u) / (NW * MT);
+    const size_t ntiles = (size_t)ntx * nty * a.N;
+    const uint32_t per_cu = std::max<uint32_t>(1u, std::min<uint32_t>(8u / NW, (uint32_t)(kMaxLds / lds)));
+    const size_t grid = (size_t)device_cus() * per_cu;
+    if (ntiles < 2u * grid || ntiles > 0xFFFFFFFFu) return 1;
+    static size_t allowed = 65536;
+    if (lds > allowed) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3x3_res<MT, NW, NT, RELU, OUTF, MASK>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+            return -1;
+        allowed = lds;
+    }
+    hipLaunchKernelGGL((k_conv3x3_res<MT, NW, NT, RELU, OUTF, MASK>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a,
+                       ntx, nty, (uint32_t)ntiles);
+    return 0;
+}
+
 template <bool RELU, bool OUTF, bool MASK>
 int launch_conv_nt(uint32_t nt, const ConvArgs& a, hipStream_t s) {
     if (a.Cin <= kTileMaxCin && a.Cin % 8u == 0 && a.in_stride % 8u == 0 && nt <= 6) {
-        const dim3 tg((a.W + kTileX - 1) / kTileX, (a.H + kTileY - 1) / kTileY, a.N), tb(256);
-        const size_t lds = ((size_t)(kTileY + 2) * (kTileX + 2) * (a.Cin + 8u) + 2u * nt * 16u * kBPitch) * 2u;
+        // resident weights where they fit and the grid has enough tiles, else weight slices
+        int rc = 1;
         switch (nt) {
-            case 1: hipLaunchKernelGGL((k_conv3x3_tile<1, RELU, OUTF, MASK>), tg, tb, lds, s, a); return 0;
-            case 2: hipLaunchKernelGGL((k_conv3x3_tile<2, RELU, OUTF, MASK>), tg, tb, lds, s, a); return 0;
-            case 3: hipLaunchKernelGGL((k_conv3x3_tile<3, RELU, OUTF, MASK>), tg, tb, lds, s, a); return 0;
-            case 4: hipLaunchKernelGGL((k_conv3x3_tile<4, RELU, OUTF, MASK>), tg, tb, lds, s, a); return 0;
-            case 5: hipLaunchKernelGGL((k_conv3x3_tile<5, RELU, OUTF, MASK>), tg, tb, lds, s, a); return 0;
-            case 6: hipLaunchKernelGGL((k_conv3x3_tile<6, RELU, OUTF, MASK>), tg, tb, lds, s, a); return 0;
+            case 1: rc = launch_res<2, 8, 1, RELU, OUTF, MASK>(a, s); break;
+            case 2: rc = launch_res<2, 8, 2, RELU, OUTF, MASK>(a, s); break;
+            case 3: rc = launch_res<2, 8, 3, RELU, OUTF, MASK>(a, s); break;
+            case 4: rc = launch_res<2, 8, 4, RELU, OUTF, MASK>(a, s); break;
+            case 5: rc = launch_res<2, 8, 5, RELU, OUTF, MASK>(a, s); break;
+            case 6: rc = launch_res<2, 8, 6, RELU, OUTF, MASK>(a, s); break;
+            default: break;
+        }
+        if (rc <= 0) return rc;
+        const dim3 tg((a.W + kTileX - 1) / kTileX, (a.H + 7u) / 8u, a.N);
+        switch (nt) {
+            case 1: return launch_tile<2, 1, RELU, OUTF, MASK>(a, tg, s);
+            case 2: return launch_tile<2, 2, RELU, OUTF, MASK>(a, tg, s);
+            case 3: return launch_tile<2, 3, RELU, OUTF, MASK>(a, tg, s);
+            case 4: return launch_tile<2, 4, RELU, OUTF, MASK>(a, tg, s);
+            case 5: return launch_tile<2, 5, RELU, OUTF, MASK>(a, tg, s);
+            case 6: return launch_tile<2, 6, RELU, OUTF, MASK>(a, tg, s);
             default: break;
         }
     }
