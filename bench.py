@@ -53,6 +53,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_MFMA_PEAK_TFLOPS = 2500.0     # MI355X dense BF16
 KERNEL_NAMES = {"render": "k_render0", "primary": "k_primary", "secondary": "k_secondary",
                 "accumulate": "k_accumulate"}
 
@@ -231,6 +232,78 @@ def summarize_trace(d) -> list:
     return sorted(rows, key=lambda r: -r[5])
 
 
+def _dispatch_rows(d):
+    """Per-dispatch {name, counters...} in dispatch order from a rocprofv3 --pmc directory."""
+    rows = {}
+    for name, c, v, disp in _counter_rows(d):
+        e = rows.setdefault(disp, {"name": name})
+        e[c] = e.get(c, 0.0) + v
+    return [rows[k] for k in sorted(rows, key=lambda x: int(x))]
+
+
+def _short(name: str) -> str:
+    return name.replace("(anonymous namespace)::", "").replace("void ", "", 1).split("(")[0]
+
+
+def summarize_launches(args) -> None:
+    """Markdown table of every launch of the last frame (from the last launch of --frame-from)
+    of a rocprofv3 --kernel-trace directory: duration, and where the PMC directories are given,
+    BF16 MFMA TFLOP/s and its fraction of peak, MFMA busy (share of SIMD cycles) and HBM bytes
+    (FETCH_SIZE / WRITE_SIZE scaled by a plain copy of known size)."""
+    import glob
+    import sqlite3
+    trace = []
+    for path in glob.glob(os.path.join(args.trace, "**", "*.db"), recursive=True):
+        db = sqlite3.connect(path)
+        try:
+            trace += list(db.execute("select name, duration, start from kernels"))
+        finally:
+            db.close()
+    trace.sort(key=lambda r: r[2])
+
+    def last_frame(rows, key):
+        idx = [i for i, r in enumerate(rows) if args.frame_from in key(r)]
+        return rows[idx[-1]:] if idx else []
+
+    launches = last_frame(trace, lambda r: r[0])
+    mfma = last_frame(_dispatch_rows(args.mfma), lambda r: r["name"]) if args.mfma else []
+    fetch = last_frame(_dispatch_rows(args.fetch), lambda r: r["name"]) if args.fetch else []
+    write = last_frame(_dispatch_rows(args.write), lambda r: r["name"]) if args.write else []
+    fk = wk = None
+    if args.cal_fetch and args.cal_write:
+        copy = "__amd_rocclr_copyBuffer"
+        fk = summarize_counters([args.cal_fetch])[copy]["FETCH_SIZE"] * 1024.0 / (256 << 20)
+        wk = summarize_counters([args.cal_write])[copy]["WRITE_SIZE"] * 1024.0 / (256 << 20)
+    print("| kernel | us | BF16 TFLOP/s | of peak | MFMA busy | fetch MB | write MB |\n|---|---|---|---|---|---|---|")
+    tot_us = tot_fl = tot_f = tot_w = 0.0
+    for i, (name, dur, _) in enumerate(launches):
+        us = float(dur) * 1e-3
+        tot_us += us
+        cells = [f"`{_short(name)}`", f"{us:.1f}"]
+        m = mfma[i] if i < len(mfma) and _short(mfma[i]["name"]) == _short(name) else None
+        if m and m.get("SQ_INSTS_VALU_MFMA_MOPS_BF16"):
+            fl = m["SQ_INSTS_VALU_MFMA_MOPS_BF16"] * 512.0
+            tot_fl += fl
+            tf = fl / (us * 1e-6) / 1e12
+            # SQ_VALU_MFMA_BUSY_CYCLES summed over SIMDs; GRBM_GUI_ACTIVE summed over 8 XCDs
+            busy = m["SQ_VALU_MFMA_BUSY_CYCLES"] / (m["GRBM_GUI_ACTIVE"] / 8.0 * 1024.0) if m.get("GRBM_GUI_ACTIVE") else None
+            cells += [f"{tf:.0f}", f"{tf / BF16_MFMA_PEAK_TFLOPS:.3f}", f"{busy:.3f}" if busy is not None else ""]
+        else:
+            cells += ["", "", ""]
+        f = fetch[i] if i < len(fetch) and _short(fetch[i]["name"]) == _short(name) else None
+        w = write[i] if i < len(write) and _short(write[i]["name"]) == _short(name) else None
+        if f and w and fk and wk:
+            fb, wb = f["FETCH_SIZE"] * 1024.0 / fk, w["WRITE_SIZE"] * 1024.0 / wk
+            tot_f += fb
+            tot_w += wb
+            cells += [f"{fb / 1e6:.1f}", f"{wb / 1e6:.1f}"]
+        else:
+            cells += ["", ""]
+        print("| " + " | ".join(cells) + " |")
+    tf = tot_fl / (tot_us * 1e-6) / 1e12 if tot_us else 0.0
+    print(f"| **frame** | {tot_us:.1f} | {tf:.0f} | {tf / BF16_MFMA_PEAK_TFLOPS:.3f} | | {tot_f / 1e6:.1f} | {tot_w / 1e6:.1f} |")
+
+
 def summarize_profiles(args) -> None:
     """Writes profiles/traffic.json (FETCH_SIZE / WRITE_SIZE per launch, scaled by a plain copy
     of known size) and profiles/pmc.json (SQ / TCC / TCP counters per launch) from rocprofv3
@@ -335,9 +408,16 @@ def main():
     ap.add_argument("--cal-write")
     ap.add_argument("--pmc", nargs="*")
     ap.add_argument("--trace", help="print a markdown table of a rocprofv3 --kernel-trace output directory")
+    ap.add_argument("--frame-from", help="with --trace: one row per launch of the last frame, which starts at the "
+                                         "last launch of this kernel (e.g. k_features)")
+    ap.add_argument("--mfma", help="with --frame-from: rocprofv3 --pmc directory with SQ_INSTS_VALU_MFMA_MOPS_BF16, "
+                                   "SQ_VALU_MFMA_BUSY_CYCLES and GRBM_GUI_ACTIVE")
     args = ap.parse_args()
     if args.summarize_profiles:
-        summarize_profiles(args)
+        if args.frame_from:
+            summarize_launches(args)
+        else:
+            summarize_profiles(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the renderer has no multi-GPU path); got --gpus {args.gpus}",
