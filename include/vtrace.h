@@ -127,7 +127,7 @@ int32_t vt_add_volume_indexed(const uint8_t* indices, uint32_t width, uint32_t h
  * (width*height*16 B, RGBA32F).  The frame's wavefront state, queues, counters and request
  * table live in a 128 MiB device array of the library (one per device, held until the
  * process ends), so the call makes no hipMalloc / hipFree / pinned-host allocation.  Frames
- * whose state exceeds it return -1 (path-traced frames need about 184 B per pixel sample,
+ * whose state exceeds it return -1 (path-traced frames need about 152 B per pixel sample,
  * padded to whole 64-pixel tiles: 640x480 at 1 spp fits, 1920x1080 at 1 spp does not).
  * Results equal render_tick's bit for bit; no G-buffer, no counters; one stream.  The call
  * waits for its frame.  Every call on a device shares that device's array: calls that overlap

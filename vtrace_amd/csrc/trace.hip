@@ -763,50 +763,66 @@ struct SurfaceRec {
     uint32_t face;
 };
 
-// Queue element: a shaded surface with its two secondary rays (64 B, written once by the launch
-// that found the surface, read by the next depth's bounce ray and sun ray).
+// Queue element: a shaded surface with its two secondary rays (48 B, written once by the launch
+// that found the surface, read by the next depth's bounce ray and sun ray).  Its world
+// position and normal are rebuilt from (instance, p, face) with
+// the operations surface_of uses, so they are the same bits; its bounce direction is drawn
+// from `rng` when the bounce ray starts (sun rays draw nothing, so every path consumes its
+// random numbers in the same order as when the direction was drawn at the hit).
 struct QueueRec {
     float p[3];        // hit point, voxel units of the hit instance
-    float X[3];        // world position
-    float w[3];        // bounce direction (world, unit)
     float c[3];        // direct sun light added if the sun ray is clear
     uint32_t inst;
     uint32_t path;
     uint16_t v[3];     // hit voxel (the start voxel is its neighbour across the face)
     uint8_t face;
     uint8_t pad;
+    uint32_t rgba;     // texel colour: a first bounce's throughput is its albedo
+    uint32_t rng;      // RNG state at the surface
 };
-static_assert(sizeof(QueueRec) == 64, "queue record is 64 bytes");
+static_assert(sizeof(QueueRec) == 48, "queue record is 48 bytes");
 
-__device__ __forceinline__ void put_rec(QueueRec* dst, const SurfaceRec& sf, const float4& dirw, const float4& contrib,
-                                        uint32_t path) {
+__device__ __forceinline__ void put_rec(QueueRec* dst, const SurfaceRec& sf, const float4& contrib, uint32_t path,
+                                        uint32_t rng) {
     const int fa = (int)(sf.face >> 1);
     const int off = (sf.face & 1u) == 0u ? -1 : 1;
     const uint32_t v0 = (uint32_t)(sf.vstart[0] - (fa == 0 ? off : 0));
     const uint32_t v1 = (uint32_t)(sf.vstart[1] - (fa == 1 ? off : 0));
     const uint32_t v2 = (uint32_t)(sf.vstart[2] - (fa == 2 ? off : 0));
     uint4* d = reinterpret_cast<uint4*>(dst);
-    d[0] = make_uint4(__float_as_uint(sf.p[0]), __float_as_uint(sf.p[1]), __float_as_uint(sf.p[2]), __float_as_uint(sf.X[0]));
-    d[1] = make_uint4(__float_as_uint(sf.X[1]), __float_as_uint(sf.X[2]), __float_as_uint(dirw.x), __float_as_uint(dirw.y));
-    d[2] = make_uint4(__float_as_uint(dirw.z), __float_as_uint(contrib.x), __float_as_uint(contrib.y),
-                      __float_as_uint(contrib.z));
-    d[3] = make_uint4(sf.inst, path, (v0 & 0xFFFFu) | (v1 << 16), (v2 & 0xFFFFu) | (sf.face << 16));
+    d[0] = make_uint4(__float_as_uint(sf.p[0]), __float_as_uint(sf.p[1]), __float_as_uint(sf.p[2]),
+                      __float_as_uint(contrib.x));
+    d[1] = make_uint4(__float_as_uint(contrib.y), __float_as_uint(contrib.z), sf.inst, path);
+    d[2] = make_uint4((v0 & 0xFFFFu) | (v1 << 16), (v2 & 0xFFFFu) | (sf.face << 16), sf.rgba, rng);
 }
 
 __device__ __forceinline__ QueueRec get_rec(const QueueRec* src) {
     const uint4* s = reinterpret_cast<const uint4*>(src);
-    const uint4 a = gld(s), b = gld(s + 1), c = gld(s + 2), d = gld(s + 3);
+    const uint4 a = gld(s), b = gld(s + 1), c = gld(s + 2);
     QueueRec r;
     r.p[0] = __uint_as_float(a.x); r.p[1] = __uint_as_float(a.y); r.p[2] = __uint_as_float(a.z);
-    r.X[0] = __uint_as_float(a.w); r.X[1] = __uint_as_float(b.x); r.X[2] = __uint_as_float(b.y);
-    r.w[0] = __uint_as_float(b.z); r.w[1] = __uint_as_float(b.w); r.w[2] = __uint_as_float(c.x);
-    r.c[0] = __uint_as_float(c.y); r.c[1] = __uint_as_float(c.z); r.c[2] = __uint_as_float(c.w);
-    r.inst = d.x;
-    r.path = d.y;
-    r.v[0] = (uint16_t)(d.z & 0xFFFFu); r.v[1] = (uint16_t)(d.z >> 16); r.v[2] = (uint16_t)(d.w & 0xFFFFu);
-    r.face = (uint8_t)(d.w >> 16);
+    r.c[0] = __uint_as_float(a.w); r.c[1] = __uint_as_float(b.x); r.c[2] = __uint_as_float(b.y);
+    r.inst = b.z;
+    r.path = b.w;
+    r.v[0] = (uint16_t)(c.x & 0xFFFFu); r.v[1] = (uint16_t)(c.x >> 16); r.v[2] = (uint16_t)(c.y & 0xFFFFu);
+    r.face = (uint8_t)(c.y >> 16);
     r.pad = 0;
+    r.rgba = c.z;
+    r.rng = c.w;
     return r;
+}
+
+// World position and normal of the surface point p (voxel units) on face `face` of instance I.
+__device__ __forceinline__ void surface_frame(const InstanceGPU* I, const float p[3], uint32_t face, V3& X, V3& n) {
+    float M[12], inv[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) { M[k] = gld(&I->M[k]); inv[k] = gld(&I->inv[k]); }
+    const V3 mp{p[0] / I->size[0] - 0.5f, p[1] / I->size[1] - 0.5f, p[2] / I->size[2] - 0.5f};
+    X = xf34(M, mp, 1.0f);
+    const int fa = (int)(face >> 1);
+    const float nm = (face & 1u) == 0u ? -1.0f : 1.0f;
+    // row fa of inverse(M): inv[c*3 + fa] (inverse-transpose applied to the model normal)
+    n = normalize3(V3{inv[0 * 3 + fa] * nm, inv[1 * 3 + fa] * nm, inv[2 * 3 + fa] * nm});
 }
 
 __device__ void surface_of(const SceneGPU& S, const HitG& h, SurfaceRec& sf) {
@@ -825,14 +841,8 @@ __device__ void surface_of(const SceneGPU& S, const HitG& h, SurfaceRec& sf) {
         sf.p[a] = x;
         sf.vstart[a] = h.d.v[a] + (a == fa ? (neg ? -1 : 1) : 0);
     }
-    float M[12], inv[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) { M[k] = gld(&I->M[k]); inv[k] = gld(&I->inv[k]); }
-    const V3 mp{sf.p[0] / I->size[0] - 0.5f, sf.p[1] / I->size[1] - 0.5f, sf.p[2] / I->size[2] - 0.5f};
-    const V3 X = xf34(M, mp, 1.0f);
-    const float nm = neg ? -1.0f : 1.0f;
-    // row fa of inverse(M): inv[c*3 + fa] (inverse-transpose applied to the model normal)
-    const V3 n = normalize3(V3{inv[0 * 3 + fa] * nm, inv[1 * 3 + fa] * nm, inv[2 * 3 + fa] * nm});
+    V3 X, n;
+    surface_frame(I, sf.p, h.face, X, n);
     sf.X[0] = X.x; sf.X[1] = X.y; sf.X[2] = X.z;
     sf.n[0] = n.x; sf.n[1] = n.y; sf.n[2] = n.z;
     sf.inst = h.inst;
@@ -845,7 +855,7 @@ __device__ void surface_of(const SceneGPU& S, const HitG& h, SurfaceRec& sf) {
 // walk per loop iteration), so lanes walking the origin and lanes walking another instance
 // share its instructions instead of running two copies of the walk one after the other.
 template <bool ST, bool DEEP>
-__device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const QueueRec& sf, V3 w, bool any_hit,
+__device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const QueueRec& sf, V3 O, V3 w, bool any_hit,
                           Ctr<ST>& ctr, const Lds& stk_lds, const Lds& mslot, TreeLds tree) {
     ctr.ray();
     HitG best;
@@ -853,7 +863,6 @@ __device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const QueueRec& 
     best.inst = VT_EMPTY;
     best.tn = INFINITY;
     const uint32_t origin = sf.inst;
-    const V3 O{sf.X[0], sf.X[1], sf.X[2]};
     const V3 inv_d{1.0f / w.x, 1.0f / w.y, 1.0f / w.z};
     TravStack<DEEP> stk;
     stk.lds = stk_lds;
@@ -1075,7 +1084,7 @@ struct WF {
     float4* L;            // per path: sky terms; w = 1.0f bits (1u) if the primary ray hit
     float4* Ls;           // per path with a primary hit: sun terms (sun rays only); w bits 1u if the
                           // primary vertex is shadowed (facing away, or its sun ray blocked)
-    float4* br;           // per path with a primary hit: throughput (xyz), RNG state (w bits)
+    float4* br;           // per path past its first bounce: throughput (xyz)
     QueueRec* rec[2];     // per queue slot
     uint32_t* sun[2];     // sun list: the queue slots whose surfaces face the sun (one sun ray each)
     uint32_t* counters;   // kSlots x kParts counters (queue sizes and fetch counters)
@@ -1086,8 +1095,7 @@ struct WF {
 // Shade a surface at `depth`: direct-light contribution for the sun ray and, if the path
 // continues, the cosine-sampled bounce direction (throughput times albedo).
 __device__ __forceinline__ void shade(const FrameGPU& F, const float* srgb_dec, const SurfaceRec& sf, uint32_t depth,
-                                      uint32_t max_b, float beta[3], Rng& rng, float4& contrib, float4& dirw,
-                                      bool& no_sun) {
+                                      uint32_t max_b, float beta[3], float4& contrib, bool& no_sun) {
     const float rho[3] = {srgb_dec[sf.rgba & 255u], srgb_dec[(sf.rgba >> 8) & 255u], srgb_dec[(sf.rgba >> 16) & 255u]};
     const float cosl = (sf.n[0] * F.sun[0] + sf.n[1] * F.sun[1]) + sf.n[2] * F.sun[2];
     contrib = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1097,13 +1105,9 @@ __device__ __forceinline__ void shade(const FrameGPU& F, const float* srgb_dec, 
         contrib = make_float4(beta[0] * (rho[0] * (F.sun_rad[0] * c)), beta[1] * (rho[1] * (F.sun_rad[1] * c)),
                               beta[2] * (rho[2] * (F.sun_rad[2] * c)), 1.0f);
     }
-    dirw = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (depth < max_b) {
-        V3 w;
-        cosine_dir(rng, V3{sf.n[0], sf.n[1], sf.n[2]}, w);
-        dirw = make_float4(w.x, w.y, w.z, 1.0f);
+    // a bounce follows: its throughput (the direction is drawn when the bounce ray starts)
+    if (depth < max_b)
         for (int k = 0; k < 3; ++k) beta[k] *= rho[k];
-    }
 }
 
 // Work index -> (pixel, sample) for mode 0: waves cover an 8 x 8 pixel tile.
@@ -1235,17 +1239,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
                                                       sqrtf((ex * ex + ey * ey) + ez * ez));
                 wf.gsamp[2 * (size_t)w + 1] = make_float4(sf.n[0], sf.n[1], sf.n[2], 1.0f);
             }
-            float4 contrib, dirw;
+            float4 contrib;
             bool no_sun;
-            shade(F, srgb_dec, sf, 0u, max_b, beta, rng, contrib, dirw, no_sun);
+            shade(F, srgb_dec, sf, 0u, max_b, beta, contrib, no_sun);
             const uint32_t slot = append_slot(slot_counters(wf.counters, 0), wf.seg_cap);
-            put_rec(&wf.rec[0][slot], sf, dirw, contrib, w);
+            put_rec(&wf.rec[0][slot], sf, contrib, w, rng.s);
             if (!no_sun) wf.sun[0][append_slot(slot_counters(wf.counters, sun_slot(0)), wf.seg_cap)] = slot;
-            // a hit path's state: sun terms (w: shadowed if facing away; its sun ray may set it),
-            // throughput and RNG state (only read by bounce rays)
+            // a hit path's state: sun terms (w: shadowed if facing away; its sun ray may set it);
+            // the first bounce's throughput is the albedo and its RNG state is in the record
             L.w = __uint_as_float(1u);
             wf.Ls[w] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(no_sun ? 1u : 0u));
-            if (max_b > 0) wf.br[w] = make_float4(beta[0], beta[1], beta[2], __uint_as_float(rng.s));
         }
         wf.L[w] = L;
     });
@@ -1256,7 +1259,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
 // (count slots) and of its sun list (ns entries): items [0, nb) are the bounce rays of its slots
 // 0..nb-1 (nb = count when paths continue, else 0), items [nb, nb + ns) the sun rays of the
 // slots its sun list names (surfaces facing away from the sun have none, so no lane idles).  The two kinds
-// write different per-path state (bounce rays: L, throughput and RNG state, the next queue; sun
+// write different per-path state (bounce rays: L, throughput, the next queue; sun
 // rays: Ls and the shadow flag), so they run side by side and fill each other's tail.  in_idx / out_idx:
 // counter slots of this depth's queue and the next one.  ONE: the batch holds one request (its
 // scene and frame are then wave-uniform, read with scalar loads).
@@ -1292,8 +1295,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         const ReqGPU& Rq = reqs[ONE ? 0u : find_req(B->work_base, nreq, path)];
         const SceneGPU S = Rq.S;
         const FrameGPU& F = Rq.F;
-        const V3 w = is_bounce ? V3{sf.w[0], sf.w[1], sf.w[2]} : V3{F.sun[0], F.sun[1], F.sun[2]};
-        HitG h = secondary<ST, DEEP>(S, F, sf, w, !is_bounce, ctr, stk, mslot, tree);
+        V3 X, n;
+        surface_frame(&S.inst[sf.inst], sf.p, sf.face, X, n);
+        // bounce rays: throughput (depth 0: 1 x albedo, the primary surface's; later: the path
+        // state) and the direction drawn from the record's RNG state
+        float beta[3] = {0.0f, 0.0f, 0.0f};
+        Rng rng{sf.rng};
+        V3 w{F.sun[0], F.sun[1], F.sun[2]};
+        if (is_bounce) {
+            if (depth == 0) {
+                beta[0] = 1.0f * srgb_dec[sf.rgba & 255u];
+                beta[1] = 1.0f * srgb_dec[(sf.rgba >> 8) & 255u];
+                beta[2] = 1.0f * srgb_dec[(sf.rgba >> 16) & 255u];
+            } else {
+                const float4 b4 = wf.br[path];
+                beta[0] = b4.x; beta[1] = b4.y; beta[2] = b4.z;
+            }
+            cosine_dir(rng, n, w);
+        }
+        HitG h = secondary<ST, DEEP>(S, F, sf, X, w, !is_bounce, ctr, stk, mslot, tree);
         if (!is_bounce) {
             if (!h.hit) {
                 float4 L = wf.Ls[path];
@@ -1308,12 +1328,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
             }
             return;
         }
-        const float4 b4 = wf.br[path];
         if (!h.hit) {
             float4 L = wf.L[path];
-            L.x = L.x + b4.x * F.sky[0];
-            L.y = L.y + b4.y * F.sky[1];
-            L.z = L.z + b4.z * F.sky[2];
+            L.x = L.x + beta[0] * F.sky[0];
+            L.y = L.y + beta[1] * F.sky[1];
+            L.z = L.z + beta[2] * F.sky[2];
             wf.L[path] = L;
             return;
         }
@@ -1321,16 +1340,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         uint8_t pal;
         ns.rgba = hit_rgba(S, h, &pal);
         surface_of(S, h, ns);
-        float beta[3] = {b4.x, b4.y, b4.z};
-        Rng rng{__float_as_uint(b4.w)};
-        float4 contrib, dirw;
+        float4 contrib;
         bool no_sun;
         const uint32_t max_b = F.mode == 2 ? F.max_bounces : 0u;
-        shade(F, srgb_dec, ns, depth + 1u, max_b, beta, rng, contrib, dirw, no_sun);
+        shade(F, srgb_dec, ns, depth + 1u, max_b, beta, contrib, no_sun);
         const uint32_t ns_slot = append_slot(slot_counters(wf.counters, out_idx), cap);
-        put_rec(&wf.rec[nq][ns_slot], ns, dirw, contrib, path);
+        put_rec(&wf.rec[nq][ns_slot], ns, contrib, path, rng.s);
         if (!no_sun) wf.sun[nq][append_slot(slot_counters(wf.counters, sun_slot(out_idx)), cap)] = ns_slot;
-        if (depth + 1u < max_b) wf.br[path] = make_float4(beta[0], beta[1], beta[2], __uint_as_float(rng.s));
+        if (depth + 1u < max_b) wf.br[path] = make_float4(beta[0], beta[1], beta[2], 0.0f);
     });
     if (ST) ctr.flush(stats);
 }
@@ -1496,7 +1513,7 @@ void vt_trace_set_persistent_blocks(uint32_t blocks) { g_persistent_blocks = blo
 
 size_t vt_wave_bytes(uint32_t npaths) {
     const size_t qcap = (size_t)kParts * seg_cap(npaths);
-    return (size_t)npaths * (16 + 16 + 16) + 2 * qcap * (64 + 4) + (size_t)kCounters * 4 + 16 * 256 +
+    return (size_t)npaths * (16 + 16 + 16) + 2 * qcap * (sizeof(QueueRec) + 4) + (size_t)kCounters * 4 + 16 * 256 +
            sizeof(ReqGPU) * VT_MAX_BATCH + sizeof(BatchGPU);
 }
 
@@ -1513,7 +1530,7 @@ int vt_wave_bind(WaveBuffers& wb, void* mem, uint32_t npaths) {
     wb.br = take((size_t)npaths * 16);
     const size_t qcap = (size_t)kParts * seg_cap(npaths);
     for (int q = 0; q < 2; ++q) {
-        wb.rec[q] = take(qcap * 64);
+        wb.rec[q] = take(qcap * sizeof(QueueRec));
         wb.sun[q] = take(qcap * 4);
     }
     wb.counters = reinterpret_cast<uint32_t*>(take((size_t)kCounters * 4));

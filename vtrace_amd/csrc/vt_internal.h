@@ -57,8 +57,8 @@ struct WaveBuffers {
     uint32_t npaths = 0;        // capacity in pixel samples
     void* L = nullptr;          // float4 per path: sky terms, w = primary-hit flag
     void* Ls = nullptr;         // float4 per path with a primary hit: sun terms, w = shadow flag
-    void* br = nullptr;         // float4 per path with a primary hit: throughput, RNG state
-    void* rec[2] = {nullptr, nullptr};      // 64 B queue records per queue slot
+    void* br = nullptr;         // float4 per path past its first bounce: throughput
+    void* rec[2] = {nullptr, nullptr};      // 48 B queue records per queue slot
     void* sun[2] = {nullptr, nullptr};      // u32 per slot: sun list (queue slots facing the sun)
     uint32_t* counters = nullptr;           // 128 u32: queue sizes and fetch counters
     void* tables = nullptr;                 // ReqGPU[VT_MAX_BATCH] then BatchGPU
