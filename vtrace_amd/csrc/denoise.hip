@@ -20,13 +20,15 @@
 //                    one tap) are 16-byte loads from NHWC tensors, B fragments from the
 //                    [co][tap][ci] weight rows.  Weights are the first MFMA operand, so a
 //                    lane's outputs are 4 consecutive channels of one pixel.  For inputs of
-//                    up to 96 channels the same computation runs from LDS:
+//                    up to 160 channels the same computation runs from LDS:
 //                    k_conv3x3_res (persistent workgroups of 8 waves with the layer's weights
 //                    resident in LDS, 16 x 16 pixel tiles with their halo, the next tile
 //                    fetched into registers during the current one, a software-pipelined K
-//                    loop with no barrier) where weights and a tile fit in 160 KiB, else
-//                    k_conv3x3_tile (16 x 8 tiles, weights streamed through LDS in 32-wide K
-//                    slices).
+//                    loop with no barrier) for up to 96 channels where weights and a tile
+//                    fit in 160 KiB, else k_conv3x3_tile (16 x 8 tiles, weights streamed
+//                    through LDS in 32-wide K slices).  Both can pool their output (2x2
+//                    mean) in the epilogue and read the decoder's upsampled channels from
+//                    the lower level in their staging.
 //   k_conv3x3_wgrad  dW[co][tap][ci] = sum_p dY[p][co] X[p+tap][ci] over a chunk of pixels:
 //                    32-pixel tiles of dY and the shifted X are staged transposed in LDS so
 //                    both MFMA operands read 8 consecutive pixels; dbias in the same pass.
@@ -81,9 +83,67 @@ struct ConvArgs {
     const float* bias;       // [Cout] or null
     void* out;               // N x H x W x out_stride, channels [out_off, out_off + Cout)
     const uint16_t* mask;    // MASK: N x H x W x mask_stride; out = 0 where mask <= 0
+    uint16_t* pool;          // optional: 2x2 mean pool of the BF16 output, N x H/2 x W/2 x Cout
+    const uint16_t* up;      // optional: input channels [0, up_C) read from this N x H/2 x W/2 x up_C
+                             // tensor at (y/2, x/2) (nearest upsampling), the rest from `in`
+    uint32_t up_C;
     uint32_t in_stride, out_stride, out_off, mask_stride;
     uint32_t N, H, W, Cin, Cout;
 };
+
+// Epilogue of the LDS-tiled kernels (waves of two 16-pixel rows, weights as the first MFMA
+// operand: a lane holds channels 16j + 4g .. 16j + 4g + 3 of pixel r of rows m = 0, 1) for a
+// layer whose BF16 output is also pooled (a.pool; H and W even, 4-aligned strides and
+// offset).  Rows m = 0, 1 are an even/odd row pair (tile origins are multiples of 8) and lanes
+// r, r ^ 1 an even/odd column pair, so the pooled value comes from this lane and its
+// neighbour with k_pool_fwd's operations: the four BF16 outputs summed in row-major order,
+// times 0.25, rounded to BF16.  Every lane reaches the shuffles; only stores are guarded.
+template <int NT, bool RELU>
+__device__ __forceinline__ void epilogue_pool(const ConvArgs& a, const f32x4 (&acc)[2][NT], const float (&bias)[NT][4],
+                                              uint32_t n, int x, int y0, uint32_t g, uint32_t r) {
+    const bool xin = x < (int)a.W, yin = y0 < (int)a.H;
+    const size_t img = (size_t)n * a.H * a.W;
+    const uint32_t hw = a.W / 2u;
+    const size_t q = ((size_t)n * (a.H / 2u) + (size_t)(y0 >> 1)) * hw + (size_t)(x >> 1);
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const uint32_t co0 = (uint32_t)j * 16u + 4u * g;
+        const bool cin = co0 < a.Cout;
+        uint32_t pk[2][2];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            float v[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                v[i] = acc[m][j][i] + bias[j][i];
+                if (RELU) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
+            }
+            pk[m][0] = pk2bf(v[0], v[1]);
+            pk[m][1] = pk2bf(v[2], v[3]);
+            if (xin && yin && cin) {
+                const size_t p = img + (size_t)(y0 + m) * a.W + (size_t)x;
+                *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(a.out) + p * a.out_stride + a.out_off + co0) =
+                    make_uint2(pk[m][0], pk[m][1]);
+            }
+        }
+        uint32_t nb[2][2];
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int t = 0; t < 2; ++t) nb[m][t] = (uint32_t)__shfl_xor((int)pk[m][t], 1, 64);
+        if ((r & 1u) == 0u && xin && yin && cin) {
+            float o[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int t = i >> 1, sh = 16 * (i & 1);
+                const float s = ((bf2f(pk[0][t] >> sh) + bf2f(nb[0][t] >> sh)) + bf2f(pk[1][t] >> sh)) +
+                                bf2f(nb[1][t] >> sh);
+                o[i] = 0.25f * s;
+            }
+            *reinterpret_cast<uint2*>(a.pool + q * a.Cout + co0) = make_uint2(pk2bf(o[0], o[1]), pk2bf(o[2], o[3]));
+        }
+    }
+}
 
 template <int NT, bool RELU, bool OUTF, bool MASK>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
@@ -165,7 +225,8 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
 // operands and their order are those of k_conv3x3 (steps past 9 Cin add zero products), so
 // every output is the same sum in the same order.
 constexpr int kTileX = 16;
-constexpr uint32_t kTileMaxCin = 96;
+constexpr uint32_t kTileMaxCin = 160;   // the sliced kernel (the resident one: kResMaxCin)
+constexpr uint32_t kResMaxCin = 96;
 
 // LDS row pitch (elements) of a staged 32-wide K slice of the weights: 80 B rows, so the 16
 // rows one MFMA operand reads start in different banks
@@ -190,7 +251,8 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(ConvArgs a) {
     // the end reads the padding, so no select is needed)
     const uint32_t cpp = a.Cin / 8u + 1u;
     const uint32_t total = (uint32_t)((TY + 2) * (kTileX + 2)) * cpp;
-    // i / cpp as (i * inv) >> 20: exact for i < 2^20 / 13 (cpp <= 13, i < 4212)
+    // i / cpp as (i * inv) >> 20: exact while i < 2^20 / cpp (cpp <= 21, i < 6000); i * inv
+    // stays below 2^32 since cpp >= 2
     const uint32_t inv = ((1u << 20) + cpp - 1u) / cpp;
     // staging in batches: all loads of a batch are issued before its LDS stores, so a thread
     // waits for one round trip per batch instead of one per 16-byte chunk
@@ -205,8 +267,11 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(ConvArgs a) {
             const int y = ty0 + ly - 1, x = tx0 + lx - 1;
             v[b] = u32x4{0u, 0u, 0u, 0u};
             if (i < total && c8 + 1u < cpp && y >= 0 && y < (int)a.H && x >= 0 && x < (int)a.W)
-                v[b] = *reinterpret_cast<const u32x4*>(a.in + (img + (size_t)y * a.W + (size_t)x) * a.in_stride +
-                                                       8u * c8);
+                v[b] = *reinterpret_cast<const u32x4*>(
+                    (a.up && 8u * c8 < a.up_C)
+                        ? a.up + (((size_t)n * (a.H >> 1) + (size_t)(y >> 1)) * (a.W >> 1) + (size_t)(x >> 1)) * a.up_C +
+                              8u * c8
+                        : a.in + (img + (size_t)y * a.W + (size_t)x) * a.in_stride + 8u * c8);
         }
 #pragma unroll
         for (uint32_t b = 0; b < kBatch; ++b) {
@@ -261,7 +326,8 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(ConvArgs a) {
     fetch(0, nx, nok);
     store(0, nx, nok);
     __syncthreads();
-    // k = 32 s + 8 g as (tap, channel): tap = (k * invc) >> 20, exact for k < 9 * 96
+    // k = 32 s + 8 g as (tap, channel): tap = (k * invc) >> 20, exact while k < 2^20 / Cin
+    // (k < 9 Cin + 32 <= 1472)
     const uint32_t invc = ((1u << 20) + a.Cin - 1u) / a.Cin;
     const uint32_t rowbase = (uint32_t)MT * wave;
     for (uint32_t sl = 0; sl < S; ++sl) {
@@ -301,6 +367,20 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(ConvArgs a) {
     // BF16 outputs at 4-channel aligned strides and offsets go out as one 8-byte store per
     // four channels (mask read the same way); otherwise one store per channel
     const int x = tx0 + (int)r;
+    if constexpr (MT == 2 && !OUTF && !MASK) {
+        if (a.pool) {
+            float bias[NT][4];
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint32_t co = (uint32_t)j * 16u + 4u * g + (uint32_t)i;
+                    bias[j][i] = (a.bias && co < a.Cout) ? a.bias[co] : 0.0f;
+                }
+            epilogue_pool<NT, RELU>(a, acc, bias, n, x, ty0 + (int)rowbase, g, r);
+            return;
+        }
+    }
     if (x >= (int)a.W) return;
     const bool vec = !OUTF && a.Cout % 4u == 0 && a.out_stride % 4u == 0 && a.out_off % 4u == 0 &&
                      (!MASK || a.mask_stride % 4u == 0);
@@ -432,7 +512,11 @@ __global__ __launch_bounds__(64 * NW) void k_conv3x3_res(ConvArgs a, uint32_t nt
             const int y = ty0 + (int)(pix / (uint32_t)(kTileX + 2)) - 1, x = tx0 + (int)(pix % (uint32_t)(kTileX + 2)) - 1;
             pre[b] = u32x4{0u, 0u, 0u, 0u};
             if (i < total && c8 + 1u < cpp && y >= 0 && y < (int)a.H && x >= 0 && x < (int)a.W)
-                pre[b] = *reinterpret_cast<const u32x4*>(a.in + (img + (size_t)y * a.W + (size_t)x) * a.in_stride + 8u * c8);
+                pre[b] = *reinterpret_cast<const u32x4*>(
+                    (a.up && 8u * c8 < a.up_C)
+                        ? a.up + (((size_t)n * (a.H >> 1) + (size_t)(y >> 1)) * (a.W >> 1) + (size_t)(x >> 1)) * a.up_C +
+                              8u * c8
+                        : a.in + (img + (size_t)y * a.W + (size_t)x) * a.in_stride + 8u * c8);
         }
     };
     uint32_t t = blockIdx.x;
@@ -512,6 +596,12 @@ __global__ __launch_bounds__(64 * NW) void k_conv3x3_res(ConvArgs a, uint32_t nt
         }
         // D layout: lane holds output channels 16j + 4g .. 16j + 4g + 3 of pixel r of row m
         const int x = tx0 + (int)r;
+        if constexpr (MT == 2 && !OUTF && !MASK) {
+            if (a.pool) {
+                epilogue_pool<NT, RELU>(a, acc, bias, n, x, ty0 + (int)rowbase, g, r);
+                continue;
+            }
+        }
         if (x >= (int)a.W) continue;
         const size_t img = (size_t)n * a.H * a.W;
 #pragma unroll
@@ -1139,6 +1229,7 @@ uint32_t device_cus() {
 template <int MT, int NW, int NT, bool RELU, bool OUTF, bool MASK>
 int launch_res(const ConvArgs& a, hipStream_t s) {
     constexpr uint32_t kMaxLds = 160u * 1024u;
+    if (a.Cin > kResMaxCin) return 1;
     const size_t lds = conv_res_lds<NT>(NW * MT, a.Cin);
     if (lds > kMaxLds) return 1;
     const uint32_t ntx = (a.W + kTileX - 1) / kTileX, nty = (a.H + NW * MT - 1u) / (NW * MT);
@@ -1204,8 +1295,13 @@ int launch_conv_nt(uint32_t nt, const ConvArgs& a, hipStream_t s) {
 }
 
 // forward conv of layer li: in (Cin channels, stride in.C) -> out (stride out_stride, offset out_off)
+// pool (optional, BF16 outputs only): also writes the 2x2 mean pool of the output (N x H/2 x W/2
+// x Cout), fused into the epilogue of the LDS-tiled kernels, else by k_pool_fwd afterwards
+// up (optional): input channels [0, up.C) are the nearest 2x upsampling of `up` (N x H/2 x W/2),
+// read by the LDS-tiled kernels' staging; returns 1 (nothing launched) when the layer does
+// not run on those kernels, so the caller materialises the upsampling instead
 int conv_fwd(int li, const Tensor& in, void* out, uint32_t out_stride, uint32_t out_off, uint32_t N, uint32_t H,
-             uint32_t W, bool outf, hipStream_t s) {
+             uint32_t W, bool outf, hipStream_t s, uint16_t* pool = nullptr, const Tensor* up = nullptr) {
     const Layer& L = net.L[li];
     ConvArgs a{};
     a.in = in.p;
@@ -1219,8 +1315,23 @@ int conv_fwd(int li, const Tensor& in, void* out, uint32_t out_stride, uint32_t 
     a.Cin = L.cin;
     a.Cout = L.cout;
     const uint32_t nt = (L.cout + 15u) / 16u;
+    const bool tiled = a.Cin <= kTileMaxCin && a.Cin % 8u == 0 && a.in_stride % 8u == 0 && nt <= 6;
+    if (up) {
+        if (!tiled || up->C % 8u != 0 || H % 2u != 0 || W % 2u != 0) return 1;
+        a.up = up->p;
+        a.up_C = up->C;
+    }
     if (outf) return launch_conv_nt<false, true, false>(nt, a, s);
-    return L.relu ? launch_conv_nt<true, false, false>(nt, a, s) : launch_conv_nt<false, false, false>(nt, a, s);
+    const bool fuse = pool && a.Cin <= kTileMaxCin && a.Cin % 8u == 0 && a.in_stride % 8u == 0 && nt <= 6 &&
+                      a.Cout % 4u == 0 && a.out_stride % 4u == 0 && a.out_off % 4u == 0 && H % 2u == 0 && W % 2u == 0;
+    a.pool = fuse ? pool : nullptr;
+    const int rc = L.relu ? launch_conv_nt<true, false, false>(nt, a, s) : launch_conv_nt<false, false, false>(nt, a, s);
+    if (rc == 0 && pool && !fuse) {
+        const size_t n8 = (size_t)N * (H / 2) * (W / 2) * (L.cout / 8);
+        hipLaunchKernelGGL(k_pool_fwd, dim3(blocks(n8)), dim3(256), 0, s, static_cast<const uint16_t*>(out), out_stride,
+                           out_off, pool, N, H / 2, W / 2, L.cout);
+    }
+    return rc;
 }
 
 // backward-data of layer li: g (Cout channels, stride g.C, g.C % 8 == 0) -> dX (Cin channels),
@@ -1289,17 +1400,17 @@ void prep_weights(hipStream_t s) {
 }
 
 // forward pass on net.x0 (N x H x W, H and W multiples of 16) -> net.y
-int forward(uint32_t N, uint32_t H, uint32_t W, hipStream_t s) {
+// fuse_up: read the decoder's upsampled channels straight from the lower level in the
+// convolutions' staging instead of materialising them in cat[l] (inference; training keeps
+// cat[l] whole, its weight gradient reads it)
+int forward(uint32_t N, uint32_t H, uint32_t W, hipStream_t s, bool fuse_up = false) {
     prep_weights(s);
     int rc = 0;
     Tensor in = net.x0;
     for (int l = 0; l < 4 && rc == 0; ++l) {
         const uint32_t h = H >> l, w = W >> l;
         rc |= conv_fwd(2 * l, in, net.a[l].p, net.a[l].C, 0, N, h, w, false, s);
-        rc |= conv_fwd(2 * l + 1, net.a[l], net.cat[l].p, net.cat[l].C, net.U[l], N, h, w, false, s);
-        const size_t n8 = (size_t)N * (h / 2) * (w / 2) * (net.E[l] / 8);
-        hipLaunchKernelGGL(k_pool_fwd, dim3(blocks(n8)), dim3(256), 0, s, net.cat[l].p, net.cat[l].C, net.U[l],
-                           net.pool[l].p, N, h / 2, w / 2, net.E[l]);
+        rc |= conv_fwd(2 * l + 1, net.a[l], net.cat[l].p, net.cat[l].C, net.U[l], N, h, w, false, s, net.pool[l].p);
         in = net.pool[l];
     }
     const uint32_t h4 = H >> 4, w4 = W >> 4;
@@ -1308,11 +1419,18 @@ int forward(uint32_t N, uint32_t H, uint32_t W, hipStream_t s) {
     Tensor prev = net.b2;
     for (int l = 3; l >= 0 && rc == 0; --l) {
         const uint32_t h = H >> l, w = W >> l;
-        const size_t n8 = (size_t)N * h * w * (net.U[l] / 8);
-        hipLaunchKernelGGL(k_up_fwd, dim3(blocks(n8)), dim3(256), 0, s, prev.p, net.cat[l].p, net.cat[l].C, 0u, N, h,
-                           w, net.U[l]);
         const int li = 10 + 2 * (3 - l);
-        rc |= conv_fwd(li, net.cat[l], net.d1[l].p, net.d1[l].C, 0, N, h, w, false, s);
+        int fused = 1;
+        if (fuse_up) {
+            fused = conv_fwd(li, net.cat[l], net.d1[l].p, net.d1[l].C, 0, N, h, w, false, s, nullptr, &prev);
+            rc |= fused < 0 ? fused : 0;
+        }
+        if (fused == 1) {
+            const size_t n8 = (size_t)N * h * w * (net.U[l] / 8);
+            hipLaunchKernelGGL(k_up_fwd, dim3(blocks(n8)), dim3(256), 0, s, prev.p, net.cat[l].p, net.cat[l].C, 0u, N,
+                               h, w, net.U[l]);
+            rc |= conv_fwd(li, net.cat[l], net.d1[l].p, net.d1[l].C, 0, N, h, w, false, s);
+        }
         rc |= conv_fwd(li + 1, net.d1[l], net.d2[l].p, net.d2[l].C, 0, N, h, w, false, s);
         prev = net.d2[l];
     }
@@ -1405,7 +1523,7 @@ int denoise_device(const float4* rad, const float4* gbuf, uint32_t h, uint32_t w
     hipEventRecord(net.e0, s);
     hipLaunchKernelGGL(k_features, dim3(blocks((size_t)hp * wp)), dim3(256), 0, s, rad, gbuf, h, w, hp, wp,
                        spp_feature(spp), net.x0.p);
-    if (forward(1, hp, wp, s) != 0) return -1;
+    if (forward(1, hp, wp, s, true) != 0) return -1;
     hipLaunchKernelGGL(k_output, dim3(blocks((size_t)h * w)), dim3(256), 0, s, net.y, net.x0.p, h, w, wp, rad,
                        blend_alpha(spp), out_dev);
     hipEventRecord(net.e1, s);
