@@ -485,18 +485,20 @@ __device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], 
                 stage_mask(V.masks, entry, mslot);
             }
         }
+        // (every value below is computed unconditionally and chosen with selects: arms written
+        // inside ?: become branches, which lanes of different kinds would take one after the other)
         const bool emp = vt_brick_empty(entry);
         const int lx = v[0] - b8[0], ly = v[1] - b8[1], lz = v[2] - b8[2];
-        int sz = 1;
-        if (((blk >> (uint32_t)(2 * (lx >> 2) + 8 * (ly >> 2) + 32 * (lz >> 2))) & 0x330033ull) == 0ull) sz = 4;
-        else if (((blk >> (uint32_t)((lx >> 1) + 4 * (ly >> 1) + 16 * (lz >> 1))) & 1ull) == 0ull) sz = 2;
-        if (!emp && sz == 1) {
-            const uint32_t bit = (uint32_t)lx + 8u * (uint32_t)ly;
-            if (((uint32_t)(mask_slice(mslot, lz) >> bit) & 1u) != 0u) {
-                li = bit + 64u * (uint32_t)lz;
-                hit = true;
-                break;
-            }
+        const bool oct_empty =
+            ((blk >> (uint32_t)(2 * (lx >> 2) + 8 * (ly >> 2) + 32 * (lz >> 2))) & 0x330033ull) == 0ull;
+        const bool blk_empty = ((blk >> (uint32_t)((lx >> 1) + 4 * (ly >> 1) + 16 * (lz >> 1))) & 1ull) == 0ull;
+        const int sz = oct_empty ? 4 : (blk_empty ? 2 : 1);
+        const uint32_t bit = (uint32_t)lx + 8u * (uint32_t)ly;
+        const uint64_t slice = mask_slice(mslot, lz);
+        if (!emp && sz == 1 && ((uint32_t)(slice >> bit) & 1u) != 0u) {
+            li = bit + 64u * (uint32_t)lz;
+            hit = true;
+            break;
         }
         // candidates: A = the centred cube (empty brick) or the block box, B = the anchored cube
         // (empty brick) or the block box again
@@ -509,10 +511,13 @@ __device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], 
             const int bl = v[a] & -sz;
             lo[a] = emp ? b8[a] - rad : bl;
             hi[a] = emp ? b8[a] + 8 + rad : bl + sz;
-            loB[a] = emp ? (st[a] < 0 ? b8[a] + 8 - k8 : b8[a]) : lo[a];
-            hiB[a] = emp ? (st[a] < 0 ? b8[a] + 8 : b8[a] + k8) : hi[a];
-            tA[a] = st[a] != 0 ? ((float)(st[a] > 0 ? hi[a] : lo[a]) - q[a]) * rinv[a] : INFINITY;
-            tB[a] = st[a] != 0 ? ((float)(st[a] > 0 ? hiB[a] : loB[a]) - q[a]) * rinv[a] : INFINITY;
+            const int cl = st[a] < 0 ? b8[a] + 8 - k8 : b8[a], ch = st[a] < 0 ? b8[a] + 8 : b8[a] + k8;
+            loB[a] = emp ? cl : lo[a];
+            hiB[a] = emp ? ch : hi[a];
+            const int ea = st[a] > 0 ? hi[a] : lo[a], eb = st[a] > 0 ? hiB[a] : loB[a];
+            const float ta = ((float)ea - q[a]) * rinv[a], tb = ((float)eb - q[a]) * rinv[a];
+            tA[a] = st[a] != 0 ? ta : INFINITY;
+            tB[a] = st[a] != 0 ? tb : INFINITY;
         }
         const float tcA = gmin(gmin(gmin(INFINITY, tA[0]), tA[1]), tA[2]);
         const float tcB = gmin(gmin(gmin(INFINITY, tB[0]), tB[1]), tB[2]);
@@ -525,10 +530,14 @@ __device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], 
             const float t = useB ? tB[a] : tA[a];
             const bool ex = st[a] != 0 && t <= tmin;
             const float x = q[a] + tmin * u[a];
-            int f = (int)floorf(x);
-            f = f < l ? l : (f > h - 1 ? h - 1 : f);
-            q[a] = ex ? (float)(st[a] > 0 ? h : l) : x;
-            v[a] = ex ? (st[a] > 0 ? h : l - 1) : f;
+            const int f0 = (int)floorf(x);
+            const int f1 = f0 < l ? l : f0;
+            const int f = f1 > h - 1 ? h - 1 : f1;
+            const int pe = st[a] > 0 ? h : l;
+            const float qe = (float)pe;
+            const int ve = st[a] > 0 ? h : l - 1;
+            q[a] = ex ? qe : x;
+            v[a] = ex ? ve : f;
             mk |= ex ? 1u << a : 0u;
         }
         mask = mk;
