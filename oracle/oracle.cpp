@@ -638,8 +638,8 @@ DdaResult dda(const Vol& V, const float mpos[3], const int v0[3], const float md
 
 // Secondary-ray walk (no reference counterpart; this project's rule).  From position q in voxel v
 // the ray crosses, one box per step, the largest box known to be empty that holds v: in an empty
-// brick the cube of radius r centred on it or the cube of side K_o anchored at it towards the
-// ray's octant o, whichever exit lies further along the ray (the centred one on ties); in a
+// brick the cube of side K_o anchored at it towards the ray's octant o when K_o - 1 > r (its exit
+// then lies further along the ray on every axis), else the cube of radius r centred on it; in a
 // non-empty brick the 4^3 octant holding v if all its 2^3 blocks are empty, else the 2^3 block
 // holding v if empty, else v itself unless v is occupied (the hit).  A step moves q to the
 // box's nearest exit plane along the ray (ties: every axis at that distance): exit axes step
@@ -672,22 +672,15 @@ DdaResult dda2(const Vol& V, const float p[3], const int v0[3], const float m[3]
         }
         int lo[3], hi[3];
         if (!V.brick_nonempty[bi]) {
+            // the anchored cube when K_o - 1 > r: its far faces then lie beyond the centred
+            // cube's on every axis the ray moves along (K_o = r + 1: the same faces)
             const int rad = 8 * (int)V.brick_radius[bi];
             const int k8 = 8 * (int)V.brick_cube[oct][bi];
-            int alo[3], ahi[3];
-            float tc = kInf, ta = kInf;
+            const bool anchored = k8 - 8 > rad;
             for (int a = 0; a < 3; ++a) {
-                lo[a] = b8[a] - rad;
-                hi[a] = b8[a] + 8 + rad;
-                alo[a] = st[a] < 0 ? b8[a] + 8 - k8 : b8[a];
-                ahi[a] = st[a] < 0 ? b8[a] + 8 : b8[a] + k8;
-                if (st[a] != 0) {
-                    tc = gmin(tc, ((float)(st[a] > 0 ? hi[a] : lo[a]) - q[a]) * rinv[a]);
-                    ta = gmin(ta, ((float)(st[a] > 0 ? ahi[a] : alo[a]) - q[a]) * rinv[a]);
-                }
+                lo[a] = anchored ? (st[a] < 0 ? b8[a] + 8 - k8 : b8[a]) : b8[a] - rad;
+                hi[a] = anchored ? (st[a] < 0 ? b8[a] + 8 : b8[a] + k8) : b8[a] + 8 + rad;
             }
-            if (ta > tc)
-                for (int a = 0; a < 3; ++a) { lo[a] = alo[a]; hi[a] = ahi[a]; }
         } else {
             const uint64_t blk = V.brick_blocks[bi];
             const int lx = v[0] - b8[0], ly = v[1] - b8[1], lz = v[2] - b8[2];

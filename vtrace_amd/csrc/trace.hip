@@ -433,16 +433,18 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
 
 // Secondary-ray walk (no reference counterpart; this project's rule).  From position q in
 // voxel v the ray crosses, one box per step, the largest box known to be empty that holds v:
-//   * in an empty brick, of the cube of radius r centred on the brick and the cube of side K_o
-//     anchored at it towards the ray's octant o (both from the brick map entry), the one whose
-//     exit lies further along the ray (the centred one on ties);
+//   * in an empty brick, the cube of side K_o anchored at it towards the ray's octant o when
+//     K_o - 1 > r (its far faces are then beyond those of the centred cube on every axis, so
+//     its exit is further along the ray), else the cube of radius r centred on the brick (both
+//     from the brick map entry);
 //   * in a non-empty brick, the brick's 4^3 octant holding v if its 8 blocks are empty, else the
 //     2^3 block holding v if it is empty (block mask), else v itself -- unless v is occupied
 //     (the LDS-staged occupancy mask), which ends the walk with a hit.
 // A step moves q to the box's nearest exit plane along the ray (ties: every axis at that
 // distance): exit axes step into the voxel beyond the plane, the other axes take floor(q)
-// clamped to the box.  Every step runs the same instructions whatever the box, so lanes of a
-// wave walking empty space and lanes walking voxels share each iteration.
+// clamped to the box.  Every step runs the same instructions whatever the box (values chosen
+// by selects), so lanes of a wave walking empty space and lanes walking voxels share each
+// iteration.
 __device__ __forceinline__ uint64_t gld64(const uint64_t* p) {
     return *(const __attribute__((address_space(1))) uint64_t*)(p);
 }
@@ -495,40 +497,35 @@ __device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], 
         const int sz = oct_empty ? 4 : (blk_empty ? 2 : 1);
         const uint32_t bit = (uint32_t)lx + 8u * (uint32_t)ly;
         const uint64_t slice = mask_slice(mslot, lz);
-        if (!emp && sz == 1 && ((uint32_t)(slice >> bit) & 1u) != 0u) {
+        if (!emp & (sz == 1) & (((uint32_t)(slice >> bit) & 1u) != 0u)) {
             li = bit + 64u * (uint32_t)lz;
             hit = true;
             break;
         }
-        // candidates: A = the centred cube (empty brick) or the block box, B = the anchored cube
-        // (empty brick) or the block box again
+        // the box: in an empty brick the anchored cube when K_o - 1 > r (it then reaches further
+        // on every axis), else the centred cube; in a non-empty brick the octant, block or voxel
         const int rad = 8 * (int)vt_brick_radius(entry);
         const int k8 = 8 * (int)vt_brick_cube(entry, oct);
-        int lo[3], hi[3], loB[3], hiB[3];
-        float tA[3], tB[3];
+        const bool anc = k8 - 8 > rad;
+        int lo[3], hi[3];
+        float tt[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const int bl = v[a] & -sz;
-            lo[a] = emp ? b8[a] - rad : bl;
-            hi[a] = emp ? b8[a] + 8 + rad : bl + sz;
-            const int cl = st[a] < 0 ? b8[a] + 8 - k8 : b8[a], ch = st[a] < 0 ? b8[a] + 8 : b8[a] + k8;
-            loB[a] = emp ? cl : lo[a];
-            hiB[a] = emp ? ch : hi[a];
-            const int ea = st[a] > 0 ? hi[a] : lo[a], eb = st[a] > 0 ? hiB[a] : loB[a];
-            const float ta = ((float)ea - q[a]) * rinv[a], tb = ((float)eb - q[a]) * rinv[a];
-            tA[a] = st[a] != 0 ? ta : INFINITY;
-            tB[a] = st[a] != 0 ? tb : INFINITY;
+            const int al = st[a] < 0 ? b8[a] + 8 - k8 : b8[a], ah = st[a] < 0 ? b8[a] + 8 : b8[a] + k8;
+            const int cl = anc ? al : b8[a] - rad, ch = anc ? ah : b8[a] + 8 + rad;
+            lo[a] = emp ? cl : bl;
+            hi[a] = emp ? ch : bl + sz;
+            const int e = st[a] > 0 ? hi[a] : lo[a];
+            const float t = ((float)e - q[a]) * rinv[a];
+            tt[a] = st[a] != 0 ? t : INFINITY;
         }
-        const float tcA = gmin(gmin(gmin(INFINITY, tA[0]), tA[1]), tA[2]);
-        const float tcB = gmin(gmin(gmin(INFINITY, tB[0]), tB[1]), tB[2]);
-        const bool useB = tcB > tcA;
-        const float tmin = useB ? tcB : tcA;
+        const float tmin = gmin(gmin(gmin(INFINITY, tt[0]), tt[1]), tt[2]);
         uint32_t mk = 0;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const int l = useB ? loB[a] : lo[a], h = useB ? hiB[a] : hi[a];
-            const float t = useB ? tB[a] : tA[a];
-            const bool ex = st[a] != 0 && t <= tmin;
+            const int l = lo[a], h = hi[a];
+            const bool ex = (st[a] != 0) & (tt[a] <= tmin);
             const float x = q[a] + tmin * u[a];
             const int f0 = (int)floorf(x);
             const int f1 = f0 < l ? l : f0;
