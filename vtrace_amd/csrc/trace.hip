@@ -574,20 +574,18 @@ __device__ __forceinline__ SlabR slab_unit(V3 o, V3 d) {
     SlabR s{-INFINITY, INFINITY, 0, true};
     const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
 #pragma unroll
+    // (both cases computed and chosen by selects; the quotients of a parallel axis are unused)
     for (int a = 0; a < 3; ++a) {
-        float tn, tf;
-        if (dd[a] != 0.0f) {
-            float t1 = (-0.5f - oo[a]) / dd[a];
-            float t2 = (0.5f - oo[a]) / dd[a];
-            tn = gmin(t1, t2);
-            tf = gmax(t1, t2);
-        } else {
-            if (oo[a] < -0.5f || oo[a] > 0.5f) s.ok = false;
-            tn = -INFINITY;
-            tf = INFINITY;
-        }
-        if (tn > s.tn) { s.tn = tn; s.axis = a; }
-        if (tf < s.tf) s.tf = tf;
+        const float t1 = (-0.5f - oo[a]) / dd[a];
+        const float t2 = (0.5f - oo[a]) / dd[a];
+        const bool mv = dd[a] != 0.0f;
+        const float tn = mv ? gmin(t1, t2) : -INFINITY;
+        const float tf = mv ? gmax(t1, t2) : INFINITY;
+        s.ok = s.ok & (mv | !((oo[a] < -0.5f) | (oo[a] > 0.5f)));
+        const bool nt = tn > s.tn;
+        s.tn = nt ? tn : s.tn;
+        s.axis = nt ? a : s.axis;
+        s.tf = tf < s.tf ? tf : s.tf;
     }
     s.ok = s.ok && s.tn <= s.tf;
     return s;
@@ -599,23 +597,18 @@ __device__ __forceinline__ bool slab_node(V3 o, V3 d, V3 id, const BvhNode& N, f
     bool ok = true;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        float n_, f_;
-        if (dd[a] != 0.0f) {
-            float t1 = (N.lo[a] - oo[a]) * ii[a];
-            float t2 = (N.hi[a] - oo[a]) * ii[a];
-            n_ = gmin(t1, t2);
-            f_ = gmax(t1, t2);
-        } else {
-            if (oo[a] < N.lo[a] || oo[a] > N.hi[a]) ok = false;
-            n_ = -INFINITY;
-            f_ = INFINITY;
-        }
-        if (n_ > tn) tn = n_;
-        if (f_ < tf) tf = f_;
+        const float t1 = (N.lo[a] - oo[a]) * ii[a];
+        const float t2 = (N.hi[a] - oo[a]) * ii[a];
+        const bool mv = dd[a] != 0.0f;
+        const float n_ = mv ? gmin(t1, t2) : -INFINITY;
+        const float f_ = mv ? gmax(t1, t2) : INFINITY;
+        ok = ok & (mv | !((oo[a] < N.lo[a]) | (oo[a] > N.hi[a])));
+        tn = n_ > tn ? n_ : tn;
+        tf = f_ < tf ? f_ : tf;
     }
     *tn_out = tn;
     *tf_out = tf;
-    return ok && tn <= tf;
+    return ok & (tn <= tf);
 }
 
 // Primary ray: enter instance i from outside along (O, D), rebuilding the fragment inputs of
@@ -636,12 +629,10 @@ __device__ bool enter_instance(const SceneGPU& S, const FrameGPU& F, uint32_t i,
     const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (a == s.axis) {
-            mp[a] = dd[a] > 0.0f ? -0.5f : 0.5f;
-        } else {
-            float x = oo[a] + s.tn * dd[a];
-            mp[a] = x < -0.5f ? -0.5f : (x > 0.5f ? 0.5f : x);
-        }
+        const float face = dd[a] > 0.0f ? -0.5f : 0.5f;
+        const float x = oo[a] + s.tn * dd[a];
+        const float xc = x < -0.5f ? -0.5f : (x > 0.5f ? 0.5f : x);
+        mp[a] = a == s.axis ? face : xc;
     }
     V3 mdir;
     {
@@ -950,13 +941,10 @@ __device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const QueueRec& 
             const float size[3] = {I->size[0], I->size[1], I->size[2]};
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                float mp;
-                if (a == sl.axis) {
-                    mp = dd[a] > 0.0f ? -0.5f : 0.5f;
-                } else {
-                    const float x = oo[a] + sl.tn * dd[a];
-                    mp = x < -0.5f ? -0.5f : (x > 0.5f ? 0.5f : x);
-                }
+                const float face = dd[a] > 0.0f ? -0.5f : 0.5f;
+                const float x = oo[a] + sl.tn * dd[a];
+                const float xc = x < -0.5f ? -0.5f : (x > 0.5f ? 0.5f : x);
+                const float mp = a == sl.axis ? face : xc;
                 mpos[a] = (mp + 0.5f) * size[a];
                 v0[a] = (int)floorf(gmin(mpos[a], size[a] - 1.0f));
             }
