@@ -20,9 +20,10 @@
 // rays and the diffuse bounce rays of the surfaces queued at that depth, appending the
 // bounce hits (wave-aggregated atomics) to a compacted queue for the next depth;
 // k_accumulate sums the samples of each pixel.
-// Every kernel is persistent: each wave fetches 64 consecutive work items at a time from
-// an atomic counter, so terminated paths never occupy lanes.  Each path's arithmetic is
-// independent of queue order, so results are deterministic.
+// The kernels are persistent: a k_primary wave fetches 64 consecutive work items at a time
+// from an atomic counter; a k_secondary wave refills its lanes as their rays finish, so
+// neither terminated paths nor the rest of a wave waiting on one long walk occupy lanes.
+// Each path's arithmetic is independent of queue order, so results are deterministic.
 //
 // DDA: primary rays walk the reference's voxel DDA (empty 8^3 bricks crossed by a tight loop
 // without memory access); secondary rays cross boxes known to be empty (brick cubes, 4^3
@@ -449,106 +450,122 @@ __device__ __forceinline__ uint64_t gld64(const uint64_t* p) {
     return *(const __attribute__((address_space(1))) uint64_t*)(p);
 }
 
-template <bool ST>
-__device__ __forceinline__ void dda2_run(const VolumeDesc& V, const float p[3], const int v0[3], V3 m, Dda& r,
-                                         Ctr<ST>& ctr, const Lds& mslot) {
-    const float len = sqrtf((m.x * m.x + m.y * m.y) + m.z * m.z);
-    const float mm[3] = {m.x, m.y, m.z};
+// One walk, held in registers between the pool rounds of k_secondary: box_walk_begin, then
+// box_walk_step until it returns false (the walk left the volume, hit a voxel, or reached the
+// guard of 2 (dx + dy + dz) + 64 steps).
+struct BoxWalk {
+    const uint32_t* brick_map;
+    const uint32_t* masks;
+    const uint64_t* blocks;
+    int n[3];
+    uint32_t bdx, bdy, guard, oct;
     float u[3], rinv[3];
     int st[3];
+    int v[3];
+    float q[3];
+    uint32_t steps, mask, entry, cur, li;
+    uint64_t blk;
+    bool hit;
+};
+
+__device__ __forceinline__ void box_walk_begin(const VolumeDesc& V, const float p[3], const int v0[3], V3 m,
+                                               BoxWalk& W) {
+    const float len = sqrtf((m.x * m.x + m.y * m.y) + m.z * m.z);
+    const float mm[3] = {m.x, m.y, m.z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        st[a] = mm[a] > 0.0f ? 1 : (mm[a] < 0.0f ? -1 : 0);
-        u[a] = mm[a] / len;
-        rinv[a] = len / mm[a];
+        W.st[a] = mm[a] > 0.0f ? 1 : (mm[a] < 0.0f ? -1 : 0);
+        W.u[a] = mm[a] / len;
+        W.rinv[a] = len / mm[a];
+        W.v[a] = v0[a];
+        W.q[a] = p[a];
+        W.n[a] = (int)V.dim[a];
     }
-    const uint32_t oct = (st[0] < 0 ? 1u : 0u) | (st[1] < 0 ? 2u : 0u) | (st[2] < 0 ? 4u : 0u);
-    int v[3] = {v0[0], v0[1], v0[2]};
-    float q[3] = {p[0], p[1], p[2]};
-    const int n[3] = {(int)V.dim[0], (int)V.dim[1], (int)V.dim[2]};
-    const uint32_t bdx = V.bdim[0], bdy = V.bdim[1];
-    uint32_t steps = 0, mask = 0, entry = VT_EMPTY, li = 0, cur = VT_EMPTY;
-    uint64_t blk = 0;
-    bool hit = false;
-    const uint32_t guard = 2u * (V.dim[0] + V.dim[1] + V.dim[2]) + 64u;
-    // One iteration is one step for every lane, whatever its box: both candidate boxes of an
-    // empty brick are evaluated, a non-empty brick's box fills both candidates, and the exit
-    // of each axis is a select, so lanes in empty and in occupied bricks run the same code.
-    for (uint32_t it = 0; it < guard; ++it) {
-        if (!((uint32_t)v[0] < (uint32_t)n[0] && (uint32_t)v[1] < (uint32_t)n[1] && (uint32_t)v[2] < (uint32_t)n[2])) break;
-        const int b8[3] = {v[0] & ~7, v[1] & ~7, v[2] & ~7};
-        const uint32_t bi = (uint32_t)(b8[0] >> 3) + bdx * ((uint32_t)(b8[1] >> 3) + bdy * (uint32_t)(b8[2] >> 3));
-        if (bi != cur) {
-            cur = bi;
-            entry = gld(V.brick_map + bi);
-            ctr.brick(!vt_brick_empty(entry));
-            if (!vt_brick_empty(entry)) {
-                blk = gld64(V.blocks + entry);   // issued with the mask loads: one round trip after the entry
-                stage_mask(V.masks, entry, mslot);
-            }
+    W.oct = (W.st[0] < 0 ? 1u : 0u) | (W.st[1] < 0 ? 2u : 0u) | (W.st[2] < 0 ? 4u : 0u);
+    W.brick_map = V.brick_map;
+    W.masks = V.masks;
+    W.blocks = V.blocks;
+    W.bdx = V.bdim[0];
+    W.bdy = V.bdim[1];
+    W.guard = 2u * (V.dim[0] + V.dim[1] + V.dim[2]) + 64u;
+    W.steps = 0;
+    W.mask = 0;
+    W.entry = VT_EMPTY;
+    W.cur = VT_EMPTY;
+    W.li = 0;
+    W.blk = 0;
+    W.hit = false;
+}
+
+template <bool ST>
+__device__ __forceinline__ bool box_walk_step(BoxWalk& W, Ctr<ST>& ctr, const Lds& mslot) {
+    const int* v = W.v;
+    if (!((W.steps < W.guard) & ((uint32_t)v[0] < (uint32_t)W.n[0]) & ((uint32_t)v[1] < (uint32_t)W.n[1]) &
+          ((uint32_t)v[2] < (uint32_t)W.n[2])))
+        return false;
+    const int b8[3] = {v[0] & ~7, v[1] & ~7, v[2] & ~7};
+    const uint32_t bi = (uint32_t)(b8[0] >> 3) + W.bdx * ((uint32_t)(b8[1] >> 3) + W.bdy * (uint32_t)(b8[2] >> 3));
+    if (bi != W.cur) {
+        W.cur = bi;
+        W.entry = gld(W.brick_map + bi);
+        ctr.brick(!vt_brick_empty(W.entry));
+        if (!vt_brick_empty(W.entry)) {
+            W.blk = gld64(W.blocks + W.entry);
+            stage_mask(W.masks, W.entry, mslot);
         }
-        // (every value below is computed unconditionally and chosen with selects: arms written
-        // inside ?: become branches, which lanes of different kinds would take one after the other)
-        const bool emp = vt_brick_empty(entry);
-        const int lx = v[0] - b8[0], ly = v[1] - b8[1], lz = v[2] - b8[2];
-        const bool oct_empty =
-            ((blk >> (uint32_t)(2 * (lx >> 2) + 8 * (ly >> 2) + 32 * (lz >> 2))) & 0x330033ull) == 0ull;
-        const bool blk_empty = ((blk >> (uint32_t)((lx >> 1) + 4 * (ly >> 1) + 16 * (lz >> 1))) & 1ull) == 0ull;
-        const int sz = oct_empty ? 4 : (blk_empty ? 2 : 1);
-        const uint32_t bit = (uint32_t)lx + 8u * (uint32_t)ly;
-        const uint64_t slice = mask_slice(mslot, lz);
-        if (!emp & (sz == 1) & (((uint32_t)(slice >> bit) & 1u) != 0u)) {
-            li = bit + 64u * (uint32_t)lz;
-            hit = true;
-            break;
-        }
-        // the box: in an empty brick the anchored cube when K_o - 1 > r (it then reaches further
-        // on every axis), else the centred cube; in a non-empty brick the octant, block or voxel
-        const int rad = 8 * (int)vt_brick_radius(entry);
-        const int k8 = 8 * (int)vt_brick_cube(entry, oct);
-        const bool anc = k8 - 8 > rad;
-        int lo[3], hi[3];
-        float tt[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const int bl = v[a] & -sz;
-            const int al = st[a] < 0 ? b8[a] + 8 - k8 : b8[a], ah = st[a] < 0 ? b8[a] + 8 : b8[a] + k8;
-            const int cl = anc ? al : b8[a] - rad, ch = anc ? ah : b8[a] + 8 + rad;
-            lo[a] = emp ? cl : bl;
-            hi[a] = emp ? ch : bl + sz;
-            const int e = st[a] > 0 ? hi[a] : lo[a];
-            const float t = ((float)e - q[a]) * rinv[a];
-            tt[a] = st[a] != 0 ? t : INFINITY;
-        }
-        const float tmin = gmin(gmin(gmin(INFINITY, tt[0]), tt[1]), tt[2]);
-        uint32_t mk = 0;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const int l = lo[a], h = hi[a];
-            const bool ex = (st[a] != 0) & (tt[a] <= tmin);
-            const float x = q[a] + tmin * u[a];
-            const int f0 = (int)floorf(x);
-            const int f1 = f0 < l ? l : f0;
-            const int f = f1 > h - 1 ? h - 1 : f1;
-            const int pe = st[a] > 0 ? h : l;
-            const float qe = (float)pe;
-            const int ve = st[a] > 0 ? h : l - 1;
-            q[a] = ex ? qe : x;
-            v[a] = ex ? ve : f;
-            mk |= ex ? 1u << a : 0u;
-        }
-        mask = mk;
-        ++steps;
     }
-    ctr.steps(steps);
-    r.v[0] = v[0]; r.v[1] = v[1]; r.v[2] = v[2];
-    r.steps = steps;
-    r.mask = hit && steps > 0 ? mask : 0u;
-    r.t_last = 0.0f;
-    r.q[0] = q[0]; r.q[1] = q[1]; r.q[2] = q[2];
-    r.entry = entry;
-    r.li = li;
-    r.hit = hit;
+    const uint32_t entry = W.entry;
+    const uint64_t blk = W.blk;
+    const bool emp = vt_brick_empty(entry);
+    const int lx = v[0] - b8[0], ly = v[1] - b8[1], lz = v[2] - b8[2];
+    const bool oct_empty = ((blk >> (uint32_t)(2 * (lx >> 2) + 8 * (ly >> 2) + 32 * (lz >> 2))) & 0x330033ull) == 0ull;
+    const bool blk_empty = ((blk >> (uint32_t)((lx >> 1) + 4 * (ly >> 1) + 16 * (lz >> 1))) & 1ull) == 0ull;
+    const int sz = oct_empty ? 4 : (blk_empty ? 2 : 1);
+    const uint32_t bit = (uint32_t)lx + 8u * (uint32_t)ly;
+    const uint64_t slice = mask_slice(mslot, lz);
+    if (!emp & (sz == 1) & (((uint32_t)(slice >> bit) & 1u) != 0u)) {
+        W.li = bit + 64u * (uint32_t)lz;
+        W.hit = true;
+        return false;
+    }
+    const int rad = 8 * (int)vt_brick_radius(entry);
+    const int k8 = 8 * (int)vt_brick_cube(entry, W.oct);
+    const bool anc = k8 - 8 > rad;
+    int lo[3], hi[3];
+    float tt[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int st = W.st[a];
+        const int bl = v[a] & -sz;
+        const int al = st < 0 ? b8[a] + 8 - k8 : b8[a], ah = st < 0 ? b8[a] + 8 : b8[a] + k8;
+        const int cl = anc ? al : b8[a] - rad, ch = anc ? ah : b8[a] + 8 + rad;
+        lo[a] = emp ? cl : bl;
+        hi[a] = emp ? ch : bl + sz;
+        const int e = st > 0 ? hi[a] : lo[a];
+        const float t = ((float)e - W.q[a]) * W.rinv[a];
+        tt[a] = st != 0 ? t : INFINITY;
+    }
+    const float tmin = gmin(gmin(gmin(INFINITY, tt[0]), tt[1]), tt[2]);
+    uint32_t mk = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int st = W.st[a];
+        const int l = lo[a], h = hi[a];
+        const bool ex = (st != 0) & (tt[a] <= tmin);
+        const float x = W.q[a] + tmin * W.u[a];
+        const int f0 = (int)floorf(x);
+        const int f1 = f0 < l ? l : f0;
+        const int f = f1 > h - 1 ? h - 1 : f1;
+        const int pe = st > 0 ? h : l;
+        const float qe = (float)pe;
+        const int ve = st > 0 ? h : l - 1;
+        W.q[a] = ex ? qe : x;
+        W.v[a] = ex ? ve : f;
+        mk |= ex ? 1u << a : 0u;
+    }
+    W.mask = mk;
+    ++W.steps;
+    return true;
 }
 
 struct HitG {
@@ -716,7 +733,7 @@ __device__ __forceinline__ TreeLds stage_tree(const ReqGPU* reqs, const BatchGPU
 }
 
 // Primary ray: closest hit over all instances through the LBVH (nodes from `tree` in LDS when
-// given).  Secondary rays have their own traversal (secondary()).
+// given).  Secondary rays have their own traversal (in k_secondary).
 template <bool ST, bool DEEP>
 __device__ HitG query(const SceneGPU& S, const FrameGPU& F, V3 O, V3 D, Ctr<ST>& ctr, const Lds& stk_lds,
                       const Lds& mslot, TreeLds tree) {
@@ -844,121 +861,6 @@ __device__ void surface_of(const SceneGPU& S, const HitG& h, SurfaceRec& sf) {
     sf.n[0] = n.x; sf.n[1] = n.y; sf.n[2] = n.z;
     sf.inst = h.inst;
     sf.face = h.face;
-}
-
-// Secondary query from a surface: the origin instance from the surface point first, then (if
-// that walk misses) the other instances the ray enters, in the tree's traversal order with the
-// closest-hit pruning of query().  Every instance walk runs through one call of the walk (one
-// walk per loop iteration), so lanes walking the origin and lanes walking another instance
-// share its instructions instead of running two copies of the walk one after the other.
-template <bool ST, bool DEEP>
-__device__ HitG secondary(const SceneGPU& S, const FrameGPU& F, const QueueRec& sf, V3 O, V3 w, bool any_hit,
-                          Ctr<ST>& ctr, const Lds& stk_lds, const Lds& mslot, TreeLds tree) {
-    ctr.ray();
-    HitG best;
-    best.hit = false;
-    best.inst = VT_EMPTY;
-    best.tn = INFINITY;
-    const uint32_t origin = sf.inst;
-    const V3 inv_d{1.0f / w.x, 1.0f / w.y, 1.0f / w.z};
-    TravStack<DEEP> stk;
-    stk.lds = stk_lds;
-    stk.sp = 0;
-    // the instance to walk next and its entry (the origin: the surface point and start voxel)
-    uint32_t inst = origin;
-    float mpos[3] = {sf.p[0], sf.p[1], sf.p[2]};
-    const int fa0 = (int)(sf.face >> 1), off0 = (sf.face & 1u) == 0u ? -1 : 1;
-    int v0[3] = {(int)sf.v[0] + (fa0 == 0 ? off0 : 0), (int)sf.v[1] + (fa0 == 1 ? off0 : 0),
-                 (int)sf.v[2] + (fa0 == 2 ? off0 : 0)};
-    float tn = 0.0f;
-    uint8_t face0 = (uint8_t)sf.face;   // face when the walk hits its start voxel
-    V3 mdir;
-    {
-        const InstanceGPU* I = &S.inst[origin];
-        float inv[12];
-#pragma unroll
-        for (int k = 0; k < 12; ++k) inv[k] = gld(&I->inv[k]);
-        mdir = xf34(inv, w, 0.0f);
-    }
-    bool walk = true;
-    for (;;) {
-        if (walk) {
-            ctr.instance();
-            const VolumeDesc V = S.vols[S.inst[inst].tex];
-            Dda r;
-            dda2_run<ST>(V, mpos, v0, mdir, r, ctr, mslot);
-            if (r.hit) {
-                HitG h;
-                h.hit = true;
-                h.inst = inst;
-                h.tn = tn;
-                h.d = r;
-                h.mpos[0] = mpos[0]; h.mpos[1] = mpos[1]; h.mpos[2] = mpos[2];
-                h.mdir = mdir;
-                if (r.steps > 0) {
-                    const int a = lowest_axis(r.mask);
-                    h.face = face_from(a, comp(mdir, a));
-                } else {
-                    h.face = face0;
-                }
-                if (inst == origin) {
-                    ctr.hit();
-                    return h;
-                }
-                if (better(h, best)) {
-                    best = h;
-                    if (any_hit) break;
-                }
-            }
-            if (inst == origin && S.ninst > 0) stk.push(0u);   // the scene query follows the origin walk
-            walk = false;
-        }
-        // the next instance the ray enters (query() with enter_instance for secondary rays)
-        while (stk.sp > 0) {
-            const uint32_t ni = stk.pop();
-            const BvhNode N = tree.on ? lds_node(tree, ni) : gld(S.nodes + ni);
-            ctr.node();
-            float a0, a1;
-            if (!slab_node(O, w, inv_d, N, &a0, &a1) || a1 < 0.0f) continue;
-            if (best.hit && a0 > prune_bound(best.tn)) continue;
-            if (N.right != VT_EMPTY) {
-                stk.push(N.right);
-                stk.push(N.left);
-                continue;
-            }
-            const uint32_t i = N.left;
-            if (i == origin) continue;
-            const InstanceGPU* I = &S.inst[i];
-            if (!gld(&I->valid)) continue;
-            float inv[12];
-#pragma unroll
-            for (int k = 0; k < 12; ++k) inv[k] = gld(&I->inv[k]);
-            const V3 o = xf34(inv, O, 1.0f);
-            const V3 d = xf34(inv, w, 0.0f);
-            const SlabR sl = slab_unit(o, d);
-            if (!sl.ok || !(sl.tn > 0.0f)) continue;
-            const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
-            const float size[3] = {I->size[0], I->size[1], I->size[2]};
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const float face = dd[a] > 0.0f ? -0.5f : 0.5f;
-                const float x = oo[a] + sl.tn * dd[a];
-                const float xc = x < -0.5f ? -0.5f : (x > 0.5f ? 0.5f : x);
-                const float mp = a == sl.axis ? face : xc;
-                mpos[a] = (mp + 0.5f) * size[a];
-                v0[a] = (int)floorf(gmin(mpos[a], size[a] - 1.0f));
-            }
-            mdir = d;
-            tn = sl.tn;
-            face0 = face_from(sl.axis, dd[sl.axis]);
-            inst = i;
-            walk = true;
-            break;
-        }
-        if (!walk) break;
-    }
-    if (best.hit) ctr.hit();
-    return best;
 }
 
 __device__ void cosine_dir(Rng& rng, V3 n, V3& w) {
@@ -1249,6 +1151,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     if (ST) ctr.flush(stats);
 }
 
+// Lanes of a wave that are still walking when fewer than kPoolLow of its 64 lanes walk (and
+// the launch has items left) pause; the others finish their rays and take new items.  (Swept
+// over 1..56 on the 16 x 256^3 1080p frame: 8-12 is fastest; larger values refill more often
+// and spend the time saved in the walk on more rounds of ray set-up and instance search.)
+constexpr uint32_t kPoolLow = 8;
+
 // The secondary rays of one depth in one launch.  Part p covers segment p of this depth's queue
 // (count slots) and of its sun list (ns entries): items [0, nb) are the bounce rays of its slots
 // 0..nb-1 (nb = count when paths continue, else 0), items [nb, nb + ns) the sun rays of the
@@ -1257,6 +1165,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
 // rays: Ls and the shadow flag), so they run side by side and fill each other's tail.  in_idx / out_idx:
 // counter slots of this depth's queue and the next one.  ONE: the batch holds one request (its
 // scene and frame are then wave-uniform, read with scalar loads).
+//
+// Lanes are a pool refilled as their rays finish: each lane holds one item through rounds of
+// (take items for the idle lanes; set up their rays; step every walking lane until fewer than
+// kPoolLow walk; for the lanes whose walk ended: keep the hit, search the tree for the next
+// instance, or finish the ray: sky or sun term, or shade and queue the hit).  A refill takes
+// consecutive items of one part for its lanes with one atomic (item counters fetch + p *
+// kPartStride).  The walk loop so keeps most lanes busy instead of running to the longest walk
+// of a fixed 64-item tile (walk-loop lane use 0.18 -> 0.57 on the 1080p frame).
+// Each ray's query: the origin instance from the surface point first, then (if that walk
+// misses) the other instances the ray enters, in the tree's traversal order with the
+// closest-hit pruning of query(); a sun ray stops at its first hit.
 template <bool ST, bool DEEP, bool ONE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_secondary(
     const ReqGPU* __restrict__ reqs, const BatchGPU* __restrict__ B, const float* __restrict__ srgb_dec, WF wf,
@@ -1264,7 +1183,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     unsigned long long* stats) {
     __shared__ uint32_t smem[(kStackLds + 16) * kBlock];
     __shared__ BvhNode tree_lds[kLdsNodes];
-    const Lds stk{&smem[threadIdx.x]}, mslot{&smem[kStackLds * kBlock + threadIdx.x]};
+    const Lds stk_lds{&smem[threadIdx.x]}, mslot{&smem[kStackLds * kBlock + threadIdx.x]};
     __shared__ uint32_t gone;
     if (threadIdx.x == 0) gone = 0u;
     const TreeLds tree = stage_tree(reqs, B, tree_lds);
@@ -1272,77 +1191,248 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     Ctr<ST> ctr;
     uint32_t* const qin = slot_counters(wf.counters, in_idx);
     uint32_t* const qsun = slot_counters(wf.counters, sun_slot(in_idx));
+    uint32_t* const fetch = slot_counters(wf.counters, fetch_idx);
     const uint32_t cap = wf.seg_cap;
     const uint32_t nq = 1u - q;
     const uint32_t nreq = B->nreq;
-    for_each_item(slot_counters(wf.counters, fetch_idx), &gone,
-                  [&](uint32_t p, uint32_t* first, uint32_t* n) {
-                      *first = 0u;
-                      *n = (bounce ? queue_count(qin, p, cap) : 0u) + queue_count(qsun, p, cap);
-                  },
-                  [&](uint32_t p, uint32_t item) {
-        const uint32_t nb = bounce ? queue_count(qin, p, cap) : 0u;
-        const bool is_bounce = item < nb;
-        const uint32_t slot = is_bounce ? p * cap + item : wf.sun[q][p * cap + (item - nb)];
-        const QueueRec sf = get_rec(&wf.rec[q][slot]);
-        const uint32_t path = sf.path;
-        const ReqGPU& Rq = reqs[ONE ? 0u : find_req(B->work_base, nreq, path)];
-        const SceneGPU S = Rq.S;
-        const FrameGPU& F = Rq.F;
-        V3 X, n;
-        surface_frame(&S.inst[sf.inst], sf.p, sf.face, X, n);
-        // bounce rays: throughput (depth 0: 1 x albedo, the primary surface's; later: the path
-        // state) and the direction drawn from the record's RNG state
-        float beta[3] = {0.0f, 0.0f, 0.0f};
-        Rng rng{sf.rng};
-        V3 w{F.sun[0], F.sun[1], F.sun[2]};
-        if (is_bounce) {
-            if (depth == 0) {
-                beta[0] = 1.0f * srgb_dec[sf.rgba & 255u];
-                beta[1] = 1.0f * srgb_dec[(sf.rgba >> 8) & 255u];
-                beta[2] = 1.0f * srgb_dec[(sf.rgba >> 16) & 255u];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64u - lane));
+    const uint32_t p0 = blockIdx.x % kParts;
+    uint32_t kp = 0;   // parts this wave has found exhausted, in the order p0, p0 + 1, ... (uniform)
+
+    // per-lane item and ray state
+    bool busy = false, walking = false, is_bounce = false;
+    uint32_t path = 0, origin = 0, ri = 0, rng = 0, inst = 0;
+    uint8_t face0 = 0;
+    float val[3] = {0.0f, 0.0f, 0.0f};   // bounce ray: throughput; sun ray: its light if clear
+    float tn = 0.0f;
+    V3 O{0.0f, 0.0f, 0.0f}, w{0.0f, 0.0f, 0.0f}, inv_d{0.0f, 0.0f, 0.0f}, mdir{0.0f, 0.0f, 0.0f};
+    HitG best;
+    best.hit = false;
+    best.inst = VT_EMPTY;
+    best.tn = INFINITY;
+    TravStack<DEEP> stk;
+    stk.lds = stk_lds;
+    stk.sp = 0;
+    BoxWalk W;
+    W.steps = 0;
+
+    for (;;) {
+        // -- take items for the idle lanes (consecutive items of one part in lane order) --
+        const uint64_t idle = __ballot(!busy);
+        bool fresh = false;
+        uint32_t item = 0, ipart = 0;
+        if (idle != 0ull && kp < kParts) {
+            const uint32_t need = (uint32_t)__popcll(idle);
+            const uint32_t rank = (uint32_t)__popcll(idle & below);
+            uint32_t placed = 0;
+            while (placed < need && kp < kParts) {
+                const uint32_t p = (p0 + kp) % kParts;
+                const bool g = (__hip_atomic_load(&gone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & (1u << p)) != 0u;
+                uint32_t take = 0, base = 0;
+                if (!g) {
+                    const uint32_t count = (bounce ? queue_count(qin, p, cap) : 0u) + queue_count(qsun, p, cap);
+                    if (lane == 0 && count > 0) base = atomicAdd(fetch + p * kPartStride, need - placed);
+                    base = __shfl(base, 0, 64);
+                    take = base < count ? min(count - base, need - placed) : 0u;
+                }
+                if (!busy && rank >= placed && rank < placed + take) {
+                    fresh = true;
+                    item = base + (rank - placed);
+                    ipart = p;
+                }
+                placed += take;
+                if (placed < need) {   // part p has no items left
+                    if (lane == 0 && !g) __hip_atomic_fetch_or(&gone, 1u << p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    ++kp;
+                }
+            }
+        }
+        // -- set up the new items' rays: the origin walk first --
+        if (fresh) {
+            const uint32_t nb = bounce ? queue_count(qin, ipart, cap) : 0u;
+            is_bounce = item < nb;
+            const uint32_t slot = is_bounce ? ipart * cap + item : wf.sun[q][ipart * cap + (item - nb)];
+            const QueueRec sf = get_rec(&wf.rec[q][slot]);
+            path = sf.path;
+            origin = sf.inst;
+            ri = ONE ? 0u : find_req(B->work_base, nreq, path);
+            const ReqGPU& Rq = reqs[ri];
+            const SceneGPU& S = Rq.S;
+            const FrameGPU& F = Rq.F;
+            V3 X, n;
+            surface_frame(&S.inst[sf.inst], sf.p, sf.face, X, n);
+            Rng r{sf.rng};
+            w = V3{F.sun[0], F.sun[1], F.sun[2]};
+            if (is_bounce) {
+                if (depth == 0) {
+                    val[0] = 1.0f * srgb_dec[sf.rgba & 255u];
+                    val[1] = 1.0f * srgb_dec[(sf.rgba >> 8) & 255u];
+                    val[2] = 1.0f * srgb_dec[(sf.rgba >> 16) & 255u];
+                } else {
+                    const float4 b4 = wf.br[path];
+                    val[0] = b4.x; val[1] = b4.y; val[2] = b4.z;
+                }
+                cosine_dir(r, n, w);
             } else {
-                const float4 b4 = wf.br[path];
-                beta[0] = b4.x; beta[1] = b4.y; beta[2] = b4.z;
+                val[0] = sf.c[0]; val[1] = sf.c[1]; val[2] = sf.c[2];
             }
-            cosine_dir(rng, n, w);
+            rng = r.s;
+            O = X;
+            inv_d = V3{1.0f / w.x, 1.0f / w.y, 1.0f / w.z};
+            ctr.ray();
+            best.hit = false;
+            best.inst = VT_EMPTY;
+            best.tn = INFINITY;
+            stk.sp = 0;
+            inst = origin;
+            tn = 0.0f;
+            face0 = (uint8_t)sf.face;
+            const int fa0 = (int)(sf.face >> 1), off0 = (sf.face & 1u) == 0u ? -1 : 1;
+            const int v0[3] = {(int)sf.v[0] + (fa0 == 0 ? off0 : 0), (int)sf.v[1] + (fa0 == 1 ? off0 : 0),
+                               (int)sf.v[2] + (fa0 == 2 ? off0 : 0)};
+            const InstanceGPU* I = &S.inst[origin];
+            float inv[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) inv[k] = gld(&I->inv[k]);
+            mdir = xf34(inv, w, 0.0f);
+            ctr.instance();
+            box_walk_begin(S.vols[I->tex], sf.p, v0, mdir, W);
+            busy = true;
+            walking = true;
         }
-        HitG h = secondary<ST, DEEP>(S, F, sf, X, w, !is_bounce, ctr, stk, mslot, tree);
-        if (!is_bounce) {
-            if (!h.hit) {
-                float4 L = wf.Ls[path];
-                L.x = L.x + sf.c[0];
-                L.y = L.y + sf.c[1];
-                L.z = L.z + sf.c[2];
-                wf.Ls[path] = L;
-            } else if (depth == 0) {
-                float4 L = wf.Ls[path];
-                L.w = __uint_as_float(1u);
-                wf.Ls[path] = L;
+        if (__ballot(busy) == 0ull) break;   // no items left and none in flight
+        // -- the walk loop: every walking lane steps until too few walk --
+        for (;;) {
+            if (walking) walking = box_walk_step<ST>(W, ctr, mslot);
+            const uint32_t nw = (uint32_t)__popcll(__ballot(walking));
+            if (nw == 0u || (nw < kPoolLow && kp < kParts)) break;
+        }
+        // -- lanes whose walk ended: keep its hit, then the next instance or the ray's end --
+        if (busy && !walking) {
+            const ReqGPU& Rq = reqs[ri];
+            const SceneGPU& S = Rq.S;
+            const FrameGPU& F = Rq.F;
+            ctr.steps(W.steps);
+            bool done = false;
+            if (W.hit) {
+                HitG h;
+                h.hit = true;
+                h.inst = inst;
+                h.tn = tn;
+                h.d.v[0] = W.v[0]; h.d.v[1] = W.v[1]; h.d.v[2] = W.v[2];
+                h.d.steps = W.steps;
+                h.d.mask = W.steps > 0 ? W.mask : 0u;
+                h.d.t_last = 0.0f;
+                h.d.q[0] = W.q[0]; h.d.q[1] = W.q[1]; h.d.q[2] = W.q[2];
+                h.d.entry = W.entry;
+                h.d.li = W.li;
+                h.d.hit = true;
+                h.mpos[0] = 0.0f; h.mpos[1] = 0.0f; h.mpos[2] = 0.0f;
+                h.mdir = mdir;
+                if (W.steps > 0) {
+                    const int a = lowest_axis(h.d.mask);
+                    h.face = face_from(a, comp(mdir, a));
+                } else {
+                    h.face = face0;
+                }
+                if (inst == origin) {
+                    best = h;
+                    done = true;
+                } else if (better(h, best)) {
+                    best = h;
+                    done = !is_bounce;   // a sun ray stops at any hit
+                }
             }
-            return;
+            if (!done) {
+                if (inst == origin && S.ninst > 0) stk.push(0u);   // the scene query follows the origin walk
+                bool found = false;
+                while (stk.sp > 0) {
+                    const uint32_t ni = stk.pop();
+                    const BvhNode N = tree.on ? lds_node(tree, ni) : gld(S.nodes + ni);
+                    ctr.node();
+                    float a0, a1;
+                    if (!slab_node(O, w, inv_d, N, &a0, &a1) || a1 < 0.0f) continue;
+                    if (best.hit && a0 > prune_bound(best.tn)) continue;
+                    if (N.right != VT_EMPTY) {
+                        stk.push(N.right);
+                        stk.push(N.left);
+                        continue;
+                    }
+                    const uint32_t i = N.left;
+                    if (i == origin) continue;
+                    const InstanceGPU* I = &S.inst[i];
+                    if (!gld(&I->valid)) continue;
+                    float inv[12];
+#pragma unroll
+                    for (int k = 0; k < 12; ++k) inv[k] = gld(&I->inv[k]);
+                    const V3 o = xf34(inv, O, 1.0f);
+                    const V3 d = xf34(inv, w, 0.0f);
+                    const SlabR sl = slab_unit(o, d);
+                    if (!sl.ok || !(sl.tn > 0.0f)) continue;
+                    const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+                    const float size[3] = {I->size[0], I->size[1], I->size[2]};
+                    float mpos[3];
+                    int v0[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const float face = dd[a] > 0.0f ? -0.5f : 0.5f;
+                        const float x = oo[a] + sl.tn * dd[a];
+                        const float xc = x < -0.5f ? -0.5f : (x > 0.5f ? 0.5f : x);
+                        const float mp = a == sl.axis ? face : xc;
+                        mpos[a] = (mp + 0.5f) * size[a];
+                        v0[a] = (int)floorf(gmin(mpos[a], size[a] - 1.0f));
+                    }
+                    mdir = d;
+                    tn = sl.tn;
+                    face0 = face_from(sl.axis, dd[sl.axis]);
+                    inst = i;
+                    ctr.instance();
+                    box_walk_begin(S.vols[I->tex], mpos, v0, mdir, W);
+                    found = true;
+                    break;
+                }
+                walking = found;
+                done = !found;
+            }
+            if (done) {
+                busy = false;
+                if (best.hit) ctr.hit();
+                if (!is_bounce) {
+                    if (!best.hit) {
+                        float4 L = wf.Ls[path];
+                        L.x = L.x + val[0];
+                        L.y = L.y + val[1];
+                        L.z = L.z + val[2];
+                        wf.Ls[path] = L;
+                    } else if (depth == 0) {
+                        float4 L = wf.Ls[path];
+                        L.w = __uint_as_float(1u);
+                        wf.Ls[path] = L;
+                    }
+                } else if (!best.hit) {
+                    float4 L = wf.L[path];
+                    L.x = L.x + val[0] * F.sky[0];
+                    L.y = L.y + val[1] * F.sky[1];
+                    L.z = L.z + val[2] * F.sky[2];
+                    wf.L[path] = L;
+                } else {
+                    SurfaceRec ns;
+                    uint8_t pal;
+                    ns.rgba = hit_rgba(S, best, &pal);
+                    surface_of(S, best, ns);
+                    float4 contrib;
+                    bool no_sun;
+                    const uint32_t max_b = F.mode == 2 ? F.max_bounces : 0u;
+                    shade(F, srgb_dec, ns, depth + 1u, max_b, val, contrib, no_sun);
+                    const uint32_t ns_slot = append_slot(slot_counters(wf.counters, out_idx), cap);
+                    put_rec(&wf.rec[nq][ns_slot], ns, contrib, path, rng);
+                    if (!no_sun) wf.sun[nq][append_slot(slot_counters(wf.counters, sun_slot(out_idx)), cap)] = ns_slot;
+                    if (depth + 1u < max_b) wf.br[path] = make_float4(val[0], val[1], val[2], 0.0f);
+                }
+            }
         }
-        if (!h.hit) {
-            float4 L = wf.L[path];
-            L.x = L.x + beta[0] * F.sky[0];
-            L.y = L.y + beta[1] * F.sky[1];
-            L.z = L.z + beta[2] * F.sky[2];
-            wf.L[path] = L;
-            return;
-        }
-        SurfaceRec ns;
-        uint8_t pal;
-        ns.rgba = hit_rgba(S, h, &pal);
-        surface_of(S, h, ns);
-        float4 contrib;
-        bool no_sun;
-        const uint32_t max_b = F.mode == 2 ? F.max_bounces : 0u;
-        shade(F, srgb_dec, ns, depth + 1u, max_b, beta, contrib, no_sun);
-        const uint32_t ns_slot = append_slot(slot_counters(wf.counters, out_idx), cap);
-        put_rec(&wf.rec[nq][ns_slot], ns, contrib, path, rng.s);
-        if (!no_sun) wf.sun[nq][append_slot(slot_counters(wf.counters, sun_slot(out_idx)), cap)] = ns_slot;
-        if (depth + 1u < max_b) wf.br[path] = make_float4(beta[0], beta[1], beta[2], 0.0f);
-    });
+    }
     if (ST) ctr.flush(stats);
 }
 
