@@ -955,16 +955,17 @@ __device__ __forceinline__ uint32_t queue_count(uint32_t* q, uint32_t p, uint32_
 // Wave-aggregated append to queue counters q (segments of `cap` slots): the wave's appending
 // lanes take consecutive slots of segment blockIdx.x % kParts in lane order with one atomic;
 // what does not fit there goes on to the next segments (a counter past cap marks its segment
-// full).  The segments together hold every path, so the loop always places all lanes.
-__device__ __forceinline__ uint32_t append_slot(uint32_t* q, uint32_t cap) {
+// full).  The segments together hold every path, so the loop always places all lanes.  k0 > 0
+// starts at segment (blockIdx.x + k0) % kParts (the earlier ones known to be full).
+__device__ __forceinline__ uint32_t append_slot(uint32_t* q, uint32_t cap, uint32_t k0 = 0) {
     const uint64_t m = __ballot(1);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
     const uint32_t n = (uint32_t)__popcll(m);
     const uint64_t lower = lane == 0 ? 0ull : (m & (~0ull >> (64u - lane)));
     const uint32_t rank = (uint32_t)__popcll(lower);
-    uint32_t slot = 0, placed = 0, p = blockIdx.x % kParts;
-    for (uint32_t k = 0; k < kParts && placed < n; ++k, p = (p + 1u) % kParts) {
+    uint32_t slot = 0, placed = 0, p = (blockIdx.x + k0) % kParts;
+    for (uint32_t k = k0; k < kParts && placed < n; ++k, p = (p + 1u) % kParts) {
         uint32_t base = 0;
         if (lane == leader) base = atomicAdd(q + p * kPartStride, n - placed);
         base = __shfl(base, (int)leader, 64);
@@ -973,6 +974,33 @@ __device__ __forceinline__ uint32_t append_slot(uint32_t* q, uint32_t cap) {
         placed += take;
     }
     return slot;
+}
+
+// Two appends in one round trip: every active lane to queue qa (slot *sa), the lanes with b also
+// to queue qb (slot *sb); the leader issues both atomics before either result is used.  Lanes
+// that do not fit segment blockIdx.x % kParts go on through append_slot from the next segment.
+__device__ __forceinline__ void append_pair(uint32_t* qa, uint32_t* qb, bool b, uint32_t cap, uint32_t* sa,
+                                            uint32_t* sb) {
+    const uint64_t ma = __ballot(1), mb = __ballot(b);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)ma) - 1u;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64u - lane));
+    const uint32_t na = (uint32_t)__popcll(ma), nb = (uint32_t)__popcll(mb);
+    const uint32_t ra = (uint32_t)__popcll(ma & below), rb = (uint32_t)__popcll(mb & below);
+    const uint32_t p = blockIdx.x % kParts;
+    uint32_t ba = 0, bb = 0;
+    if (lane == leader) {
+        ba = atomicAdd(qa + p * kPartStride, na);
+        if (nb > 0u) bb = atomicAdd(qb + p * kPartStride, nb);
+    }
+    ba = __shfl(ba, (int)leader, 64);
+    bb = __shfl(bb, (int)leader, 64);
+    const uint32_t ta = ba < cap ? min(cap - ba, na) : 0u;
+    const uint32_t tb = bb < cap ? min(cap - bb, nb) : 0u;
+    *sa = p * cap + ba + ra;
+    *sb = p * cap + bb + rb;
+    if (ra >= ta) *sa = append_slot(qa, cap, 1u);
+    if (b && rb >= tb) *sb = append_slot(qb, cap, 1u);
 }
 
 // Wavefront buffers (device pointers), see WaveBuffers in vt_internal.h.
@@ -1138,9 +1166,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
             float4 contrib;
             bool no_sun;
             shade(F, srgb_dec, sf, 0u, max_b, beta, contrib, no_sun);
-            const uint32_t slot = append_slot(slot_counters(wf.counters, 0), wf.seg_cap);
+            uint32_t slot, ls;
+            append_pair(slot_counters(wf.counters, 0), slot_counters(wf.counters, sun_slot(0)), !no_sun, wf.seg_cap, &slot,
+                        &ls);
             put_rec(&wf.rec[0][slot], sf, contrib, w, rng.s);
-            if (!no_sun) wf.sun[0][append_slot(slot_counters(wf.counters, sun_slot(0)), wf.seg_cap)] = slot;
+            if (!no_sun) wf.sun[0][ls] = slot;
             // a hit path's state: sun terms (w: shadowed if facing away; its sun ray may set it);
             // the first bounce's throughput is the albedo and its RNG state is in the record
             L.w = __uint_as_float(1u);
@@ -1309,10 +1339,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
             if (nw == 0u || (nw < kPoolLow && kp < kParts)) break;
         }
         // -- lanes whose walk ended: keep its hit, then the next instance or the ray's end --
+        bool fin = false;
         if (busy && !walking) {
-            const ReqGPU& Rq = reqs[ri];
-            const SceneGPU& S = Rq.S;
-            const FrameGPU& F = Rq.F;
+            const SceneGPU& S = reqs[ri].S;
             ctr.steps(W.steps);
             bool done = false;
             if (W.hit) {
@@ -1346,22 +1375,30 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
             }
             if (!done) {
                 if (inst == origin && S.ninst > 0) stk.push(0u);   // the scene query follows the origin walk
+                // The search alternates two loops so that lanes at inner nodes and lanes at
+                // leaves do not pay for each other's work in one iteration: node tests until each
+                // lane holds its next leaf, then the leaf's instance entry test for all of them.
                 bool found = false;
-                while (stk.sp > 0) {
-                    const uint32_t ni = stk.pop();
-                    const BvhNode N = tree.on ? lds_node(tree, ni) : gld(S.nodes + ni);
-                    ctr.node();
-                    float a0, a1;
-                    if (!slab_node(O, w, inv_d, N, &a0, &a1) || a1 < 0.0f) continue;
-                    if (best.hit && a0 > prune_bound(best.tn)) continue;
-                    if (N.right != VT_EMPTY) {
-                        stk.push(N.right);
-                        stk.push(N.left);
-                        continue;
+                for (;;) {
+                    uint32_t cand = VT_EMPTY;
+                    while (stk.sp > 0) {
+                        const uint32_t ni = stk.pop();
+                        const BvhNode N = tree.on ? lds_node(tree, ni) : gld(S.nodes + ni);
+                        ctr.node();
+                        float a0, a1;
+                        if (!slab_node(O, w, inv_d, N, &a0, &a1) || a1 < 0.0f) continue;
+                        if (best.hit && a0 > prune_bound(best.tn)) continue;
+                        if (N.right != VT_EMPTY) {
+                            stk.push(N.right);
+                            stk.push(N.left);
+                            continue;
+                        }
+                        if (N.left == origin) continue;
+                        cand = N.left;
+                        break;
                     }
-                    const uint32_t i = N.left;
-                    if (i == origin) continue;
-                    const InstanceGPU* I = &S.inst[i];
+                    if (cand == VT_EMPTY) break;
+                    const InstanceGPU* I = &S.inst[cand];
                     if (!gld(&I->valid)) continue;
                     float inv[12];
 #pragma unroll
@@ -1386,7 +1423,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
                     mdir = d;
                     tn = sl.tn;
                     face0 = face_from(sl.axis, dd[sl.axis]);
-                    inst = i;
+                    inst = cand;
                     ctr.instance();
                     box_walk_begin(S.vols[I->tex], mpos, v0, mdir, W);
                     found = true;
@@ -1395,41 +1432,48 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
                 walking = found;
                 done = !found;
             }
-            if (done) {
-                busy = false;
-                if (best.hit) ctr.hit();
-                if (!is_bounce) {
-                    if (!best.hit) {
-                        float4 L = wf.Ls[path];
-                        L.x = L.x + val[0];
-                        L.y = L.y + val[1];
-                        L.z = L.z + val[2];
-                        wf.Ls[path] = L;
-                    } else if (depth == 0) {
-                        float4 L = wf.Ls[path];
-                        L.w = __uint_as_float(1u);
-                        wf.Ls[path] = L;
-                    }
-                } else if (!best.hit) {
-                    float4 L = wf.L[path];
-                    L.x = L.x + val[0] * F.sky[0];
-                    L.y = L.y + val[1] * F.sky[1];
-                    L.z = L.z + val[2] * F.sky[2];
-                    wf.L[path] = L;
-                } else {
-                    SurfaceRec ns;
-                    uint8_t pal;
-                    ns.rgba = hit_rgba(S, best, &pal);
-                    surface_of(S, best, ns);
-                    float4 contrib;
-                    bool no_sun;
-                    const uint32_t max_b = F.mode == 2 ? F.max_bounces : 0u;
-                    shade(F, srgb_dec, ns, depth + 1u, max_b, val, contrib, no_sun);
-                    const uint32_t ns_slot = append_slot(slot_counters(wf.counters, out_idx), cap);
-                    put_rec(&wf.rec[nq][ns_slot], ns, contrib, path, rng);
-                    if (!no_sun) wf.sun[nq][append_slot(slot_counters(wf.counters, sun_slot(out_idx)), cap)] = ns_slot;
-                    if (depth + 1u < max_b) wf.br[path] = make_float4(val[0], val[1], val[2], 0.0f);
+            fin = done;
+        }
+        // -- rays that ended: sky or sun term, or shade the hit and queue it for the next depth --
+        if (fin) {
+            const ReqGPU& Rq = reqs[ri];
+            const SceneGPU& S = Rq.S;
+            const FrameGPU& F = Rq.F;
+            busy = false;
+            if (best.hit) ctr.hit();
+            if (!is_bounce) {
+                if (!best.hit) {
+                    float4 L = wf.Ls[path];
+                    L.x = L.x + val[0];
+                    L.y = L.y + val[1];
+                    L.z = L.z + val[2];
+                    wf.Ls[path] = L;
+                } else if (depth == 0) {
+                    float4 L = wf.Ls[path];
+                    L.w = __uint_as_float(1u);
+                    wf.Ls[path] = L;
                 }
+            } else if (!best.hit) {
+                float4 L = wf.L[path];
+                L.x = L.x + val[0] * F.sky[0];
+                L.y = L.y + val[1] * F.sky[1];
+                L.z = L.z + val[2] * F.sky[2];
+                wf.L[path] = L;
+            } else {
+                SurfaceRec ns;
+                uint8_t pal;
+                ns.rgba = hit_rgba(S, best, &pal);
+                surface_of(S, best, ns);
+                float4 contrib;
+                bool no_sun;
+                const uint32_t max_b = F.mode == 2 ? F.max_bounces : 0u;
+                shade(F, srgb_dec, ns, depth + 1u, max_b, val, contrib, no_sun);
+                uint32_t ns_slot, ls;
+                append_pair(slot_counters(wf.counters, out_idx), slot_counters(wf.counters, sun_slot(out_idx)), !no_sun,
+                            cap, &ns_slot, &ls);
+                put_rec(&wf.rec[nq][ns_slot], ns, contrib, path, rng);
+                if (!no_sun) wf.sun[nq][ls] = ns_slot;
+                if (depth + 1u < max_b) wf.br[path] = make_float4(val[0], val[1], val[2], 0.0f);
             }
         }
     }
