@@ -36,6 +36,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <cstddef>
 
 #include <cstring>
 
@@ -155,6 +156,35 @@ __device__ __forceinline__ BvhNode gld(const BvhNode* p) {
     n.left = b.z;
     n.right = b.w;
     return n;
+}
+
+__device__ __forceinline__ uint64_t gld64(const uint64_t* p) {
+    return *(const __attribute__((address_space(1))) uint64_t*)(p);
+}
+__device__ __forceinline__ uint8_t gld8(const uint8_t* p) { return *(const __attribute__((address_space(1))) uint8_t*)(p); }
+
+// A volume descriptor read through the global address space (its fields at their offsets).
+static_assert(offsetof(VolumeDesc, fmt) == 24 && offsetof(VolumeDesc, brick_map) == 32 &&
+                  offsetof(VolumeDesc, blocks) == 64 && sizeof(VolumeDesc) == 72,
+              "VolumeDesc layout");
+__device__ __forceinline__ VolumeDesc gld_vol(const VolumeDesc* p) {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
+    const uint64_t* q = reinterpret_cast<const uint64_t*>(w + 8);
+    VolumeDesc v;
+    v.dim[0] = gld(w + 0); v.dim[1] = gld(w + 1); v.dim[2] = gld(w + 2);
+    v.bdim[0] = gld(w + 3); v.bdim[1] = gld(w + 4); v.bdim[2] = gld(w + 5);
+    v.fmt = gld(w + 6);
+    v.nbricks = gld(w + 7);
+    v.brick_map = reinterpret_cast<const uint32_t*>(gld64(q + 0));
+    v.masks = reinterpret_cast<const uint32_t*>(gld64(q + 1));
+    v.payload = reinterpret_cast<const uint8_t*>(gld64(q + 2));
+    v.palette = reinterpret_cast<const uint32_t*>(gld64(q + 3));
+    v.blocks = reinterpret_cast<const uint64_t*>(gld64(q + 4));
+    return v;
+}
+// The volume of instance I.
+__device__ __forceinline__ VolumeDesc inst_vol(const SceneGPU& S, const InstanceGPU* I) {
+    return gld_vol(&S.vols[gld(&I->tex)]);
 }
 
 // Stage a brick's 64 B occupancy mask (16 u32: z-slice z is words 2z, 2z+1) in the LDS slot.
@@ -446,10 +476,6 @@ __device__ __forceinline__ void dda_run(const VolumeDesc& V, const float mpos[3]
 // clamped to the box.  Every step runs the same instructions whatever the box (values chosen
 // by selects), so lanes of a wave walking empty space and lanes walking voxels share each
 // iteration.
-__device__ __forceinline__ uint64_t gld64(const uint64_t* p) {
-    return *(const __attribute__((address_space(1))) uint64_t*)(p);
-}
-
 // One walk, held in registers between the pool rounds of k_secondary: box_walk_begin, then
 // box_walk_step until it returns false (the walk left the volume, hit a voxel, or reached the
 // guard of 2 (dx + dy + dz) + 64 steps).
@@ -664,10 +690,10 @@ __device__ bool enter_instance(const SceneGPU& S, const FrameGPU& F, uint32_t i,
         mdir = xf34(inv, dir, 0.0f);
     }
     ctr.instance();
-    const VolumeDesc V = S.vols[I->tex];
+    const VolumeDesc V = inst_vol(S, I);
     float mpos[3];
     int v0[3];
-    const float size[3] = {I->size[0], I->size[1], I->size[2]};
+    const float size[3] = {gld(&I->size[0]), gld(&I->size[1]), gld(&I->size[2])};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         mpos[a] = (mp[a] + 0.5f) * size[a];
@@ -831,7 +857,7 @@ __device__ __forceinline__ void surface_frame(const InstanceGPU* I, const float 
     float M[12], inv[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) { M[k] = gld(&I->M[k]); inv[k] = gld(&I->inv[k]); }
-    const V3 mp{p[0] / I->size[0] - 0.5f, p[1] / I->size[1] - 0.5f, p[2] / I->size[2] - 0.5f};
+    const V3 mp{p[0] / gld(&I->size[0]) - 0.5f, p[1] / gld(&I->size[1]) - 0.5f, p[2] / gld(&I->size[2]) - 0.5f};
     X = xf34(M, mp, 1.0f);
     const int fa = (int)(face >> 1);
     const float nm = (face & 1u) == 0u ? -1.0f : 1.0f;
@@ -882,15 +908,14 @@ __device__ void cosine_dir(Rng& rng, V3 n, V3& w) {
 }
 
 __device__ __forceinline__ uint32_t hit_rgba(const SceneGPU& S, const HitG& h, uint8_t* pal) {
-    const InstanceGPU* I = &S.inst[h.inst];
-    const VolumeDesc V = S.vols[I->tex];
+    const VolumeDesc V = inst_vol(S, &S.inst[h.inst]);
     if (V.fmt == 0) {
-        const uint32_t idx = V.payload[(size_t)h.d.entry * 512 + h.d.li];
+        const uint32_t idx = gld8(V.payload + (size_t)h.d.entry * 512 + h.d.li);
         *pal = (uint8_t)idx;
-        return V.palette[idx];
+        return gld(V.palette + idx);
     }
     *pal = 0;
-    return reinterpret_cast<const uint32_t*>(V.payload)[(size_t)h.d.entry * 512 + h.d.li];
+    return gld(reinterpret_cast<const uint32_t*>(V.payload) + (size_t)h.d.entry * 512 + h.d.li);
 }
 
 __device__ __forceinline__ uint32_t pixel_seed(uint32_t seed, uint32_t frame, uint32_t pixel, uint32_t sample) {
@@ -1327,7 +1352,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
             for (int k = 0; k < 12; ++k) inv[k] = gld(&I->inv[k]);
             mdir = xf34(inv, w, 0.0f);
             ctr.instance();
-            box_walk_begin(S.vols[I->tex], sf.p, v0, mdir, W);
+            box_walk_begin(inst_vol(S, I), sf.p, v0, mdir, W);
             busy = true;
             walking = true;
         }
@@ -1408,7 +1433,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
                     const SlabR sl = slab_unit(o, d);
                     if (!sl.ok || !(sl.tn > 0.0f)) continue;
                     const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
-                    const float size[3] = {I->size[0], I->size[1], I->size[2]};
+                    const float size[3] = {gld(&I->size[0]), gld(&I->size[1]), gld(&I->size[2])};
                     float mpos[3];
                     int v0[3];
 #pragma unroll
@@ -1425,7 +1450,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
                     face0 = face_from(sl.axis, dd[sl.axis]);
                     inst = cand;
                     ctr.instance();
-                    box_walk_begin(S.vols[I->tex], mpos, v0, mdir, W);
+                    box_walk_begin(inst_vol(S, I), mpos, v0, mdir, W);
                     found = true;
                     break;
                 }
