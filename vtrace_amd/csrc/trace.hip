@@ -544,13 +544,15 @@ __device__ __forceinline__ bool box_walk_step(BoxWalk& W, Ctr<ST>& ctr, const Ld
     const uint64_t blk = W.blk;
     const bool emp = vt_brick_empty(entry);
     const int lx = v[0] - b8[0], ly = v[1] - b8[1], lz = v[2] - b8[2];
-    const bool oct_empty = ((blk >> (uint32_t)(2 * (lx >> 2) + 8 * (ly >> 2) + 32 * (lz >> 2))) & 0x330033ull) == 0ull;
-    const bool blk_empty = ((blk >> (uint32_t)((lx >> 1) + 4 * (ly >> 1) + 16 * (lz >> 1))) & 1ull) == 0ull;
-    const int sz = oct_empty ? 4 : (blk_empty ? 2 : 1);
-    const uint32_t bit = (uint32_t)lx + 8u * (uint32_t)ly;
-    const uint64_t slice = mask_slice(mslot, lz);
-    if (!emp & (sz == 1) & (((uint32_t)(slice >> bit) & 1u) != 0u)) {
-        W.li = bit + 64u * (uint32_t)lz;
+    // (32-bit halves: the block mask's half lz >> 2, the occupancy word of row group ly >> 2 of
+    // slice lz; an empty octant's blocks are all empty, so sz = 1 + blk_empty + 2 oct_empty)
+    const uint32_t bh = lz >= 4 ? (uint32_t)(blk >> 32) : (uint32_t)blk;
+    const bool oct_empty = ((bh >> (uint32_t)(2 * (lx >> 2) + 8 * (ly >> 2))) & 0x330033u) == 0u;
+    const bool blk_empty = ((bh >> (uint32_t)((lx >> 1) + 4 * (ly >> 1) + 16 * ((lz >> 1) & 1))) & 1u) == 0u;
+    const int sz = 1 + (int)blk_empty + 2 * (int)oct_empty;
+    const uint32_t word = mslot.at(2 * lz + (ly >> 2));
+    if (!emp & (sz == 1) & (((word >> (uint32_t)(lx + 8 * (ly & 3))) & 1u) != 0u)) {
+        W.li = (uint32_t)lx + 8u * (uint32_t)ly + 64u * (uint32_t)lz;
         W.hit = true;
         return false;
     }
