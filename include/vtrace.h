@@ -106,6 +106,12 @@ void vt_set_cursor(double x, double y);
  * accumulation follows the previous one's, so the per-pixel sums (radiance and G-buffer) stay
  * in sample order. */
 int32_t vt_set_frame_split(uint32_t chunks);
+/* Workgroups per persistent launch of a one-stream frame (0: 4 per CU, the default).  Results
+ * are identical for any value: work is fetched from counters and queued per path, and an append
+ * that does not fit a queue segment goes on to the next one; fewer workgroups put the appends
+ * in fewer segments (1: every append starts at segment 0, which overflows).  No counterpart in
+ * the reference; for tests and launch-size sweeps.  Returns 0, or -1 before entry(). */
+int32_t vt_set_persistent_blocks(uint32_t blocks);
 /* Like end_update_instances for a list of the same length as the last committed one (moved
  * instances): the records are committed and the LBVH keeps its topology, its boxes refitted
  * bottom-up (no Morton sort or rebuild).  Hits are those of a rebuilt tree; the traversal

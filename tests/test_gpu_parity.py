@@ -279,6 +279,38 @@ def test_split_frames_equal_one_stream(spp, chunks, gbuffer):
         assert np.array_equal(r.gbuffer().view(np.uint32), gb1.view(np.uint32))
 
 
+@pytest.mark.parametrize("blocks", [1, 3])
+def test_few_workgroups_overflow_queue_segments(blocks):
+    """With 1 or 3 persistent workgroups every queue append starts in segment 0 (or 0-2), so the
+    segments overflow into the next ones (the paired bounce-queue and sun-list append's
+    fallback) and a secondary wave's pool refills from every part in turn: hits, radiance and
+    counters equal the default launch's bit for bit and the oracle's (hits exact, radiance
+    within tolerance)."""
+    from vtrace_amd import renderer as R
+    sc = scenes.multi_volume_scene(count=6, n=64, world=512, seed=11, width=192, height=108, spp=2, max_bounces=4)
+    _say(f"case: persistent blocks={blocks}")
+    r = R.reset_renderer()
+    R.load_scene(r, sc, stats=True)
+    r.lib.vt_set_frame_split(1)
+    r.render(sc.P, sc.C)
+    h1, rad1, st1 = r.hits().copy(), r.radiance().copy(), r.stats()
+    assert (h1["inst"] != 0xFFFFFFFF).mean() > 0.2   # more hits than one segment holds
+    try:
+        assert r.lib.vt_set_persistent_blocks(blocks) == 0
+        r.render(sc.P, sc.C)
+        h2, rad2, st2 = r.hits().copy(), r.radiance().copy(), r.stats()
+    finally:
+        r.lib.vt_set_persistent_blocks(0)
+    assert np.array_equal(h2, h1)
+    assert np.array_equal(rad2.view(np.uint32), rad1.view(np.uint32))
+    assert st2 == st1
+    o_hits, o_rad, o_st = O.render(sc, use_bvh=True)
+    assert_hits_equal(h2, o_hits, f"blocks={blocks}")
+    mx, _ = radiance_error(rad2, o_rad)
+    assert mx <= REL_TOL
+    assert st2 == o_st
+
+
 def test_full_size_benchmark_frame():
     """The benchmark workload at its full size (16 x 256^3 instances, 1920x1080, 4 spp, <= 4
     bounces) on the benchmark's path: the four-chunk frame equals the one-stream frame bit for

@@ -11,7 +11,7 @@ import ctypes as C
 REFERENCE_EXPORTS = ("entry", "render_tick", "get_input_data_pointer", "add_texture", "start_update_instances",
                      "end_update_instances", "cleanup")
 EXTENSION_EXPORTS = ("vt_set_params", "vt_get_params", "vt_add_volume_indexed", "vt_add_textures", "vt_lbvh_builds",
-                     "vt_set_cursor", "vt_set_frame_split", "vt_refit_instances",
+                     "vt_set_cursor", "vt_set_frame_split", "vt_set_persistent_blocks", "vt_refit_instances",
                      "vt_read_hits", "vt_read_radiance",
                      "vt_read_bgra8", "vt_read_stats", "vt_read_stage_stats", "vt_last_frame_ms", "vt_bench_frames",
                      "vt_bench_stages", "vt_read_volume", "vt_read_brick_map",
@@ -63,6 +63,8 @@ def declare(lib: C.CDLL) -> None:
     lib.vt_refit_instances.argtypes = [C.c_uint32]
     lib.vt_set_frame_split.restype = C.c_int32
     lib.vt_set_frame_split.argtypes = [C.c_uint32]
+    lib.vt_set_persistent_blocks.restype = C.c_int32
+    lib.vt_set_persistent_blocks.argtypes = [C.c_uint32]
     lib.vt_set_cursor.restype = None
     lib.vt_set_cursor.argtypes = [C.c_double, C.c_double]
     lib.start_update_instances.restype = C.POINTER(C.c_float)

@@ -994,6 +994,14 @@ int32_t vt_set_frame_split(uint32_t chunks) {
     return 0;
 }
 
+int32_t vt_set_persistent_blocks(uint32_t blocks) {
+    Guard G;
+    if (!g.inited) return -1;
+    hipStreamSynchronize(g.stream);
+    vt_trace_set_persistent_blocks(blocks ? blocks : g.cus * 4u);
+    return 0;
+}
+
 int32_t vt_bench_stages(uint32_t warmup, uint32_t iters, float* kernel_ms, float* wall_ms, float* stage_ms,
                         uint32_t max_stages, uint8_t* kinds) {
     Guard G;
