@@ -559,16 +559,18 @@ __device__ __forceinline__ bool box_walk_step(BoxWalk& W, Ctr<ST>& ctr, const Ld
     const int rad = 8 * (int)vt_brick_radius(entry);
     const int k8 = 8 * (int)vt_brick_cube(entry, W.oct);
     const bool anc = k8 - 8 > rad;
+    // The box's side is the same on every axis (anchored cube k8, centred cube 8 + 2 rad, or
+    // sz); in an empty brick its low face is b8 + an offset that depends only on whether the
+    // axis steps down (anchored: 8 - k8, else 0; centred: -rad either way).
+    const int side = emp ? (anc ? k8 : 8 + 2 * rad) : sz;
+    const int off_up = anc ? 0 : -rad, off_down = anc ? 8 - k8 : -rad;
     int lo[3], hi[3];
     float tt[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const int st = W.st[a];
-        const int bl = v[a] & -sz;
-        const int al = st < 0 ? b8[a] + 8 - k8 : b8[a], ah = st < 0 ? b8[a] + 8 : b8[a] + k8;
-        const int cl = anc ? al : b8[a] - rad, ch = anc ? ah : b8[a] + 8 + rad;
-        lo[a] = emp ? cl : bl;
-        hi[a] = emp ? ch : bl + sz;
+        lo[a] = emp ? b8[a] + (st < 0 ? off_down : off_up) : (v[a] & -sz);
+        hi[a] = lo[a] + side;
         const int e = st > 0 ? hi[a] : lo[a];
         const float t = ((float)e - W.q[a]) * W.rinv[a];
         tt[a] = st != 0 ? t : INFINITY;
