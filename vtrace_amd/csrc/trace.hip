@@ -525,13 +525,14 @@ __device__ __forceinline__ void box_walk_begin(const VolumeDesc& V, const float 
 
 template <bool ST>
 __device__ __forceinline__ bool box_walk_step(BoxWalk& W, Ctr<ST>& ctr, const Lds& mslot) {
+    // One exit: the step is computed for every lane and committed only where the walk goes on
+    // (inside the volume, under the guard, v not occupied); only the brick load is guarded.
     const int* v = W.v;
-    if (!((W.steps < W.guard) & ((uint32_t)v[0] < (uint32_t)W.n[0]) & ((uint32_t)v[1] < (uint32_t)W.n[1]) &
-          ((uint32_t)v[2] < (uint32_t)W.n[2])))
-        return false;
+    const bool inb = (W.steps < W.guard) & ((uint32_t)v[0] < (uint32_t)W.n[0]) &
+                     ((uint32_t)v[1] < (uint32_t)W.n[1]) & ((uint32_t)v[2] < (uint32_t)W.n[2]);
     const int b8[3] = {v[0] & ~7, v[1] & ~7, v[2] & ~7};
     const uint32_t bi = (uint32_t)(b8[0] >> 3) + W.bdx * ((uint32_t)(b8[1] >> 3) + W.bdy * (uint32_t)(b8[2] >> 3));
-    if (bi != W.cur) {
+    if (inb & (bi != W.cur)) {
         W.cur = bi;
         W.entry = gld(W.brick_map + bi);
         ctr.brick(!vt_brick_empty(W.entry));
@@ -550,12 +551,9 @@ __device__ __forceinline__ bool box_walk_step(BoxWalk& W, Ctr<ST>& ctr, const Ld
     const bool oct_empty = ((bh >> (uint32_t)(2 * (lx >> 2) + 8 * (ly >> 2))) & 0x330033u) == 0u;
     const bool blk_empty = ((bh >> (uint32_t)((lx >> 1) + 4 * (ly >> 1) + 16 * ((lz >> 1) & 1))) & 1u) == 0u;
     const int sz = 1 + (int)blk_empty + 2 * (int)oct_empty;
-    const uint32_t word = mslot.at(2 * lz + (ly >> 2));
-    if (!emp & (sz == 1) & (((word >> (uint32_t)(lx + 8 * (ly & 3))) & 1u) != 0u)) {
-        W.li = (uint32_t)lx + 8u * (uint32_t)ly + 64u * (uint32_t)lz;
-        W.hit = true;
-        return false;
-    }
+    const uint32_t word = mslot.at(2 * lz + (ly >> 2));   // (lx, ly, lz in 0..7 for any v)
+    const bool hit = inb & !emp & (sz == 1) & (((word >> (uint32_t)(lx + 8 * (ly & 3))) & 1u) != 0u);
+    const bool go = inb & !hit;
     const int rad = 8 * (int)vt_brick_radius(entry);
     const int k8 = 8 * (int)vt_brick_cube(entry, W.oct);
     const bool anc = k8 - 8 > rad;
@@ -589,13 +587,17 @@ __device__ __forceinline__ bool box_walk_step(BoxWalk& W, Ctr<ST>& ctr, const Ld
         const int pe = st > 0 ? h : l;
         const float qe = (float)pe;
         const int ve = st > 0 ? h : l - 1;
-        W.q[a] = ex ? qe : x;
-        W.v[a] = ex ? ve : f;
+        const float qn = ex ? qe : x;
+        const int vn = ex ? ve : f;
+        W.q[a] = go ? qn : W.q[a];
+        W.v[a] = go ? vn : W.v[a];
         mk |= ex ? 1u << a : 0u;
     }
-    W.mask = mk;
-    ++W.steps;
-    return true;
+    W.mask = go ? mk : W.mask;
+    W.steps += go ? 1u : 0u;
+    W.li = hit ? (uint32_t)lx + 8u * (uint32_t)ly + 64u * (uint32_t)lz : W.li;
+    W.hit = hit;
+    return go;
 }
 
 struct HitG {
