@@ -1417,16 +1417,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
                         const BvhNode N = tree.on ? lds_node(tree, ni) : gld(S.nodes + ni);
                         ctr.node();
                         float a0, a1;
-                        if (!slab_node(O, w, inv_d, N, &a0, &a1) || a1 < 0.0f) continue;
-                        if (best.hit && a0 > prune_bound(best.tn)) continue;
-                        if (N.right != VT_EMPTY) {
+                        // (the tests combined without short-circuits: two branches per node)
+                        const bool crossed = slab_node(O, w, inv_d, N, &a0, &a1);
+                        const bool ok = crossed & !(a1 < 0.0f) & !(best.hit & (a0 > prune_bound(best.tn)));
+                        const bool inner = N.right != VT_EMPTY;
+                        if (ok & inner) {
                             stk.push(N.right);
                             stk.push(N.left);
-                            continue;
                         }
-                        if (N.left == origin) continue;
-                        cand = N.left;
-                        break;
+                        if (ok & !inner & (N.left != origin)) {
+                            cand = N.left;
+                            break;
+                        }
                     }
                     if (cand == VT_EMPTY) break;
                     const InstanceGPU* I = &S.inst[cand];
