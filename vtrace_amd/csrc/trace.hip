@@ -1282,7 +1282,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         // -- take items for the idle lanes (consecutive items of one part in lane order) --
         const uint64_t idle = __ballot(!busy);
         bool fresh = false;
-        uint32_t item = 0, ipart = 0;
+        uint32_t item = 0, ipart = 0, inb_count = 0;
         if (idle != 0ull && kp < kParts) {
             const uint32_t need = (uint32_t)__popcll(idle);
             const uint32_t rank = (uint32_t)__popcll(idle & below);
@@ -1290,9 +1290,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
             while (placed < need && kp < kParts) {
                 const uint32_t p = (p0 + kp) % kParts;
                 const bool g = (__hip_atomic_load(&gone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & (1u << p)) != 0u;
-                uint32_t take = 0, base = 0;
+                uint32_t take = 0, base = 0, nb_p = 0;
                 if (!g) {
-                    const uint32_t count = (bounce ? queue_count(qin, p, cap) : 0u) + queue_count(qsun, p, cap);
+                    // (both counts loaded before either is used: one round trip)
+                    const uint32_t cb = qin[p * kPartStride], cs = qsun[p * kPartStride];
+                    nb_p = bounce ? min(cb, cap) : 0u;
+                    const uint32_t count = nb_p + min(cs, cap);
                     if (lane == 0 && count > 0) base = atomicAdd(fetch + p * kPartStride, need - placed);
                     base = __shfl(base, 0, 64);
                     take = base < count ? min(count - base, need - placed) : 0u;
@@ -1301,6 +1304,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
                     fresh = true;
                     item = base + (rank - placed);
                     ipart = p;
+                    inb_count = nb_p;
                 }
                 placed += take;
                 if (placed < need) {   // part p has no items left
@@ -1311,7 +1315,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         }
         // -- set up the new items' rays: the origin walk first --
         if (fresh) {
-            const uint32_t nb = bounce ? queue_count(qin, ipart, cap) : 0u;
+            const uint32_t nb = inb_count;   // bounce items of part ipart (read at the refill)
             is_bounce = item < nb;
             const uint32_t slot = is_bounce ? ipart * cap + item : wf.sun[q][ipart * cap + (item - nb)];
             const QueueRec sf = get_rec(&wf.rec[q][slot]);
@@ -1321,6 +1325,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
             const ReqGPU& Rq = reqs[ri];
             const SceneGPU& S = Rq.S;
             const FrameGPU& F = Rq.F;
+            const InstanceGPU* I = &S.inst[origin];
+            const VolumeDesc V0 = inst_vol(S, I);   // (its loads issued before the sampling below)
             V3 X, n;
             surface_frame(&S.inst[sf.inst], sf.p, sf.face, X, n);
             Rng r{sf.rng};
@@ -1352,13 +1358,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
             const int fa0 = (int)(sf.face >> 1), off0 = (sf.face & 1u) == 0u ? -1 : 1;
             const int v0[3] = {(int)sf.v[0] + (fa0 == 0 ? off0 : 0), (int)sf.v[1] + (fa0 == 1 ? off0 : 0),
                                (int)sf.v[2] + (fa0 == 2 ? off0 : 0)};
-            const InstanceGPU* I = &S.inst[origin];
             float inv[12];
 #pragma unroll
             for (int k = 0; k < 12; ++k) inv[k] = gld(&I->inv[k]);
             mdir = xf34(inv, w, 0.0f);
             ctr.instance();
-            box_walk_begin(inst_vol(S, I), sf.p, v0, mdir, W);
+            box_walk_begin(V0, sf.p, v0, mdir, W);
             busy = true;
             walking = true;
         }
@@ -1433,6 +1438,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
                     if (cand == VT_EMPTY) break;
                     const InstanceGPU* I = &S.inst[cand];
                     if (!gld(&I->valid)) continue;
+                    const VolumeDesc Vc = inst_vol(S, I);   // (loads issued before the entry test)
                     float inv[12];
 #pragma unroll
                     for (int k = 0; k < 12; ++k) inv[k] = gld(&I->inv[k]);
@@ -1458,7 +1464,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
                     face0 = face_from(sl.axis, dd[sl.axis]);
                     inst = cand;
                     ctr.instance();
-                    box_walk_begin(inst_vol(S, I), mpos, v0, mdir, W);
+                    box_walk_begin(Vc, mpos, v0, mdir, W);
                     found = true;
                     break;
                 }
