@@ -322,18 +322,25 @@ __device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST
                 uint32_t rx = (uint32_t)__mul24(stx, ox - vx);
                 uint32_t ry = (uint32_t)__mul24(sty, oy - vy);
                 uint32_t rz = (uint32_t)__mul24(stz, oz - vz);
-                float t;
-                for (;;) {
-                    t = fminf(fminf(sx, sy), sz);
-                    const bool mx = sx <= t, my = sy <= t, mz = sz <= t;
-                    rx -= mx ? 1u : 0u;
-                    ry -= my ? 1u : 0u;
-                    rz -= mz ? 1u : 0u;
-                    if (min(rx, min(ry, rz)) == 0u) break;
+                // (the exit test at the bottom: the loop body is one block, without the exec
+                // saves a test in its middle costs)
+                float t = fminf(fminf(sx, sy), sz);
+                bool mx = sx <= t, my = sy <= t, mz = sz <= t;
+                rx -= mx ? 1u : 0u;
+                ry -= my ? 1u : 0u;
+                rz -= mz ? 1u : 0u;
+                while (min(rx, min(ry, rz)) != 0u) {
                     sx += mx ? dx : zx;
                     sy += my ? dy : zy;
                     sz += mz ? dz : zz;
                     ++steps;
+                    t = fminf(fminf(sx, sy), sz);
+                    mx = sx <= t;
+                    my = sy <= t;
+                    mz = sz <= t;
+                    rx -= mx ? 1u : 0u;
+                    ry -= my ? 1u : 0u;
+                    rz -= mz ? 1u : 0u;
                 }
                 const bool lmx = sx <= t, lmy = sy <= t, lmz = sz <= t;
                 lm = (lmx ? 1u : 0u) | (lmy ? 2u : 0u) | (lmz ? 4u : 0u);
