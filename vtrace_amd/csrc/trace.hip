@@ -301,8 +301,10 @@ __device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST
         lm = (mx ? 1u : 0u) | (my ? 2u : 0u) | (mz ? 4u : 0u);                          \
         ++steps;                                                                        \
     }
-    while (steps < max_steps && (uint32_t)vx < (uint32_t)nx && (uint32_t)vy < (uint32_t)ny &&
-           (uint32_t)vz < (uint32_t)nz) {
+    // The brick loop tests its condition at the bottom (continue goes there) and a hit ends it
+    // through the condition, so the loop has one exit.
+    if ((steps < max_steps) & ((uint32_t)vx < (uint32_t)nx) & ((uint32_t)vy < (uint32_t)ny) &
+        ((uint32_t)vz < (uint32_t)nz)) do {
         // entering a brick: the current voxel is inside the volume
         const int bx8 = vx & ~7, by8 = vy & ~7, bz8 = vz & ~7;
         // the voxel through which each stepping axis leaves this brick (or the volume); in the
@@ -410,7 +412,6 @@ __device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST
             if (min(rx, min(ry, rz)) != 0u) {
                 li = bit + 64u * (uint32_t)lz;
                 hit = true;
-                break;
             }
             continue;
         }
@@ -431,8 +432,8 @@ __device__ __forceinline__ void dda_walk(const VolumeDesc& V, DdaWalk& w, Ctr<ST
             VT_DDA_STEP();
             if (!(steps < max_steps && vx != ox && vy != oy && vz != oz)) break;
         }
-        if (hit) break;
-    }
+    } while (!hit & (steps < max_steps) & ((uint32_t)vx < (uint32_t)nx) & ((uint32_t)vy < (uint32_t)ny) &
+             ((uint32_t)vz < (uint32_t)nz));
 #undef VT_DDA_STEP
     w.vx = vx; w.vy = vy; w.vz = vz;
     w.steps = steps; w.entry = entry; w.li = li; w.t_last = t_last;
