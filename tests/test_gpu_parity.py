@@ -311,6 +311,50 @@ def test_few_workgroups_overflow_queue_segments(blocks):
     assert st2 == o_st
 
 
+def _deep_chain_scene():
+    """31 instances of 8^3 volumes (6 world units wide) centred at 8 * 2^k units along each
+    axis (k = 0..9), plus one at the far corner: their Morton codes are powers of two, so the
+    LBVH is a chain about 30 levels deep; path traced at 160x120, 2 spp, <= 3 bounces."""
+    vols = [scenes.Volume(scenes.gen_volume((8, 8, 8), 50 + k, 0.5), scenes.gen_palette(50 + k)) for k in range(4)]
+    recs = []
+    for a in range(3):
+        for k in range(10):
+            c = [0.0, 0.0, 0.0]
+            c[a] = 8.0 * 2 ** k
+            recs.append(scenes.instance_record(scenes.model_matrix(tuple(c), 6.0), len(recs) % 4))
+    recs.append(scenes.instance_record(scenes.model_matrix((8184.0, 8184.0, 8184.0), 6.0), 0))
+    p = scenes.RenderParams(160, 120, viewport_square=0, mode=2, spp=2, max_bounces=3)
+    return scenes.Scene(vols, np.stack(recs).astype(np.float32), scenes.perspective(scenes.FOV, 160 / 120),
+                        scenes.look_at((-12.0, 10.0, -12.0), (20.0, 0.0, 20.0)), p, name="deep_chain_31")
+
+
+def _tree_depth(nodes):
+    depth, stack = 0, [(0, 1)]
+    while stack:
+        i, k = stack.pop()
+        depth = max(depth, k)
+        if nodes[i, 7] != 0xFFFFFFFF:
+            stack += [(int(nodes[i, 6]), k + 1), (int(nodes[i, 7]), k + 1)]
+    return depth
+
+
+def test_deep_tree_traversal():
+    """A tree deeper than the traversal stack held in LDS (16 entries): primary queries and the
+    secondary search use the per-lane stack in private memory.  Hits bit-exact, radiance within
+    tolerance and every counter equal to the oracle's."""
+    sc = _deep_chain_scene()
+    o_nodes, _, _ = O.lbvh(sc.records, len(sc.volumes))   # the device builds the same tree
+    assert _tree_depth(o_nodes) > 17
+    _say("case: deep LBVH chain, 31 instances, 160x120 2 spp 3 bounces")
+    g_hits, g_rad, g_st = gpu_render(sc, stats=True)
+    o_hits, o_rad, o_st = O.render(sc, use_bvh=True)
+    assert_hits_equal(g_hits, o_hits, "deep chain")
+    assert (g_hits["inst"] != 0xFFFFFFFF).mean() > 0.1
+    mx, _ = radiance_error(g_rad, o_rad)
+    assert mx <= REL_TOL
+    assert g_st == o_st, (g_st, o_st)
+
+
 def test_full_size_benchmark_frame():
     """The benchmark workload at its full size (16 x 256^3 instances, 1920x1080, 4 spp, <= 4
     bounces) on the benchmark's path: the four-chunk frame equals the one-stream frame bit for
