@@ -280,9 +280,14 @@ class Renderer:
             raise RendererError(self.error())
         return out.reshape(h, w)
 
-    def radiance(self) -> np.ndarray:
+    def radiance(self, out: np.ndarray | None = None) -> np.ndarray:
+        """(h, w, 4) float32 radiance of the last frame; into `out` (C-contiguous float32 of that
+        shape, e.g. a view of a page-locked buffer) when given."""
         w, h = self.window_width, self.window_height
-        out = np.zeros((h, w, 4), dtype=np.float32)
+        if out is None:
+            out = np.zeros((h, w, 4), dtype=np.float32)
+        elif out.shape != (h, w, 4) or out.dtype != np.float32 or not out.flags.c_contiguous:
+            raise ValueError("radiance: out must be C-contiguous float32 of shape (height, width, 4)")
         if self.lib.vt_read_radiance(out.ctypes.data, out.nbytes) != 0:
             raise RendererError(self.error())
         return out
