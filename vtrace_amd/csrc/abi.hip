@@ -524,6 +524,7 @@ bool vt_rt_inited() { return g.inited; }
 int vt_rt_device() { return g.device; }
 uint32_t vt_rt_cus() { return g.cus; }
 hipStream_t vt_rt_stream() { return g.stream; }
+hipStream_t vt_rt_chunk_stream(uint32_t k) { return k == 0 ? g.stream : k < kMaxChunks ? g.xstream[k] : nullptr; }
 const VolumeDesc* vt_rt_vol_table() { return g.vol_table; }
 uint32_t vt_rt_nvol() { return (uint32_t)g.vols.size(); }
 void vt_rt_set_error(const char* what) { set_err(what); }

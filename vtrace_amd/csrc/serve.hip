@@ -110,8 +110,10 @@ struct Serve {
     Arena arena;
     void* wave_mem = nullptr;
     WaveBuffers wave;
-    // sub-batches 1..streams-1: own wavefront block (work budget + riders' quarter), stream,
-    // graphs; acc_done[j] marks sub-batch j accumulated
+    // sub-batches 1..streams-1: own wavefront block (work budget + riders' quarter), graphs,
+    // and the plain path's chunk streams (not new ones: a process gets few hardware queues,
+    // and streams beyond them share one, so extra streams would run sub-batches one after
+    // another); acc_done[j] marks sub-batch j accumulated
     void* sub_mem[kSubBatches] = {};
     WaveBuffers sub_wave[kSubBatches];
     hipStream_t sub[kSubBatches] = {};
@@ -154,7 +156,6 @@ void free_all() {
             for (auto& gx : per_deep)
                 if (gx) hipGraphExecDestroy(gx);
         hipFree(sv.sub_mem[j]);
-        if (sv.sub[j]) hipStreamDestroy(sv.sub[j]);
         if (sv.acc_done[j]) hipEventDestroy(sv.acc_done[j]);
     }
     if (sv.start) hipEventDestroy(sv.start);
@@ -305,8 +306,8 @@ int32_t vt_serve_init(const vt_serve_config* cfg) {
     ok = ok && hipEventCreateWithFlags(&sv.start, hipEventDisableTiming) == hipSuccess;
     for (uint32_t j = 0; j < c.streams && ok; ++j) {
         ok = hipEventCreateWithFlags(&sv.acc_done[j], hipEventDisableTiming) == hipSuccess;
-        if (j > 0)  // default priority, like the library stream
-            ok = ok && hipStreamCreateWithFlags(&sv.sub[j], hipStreamNonBlocking) == hipSuccess &&
+        if (j > 0)
+            ok = ok && (sv.sub[j] = vt_rt_chunk_stream(j)) != nullptr &&
                  hipMalloc(&sv.sub_mem[j], vt_wave_bytes((uint32_t)sub_paths)) == hipSuccess;
         if (ok && j > 0) vt_wave_bind(sv.sub_wave[j], sv.sub_mem[j], (uint32_t)sub_paths);
     }

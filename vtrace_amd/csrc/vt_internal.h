@@ -104,6 +104,8 @@ bool vt_rt_inited();
 int vt_rt_device();
 uint32_t vt_rt_cus();   // compute units of the device (0 if unknown)
 hipStream_t vt_rt_stream();
+// the plain path's stream of sample chunk k (0 = the library stream; nullptr if k is too large)
+hipStream_t vt_rt_chunk_stream(uint32_t k);
 const VolumeDesc* vt_rt_vol_table();
 uint32_t vt_rt_nvol();
 void vt_rt_set_error(const char* what);
