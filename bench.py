@@ -8,9 +8,9 @@ A step is one frame: LBVH traversal + brick DDA + shading for every pixel sample
 volumes, LBVH and instances already resident in HBM.  Rays counted = traversal queries
 (primary + shadow + bounce) from the library's own counters on an untimed counting frame.
 `value` = rays per frame / mean host time per frame over the K timed frames, rendered as
-render_tick renders them (four sample chunks on four streams, each frame finished before the
-next).  The launch-stage table and the roofline come from the same frames on one stream with
-HIP events around every launch (`single_stream_ms_per_frame`).
+render_tick renders them (two sample chunks on two streams, the library's default, each frame
+finished before the next).  The launch-stage table and the roofline come from the same frames
+on one stream with HIP events around every launch (`single_stream_ms_per_frame`).
 
 Roofline (HBM-bound gather work, no MFMA): algorithmic bytes = 32 B per LBVH node visited
 + 64 B per instance entered + (4 B + 64 B if non-empty) per 8^3 brick entered + 5 B per hit,
@@ -54,6 +54,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BF16_MFMA_PEAK_TFLOPS = 2500.0     # MI355X dense BF16
+FRAME_CHUNKS = 2                   # sample chunks (streams) per frame: the library's default
 KERNEL_NAMES = {"render": "k_render0", "primary": "k_primary", "secondary": "k_secondary",
                 "accumulate": "k_accumulate"}
 
@@ -437,7 +438,7 @@ def main():
     sc = scenes.config2(args.width, args.height)
     t_gen = time.perf_counter() - t0
     r = R.get_renderer()
-    r.lib.vt_set_frame_split(1 if args.single_stream else 4)
+    r.lib.vt_set_frame_split(1 if args.single_stream else FRAME_CHUNKS)
     t0 = time.perf_counter()
     R.load_scene(r, sc, stats=True)
     t_up = time.perf_counter() - t0
@@ -454,7 +455,7 @@ def main():
     r.render(sc.P, sc.C)
     # timed: W untimed frames, then exactly K frames as render_tick renders them (each complete
     # before the next); wall_ms = host time per frame, kernel_ms = GPU time per frame
-    r.lib.vt_set_frame_split(1 if args.single_stream else 4)
+    r.lib.vt_set_frame_split(1 if args.single_stream else FRAME_CHUNKS)
     r.bench(args.warmup, 1) if args.warmup > 0 else None
     r.lib.vt_synchronize()
     t0 = time.perf_counter()
@@ -465,7 +466,7 @@ def main():
     # around every launch; the roofline of the dominant kernel comes from these
     r.lib.vt_set_frame_split(1)
     single_kernel_ms, single_wall_ms, stage_ms = r.bench_stages(args.warmup, args.steps)
-    r.lib.vt_set_frame_split(1 if args.single_stream else 4)
+    r.lib.vt_set_frame_split(1 if args.single_stream else FRAME_CHUNKS)
     # per-frame latency distribution (render_tick round trips, GPU time)
     lat = []
     for _ in range(max(5, min(args.steps, 20))):

@@ -102,7 +102,7 @@ int32_t vt_add_textures(const uint8_t* rgba8, uint32_t count, uint32_t width, ui
  * get_input_data_pointer directly, as the reference's key callback does (lib/entry.c:31-52). */
 void vt_set_cursor(double x, double y);
 /* Path-traced frames of 2+ samples run as min(chunks, spp) sample chunks on as many streams
- * (chunks <= 4, default 4; 0 or 1: one stream).  Results are identical: each chunk's
+ * (chunks <= 4, default 2; 0 or 1: one stream).  Results are identical: each chunk's
  * accumulation follows the previous one's, so the per-pixel sums (radiance and G-buffer) stay
  * in sample order. */
 int32_t vt_set_frame_split(uint32_t chunks);

@@ -357,8 +357,9 @@ def test_deep_tree_traversal():
 
 def test_full_size_benchmark_frame():
     """The benchmark workload at its full size (16 x 256^3 instances, 1920x1080, 4 spp, <= 4
-    bounces) on the benchmark's path: the four-chunk frame equals the one-stream frame bit for
-    bit, and bands of rows of it match the oracle (hits bit-exact, radiance within 1e-5)."""
+    bounces) on the benchmark's path: the two-chunk (default) and four-chunk frames equal the
+    one-stream frame bit for bit, and bands of rows of it match the oracle (hits bit-exact,
+    radiance within 1e-5)."""
     from vtrace_amd import renderer as R
     sc = scenes.config2()
     _say("case: full-size benchmark frame 1920x1080 4 spp")
@@ -367,10 +368,11 @@ def test_full_size_benchmark_frame():
     r.lib.vt_set_frame_split(1)
     r.render(sc.P, sc.C)
     h1, rad1 = r.hits().copy(), r.radiance().copy()
-    r.lib.vt_set_frame_split(4)
-    r.render(sc.P, sc.C)
-    assert np.array_equal(r.hits(), h1)
-    assert np.array_equal(r.radiance().view(np.uint32), rad1.view(np.uint32))
+    for chunks in (2, 4):
+        r.lib.vt_set_frame_split(chunks)
+        r.render(sc.P, sc.C)
+        assert np.array_equal(r.hits(), h1)
+        assert np.array_equal(r.radiance().view(np.uint32), rad1.view(np.uint32))
     assert (h1["inst"] != 0xFFFFFFFF).mean() > 0.2
     osc = O.OracleScene(sc, use_bvh=True)
     for y0 in (0, 200, 360, 540, 720, 1064):

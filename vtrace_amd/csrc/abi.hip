@@ -83,7 +83,10 @@ struct State {
     WaveBuffers wave;
     // frames in sample chunks on several streams: chunk k > 0 on xstream[k] with its own
     // wavefront block; ev_chunk[k] marks chunk k accumulated
-    uint32_t split = kMaxChunks;  // chunks per path-traced frame (1 = one stream)
+    // chunks per path-traced frame (1 = one stream).  Two: on the 1080p 4-spp frame, 2 chunks of
+    // 3 workgroups per CU took 2.88 ms, 2 of 4 per CU 2.89, 4 of 2 per CU 2.91, 3 of 3 2.98, one
+    // stream 3.04 (means over 4-6 runs)
+    uint32_t split = 2;
     uint32_t cus = 0;             // compute units (persistent grids are sized per CU)
     hipStream_t xstream[kMaxChunks] = {};
     hipEvent_t ev_chunk[kMaxChunks] = {};
@@ -475,9 +478,10 @@ int render_split(const FrameGPU& F, const SceneGPU& S, uint32_t n) {
     R.gbuf = g.gbuf_on ? g.gbuf : nullptr;
     R.gacc = g.gbuf_on ? g.gacc : nullptr;
     R.tile_w = 8;
-    // persistent workgroups per launch: 4 per CU fill the GPU alone; with 3-4 chunks in flight,
-    // 2 per CU let two chunks' launches share it (measured on the 1080p 4-spp frame)
-    const uint32_t blocks = g.cus ? g.cus * std::min(4u, std::max(2u, 8u / n)) : 0u;
+    // persistent workgroups per launch: 4 per CU fill the GPU alone; two chunks in flight take 3
+    // per CU each, 3-4 chunks 2 per CU, so the launches of two chunks share the GPU (measured on
+    // the 1080p 4-spp frame)
+    const uint32_t blocks = g.cus ? g.cus * (n == 2 ? 3u : 2u) : 0u;
     int rc = 0;
     for (uint32_t k = 0; k < n && rc == 0; ++k) {
         R.s0 = k * spp / n;
