@@ -56,8 +56,10 @@ def _records(n, seed, spread=4096.0, size=64.0):
     return np.stack(recs)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 16, 255, 1000, 5000])
+@pytest.mark.parametrize("n", [1, 2, 3, 16, 255, 1000, 4096, 5000, 9261, 40000])
 def test_lbvh_matches_oracle(n):
+    """Up to 4096 instances the sort runs in one workgroup, above it one workgroup per 1024-key
+    chunk per pass: nodes, order and instance data equal the oracle's either way."""
     _say(f"case: lbvh n={n}")
     r = R.reset_renderer()
     for k in range(3):

@@ -9,6 +9,8 @@
 //   k_morton          30-bit Morton code of the normalized centroid;
 //   k_radix_sort      stable LSD radix sort (4 x 8-bit digits) in one workgroup, chunked
 //                     through LDS: per-wave digit match by ballots, per-wave counts, prefix;
+//                     above kSortOneGroupMax keys the same passes run one workgroup per
+//                     1024-key chunk (k_sort_hist, k_sort_scan, k_sort_scatter);
 //   k_karras          internal nodes (Karras, HPG 2012), leaves, parent links;
 //   k_refit           bottom-up AABB union, second arrival at a node continues (agent-scope
 //                     atomics and loads, so no stale L1 data is read).
@@ -23,6 +25,8 @@ namespace {
 
 constexpr int kSortThreads = 1024;
 constexpr int kSortWaves = kSortThreads / 64;
+// keys sorted by the one-workgroup kernel; more keys: one workgroup per chunk per pass
+constexpr uint32_t kSortOneGroupMax = 4096;
 
 __device__ inline float gmin(float x, float y) { return (y < x) ? y : x; }
 __device__ inline float gmax(float x, float y) { return (x < y) ? y : x; }
@@ -168,13 +172,44 @@ __global__ void k_morton(const float* cen, uint32_t n, const float* bounds, uint
     vals[i] = i;
 }
 
+// Rank of a key among the keys of its 1024-key chunk with the same digit d that come before
+// it (a stable counting sort's offset within the digit): the wave's keys of digit d are
+// matched by 8 ballots, counted per wave, and the counts prefixed over the waves.  On return
+// wcount[w][d] holds the keys of digit d in waves before w; tot[d] (thread d < 256 writes it)
+// the chunk's keys of digit d.  Called by every thread of the workgroup.
+__device__ __forceinline__ uint32_t chunk_rank(uint32_t d, bool valid, uint32_t (*wcount)[256], uint32_t* tot) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (uint32_t j = tid; j < kSortWaves * 256; j += kSortThreads) (&wcount[0][0])[j] = 0;
+    __syncthreads();
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < 8; ++b) {
+        uint64_t bal = __ballot((d >> b) & 1u);
+        peers &= ((d >> b) & 1u) ? bal : ~bal;
+    }
+    const uint64_t lower = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t rank = (uint32_t)__popcll(peers & lower);
+    if (valid && rank == 0) wcount[wave][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (tid < 256) {  // prefix over waves, per digit
+        uint32_t acc = 0;
+        for (int w = 0; w < kSortWaves; ++w) {
+            uint32_t c = wcount[w][tid];
+            wcount[w][tid] = acc;
+            acc += c;
+        }
+        tot[tid] = acc;
+    }
+    __syncthreads();
+    return wcount[wave][d] + rank;
+}
+
 // Stable LSD radix sort, one workgroup, chunks of 1024 keys through LDS.
 __global__ __launch_bounds__(kSortThreads) void k_radix_sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
                                                              uint32_t n) {
     __shared__ uint32_t base[256];
     __shared__ uint32_t wcount[kSortWaves][256];
     __shared__ uint32_t hist[256];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t tid = threadIdx.x;
     uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;
     for (int pass = 0; pass < 4; ++pass) {
         const uint32_t shift = pass * 8;
@@ -195,29 +230,9 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_sort(uint32_t* k0, uint3
             const bool valid = i < n;
             uint32_t key = valid ? kin[i] : 0u, val = valid ? vin[i] : 0u;
             uint32_t d = (key >> shift) & 255u;
-            for (uint32_t j = tid; j < kSortWaves * 256; j += kSortThreads) (&wcount[0][0])[j] = 0;
-            __syncthreads();
-            uint64_t peers = __ballot(valid);
-            for (int b = 0; b < 8; ++b) {
-                uint64_t bal = __ballot((d >> b) & 1u);
-                peers &= ((d >> b) & 1u) ? bal : ~bal;
-            }
-            const uint64_t lower = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-            const uint32_t rank = (uint32_t)__popcll(peers & lower);
-            if (valid && rank == 0) wcount[wave][d] = (uint32_t)__popcll(peers);
-            __syncthreads();
-            if (tid < 256) {  // prefix over waves, per digit
-                uint32_t acc = 0;
-                for (int w = 0; w < kSortWaves; ++w) {
-                    uint32_t c = wcount[w][tid];
-                    wcount[w][tid] = acc;
-                    acc += c;
-                }
-                hist[tid] = acc;  // chunk total per digit
-            }
-            __syncthreads();
+            const uint32_t r = chunk_rank(d, valid, wcount, hist);   // hist: chunk total per digit
             if (valid) {
-                uint32_t pos = base[d] + wcount[wave][d] + rank;
+                uint32_t pos = base[d] + r;
                 kout[pos] = key;
                 vout[pos] = val;
             }
@@ -231,6 +246,65 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_sort(uint32_t* k0, uint3
         __syncthreads();
     }
     // 4 passes: result is back in (k0, v0)
+}
+
+// The same passes with one workgroup per 1024-key chunk.  k_sort_hist: the chunk's digit counts,
+// stored digit-major (ghist[d * nch + c]), so an exclusive scan of ghist gives each chunk's first
+// position for each digit, chunks in order.
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t* kin, uint32_t n, uint32_t shift,
+                                                            uint32_t* ghist) {
+    __shared__ uint32_t hist[256];
+    const uint32_t tid = threadIdx.x, c = blockIdx.x, nch = gridDim.x;
+    if (tid < 256) hist[tid] = 0;
+    __syncthreads();
+    const uint32_t i = c * kSortThreads + tid;
+    if (i < n) atomicAdd(&hist[(kin[i] >> shift) & 255u], 1u);
+    __syncthreads();
+    if (tid < 256) ghist[tid * nch + c] = hist[tid];
+}
+
+// Exclusive scan of ghist[0, total) in place, one workgroup: each thread sums a contiguous
+// segment, the segment sums are scanned in LDS, then each segment is rewritten.
+__global__ __launch_bounds__(kSortThreads) void k_sort_scan(uint32_t* ghist, uint32_t total) {
+    __shared__ uint32_t part[kSortThreads];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t per = (total + kSortThreads - 1) / kSortThreads;
+    const uint32_t b = min(tid * per, total), e = min(b + per, total);
+    uint32_t s = 0;
+    for (uint32_t j = b; j < e; ++j) s += ghist[j];
+    part[tid] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < (uint32_t)kSortThreads; off <<= 1) {
+        const uint32_t v = tid >= off ? part[tid - off] : 0u;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    uint32_t acc = part[tid] - s;
+    for (uint32_t j = b; j < e; ++j) {
+        const uint32_t c = ghist[j];
+        ghist[j] = acc;
+        acc += c;
+    }
+}
+
+// Scatter of chunk c: position = the chunk's first slot for the digit + the key's stable rank.
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                                                               uint32_t* vout, uint32_t n, uint32_t shift,
+                                                               const uint32_t* ghist) {
+    __shared__ uint32_t wcount[kSortWaves][256];
+    __shared__ uint32_t tot[256];
+    const uint32_t tid = threadIdx.x, c = blockIdx.x, nch = gridDim.x;
+    const uint32_t i = c * kSortThreads + tid;
+    const bool valid = i < n;
+    const uint32_t key = valid ? kin[i] : 0u, val = valid ? vin[i] : 0u;
+    const uint32_t d = (key >> shift) & 255u;
+    const uint32_t r = chunk_rank(d, valid, wcount, tot);
+    if (valid) {
+        const uint32_t pos = ghist[d * nch + c] + r;
+        kout[pos] = key;
+        vout[pos] = val;
+    }
 }
 
 __device__ inline int delta_k(const uint32_t* k, int i, int j, int n) {
@@ -337,7 +411,8 @@ int vt_lbvh_reserve(LbvhWork& w, uint32_t n) {
               hipMalloc(&w.keys[0], cap * 4) == hipSuccess && hipMalloc(&w.keys[1], cap * 4) == hipSuccess &&
               hipMalloc(&w.vals[0], cap * 4) == hipSuccess && hipMalloc(&w.vals[1], cap * 4) == hipSuccess &&
               hipMalloc(&w.parent, nn * 4) == hipSuccess && hipMalloc(&w.counters, cap * 4) == hipSuccess &&
-              hipMalloc(&w.bounds, 6 * 4) == hipSuccess && hipMalloc(&w.depth, 4) == hipSuccess;
+              hipMalloc(&w.bounds, 6 * 4) == hipSuccess && hipMalloc(&w.depth, 4) == hipSuccess &&
+              hipMalloc(&w.sort_hist, 256 * 4 * ((size_t)(cap + kSortThreads - 1) / kSortThreads)) == hipSuccess;
     if (!ok) {
         vt_lbvh_free(w);
         return -1;
@@ -349,7 +424,7 @@ int vt_lbvh_reserve(LbvhWork& w, uint32_t n) {
 void vt_lbvh_free(LbvhWork& w) {
     hipFree(w.inst); hipFree(w.nodes); hipFree(w.cen);
     hipFree(w.keys[0]); hipFree(w.keys[1]); hipFree(w.vals[0]); hipFree(w.vals[1]);
-    hipFree(w.parent); hipFree(w.counters); hipFree(w.bounds); hipFree(w.depth);
+    hipFree(w.parent); hipFree(w.counters); hipFree(w.bounds); hipFree(w.depth); hipFree(w.sort_hist);
     w = LbvhWork{};
 }
 
@@ -380,7 +455,19 @@ int vt_lbvh_build(LbvhWork& w, const float* records_dev, uint32_t n, const Volum
                        w.cen);
     hipLaunchKernelGGL(k_bounds, dim3(1), dim3(1024), 0, s, w.cen, n, w.bounds);
     hipLaunchKernelGGL(k_morton, dim3(nb), dim3(tb), 0, s, w.cen, n, w.bounds, w.keys[0], w.vals[0]);
-    hipLaunchKernelGGL(k_radix_sort, dim3(1), dim3(kSortThreads), 0, s, w.keys[0], w.vals[0], w.keys[1], w.vals[1], n);
+    if (n <= kSortOneGroupMax) {
+        hipLaunchKernelGGL(k_radix_sort, dim3(1), dim3(kSortThreads), 0, s, w.keys[0], w.vals[0], w.keys[1], w.vals[1],
+                           n);
+    } else {
+        const uint32_t nch = (n + kSortThreads - 1) / kSortThreads;   // 256 * nch <= sort_hist's size
+        for (uint32_t pass = 0; pass < 4; ++pass) {   // pass p reads buffer p & 1: the result ends in (keys[0], vals[0])
+            const uint32_t a = pass & 1u, b = a ^ 1u, shift = 8 * pass;
+            hipLaunchKernelGGL(k_sort_hist, dim3(nch), dim3(kSortThreads), 0, s, w.keys[a], n, shift, w.sort_hist);
+            hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kSortThreads), 0, s, w.sort_hist, 256 * nch);
+            hipLaunchKernelGGL(k_sort_scatter, dim3(nch), dim3(kSortThreads), 0, s, w.keys[a], w.vals[a], w.keys[b],
+                               w.vals[b], n, shift, w.sort_hist);
+        }
+    }
     hipLaunchKernelGGL(k_leaves, dim3(nb), dim3(tb), 0, s, w.vals[0], n, leaves_tmp, w.nodes);
     hipMemsetAsync(w.depth, 0, 4, s);
     if (n > 1) {

@@ -44,6 +44,7 @@ struct LbvhWork {
     uint32_t* counters = nullptr;  // cap
     float* bounds = nullptr;    // 6
     uint32_t* depth = nullptr;  // tree depth (edges on the longest root-leaf path)
+    uint32_t* sort_hist = nullptr;  // 256 digit counts per 1024-key chunk (multi-workgroup sort)
 };
 int vt_lbvh_reserve(LbvhWork& w, uint32_t n);
 void vt_lbvh_free(LbvhWork& w);
