@@ -1102,6 +1102,10 @@ __device__ __forceinline__ uint32_t find_req(const uint32_t* base, uint32_t n, u
     return lo;
 }
 
+// Work items of a request: tiles of 64 pixels (tw x th) in row-major order, each tile's
+// 64 * ns items its pixels' samples, a pixel's ns samples side by side; the pixels of an 8 x 8
+// tile in Morton order, so a wave's 64 items cover a compact block of 64 / ns pixels (8 x 4 at
+// 2 samples, 4 x 4 at 4) whose rays stay close together.
 struct TileMap {
     uint32_t tw, th, tiles_x, ns;
     __device__ __forceinline__ TileMap(const ReqGPU& R) {
@@ -1110,20 +1114,35 @@ struct TileMap {
         tiles_x = (R.F.width + tw - 1u) / tw;
         ns = R.s1 - R.s0;
     }
+    // Morton order of an 8 x 8 tile: x in bits 0, 2, 4 and y in bits 1, 3, 5
+    __device__ __forceinline__ void tile_xy(uint32_t pt, uint32_t* x, uint32_t* y) const {
+        if (tw == 8u) {
+            *x = (pt & 1u) | ((pt >> 1) & 2u) | ((pt >> 2) & 4u);
+            *y = ((pt >> 1) & 1u) | ((pt >> 2) & 2u) | ((pt >> 3) & 4u);
+        } else {
+            *x = pt % tw;
+            *y = pt / tw;
+        }
+    }
+    __device__ __forceinline__ uint32_t tile_pt(uint32_t x, uint32_t y) const {
+        if (tw == 8u)
+            return (x & 1u) | ((y & 1u) << 1) | ((x & 2u) << 1) | ((y & 2u) << 2) | ((x & 4u) << 2) | ((y & 4u) << 3);
+        return y * tw + x;
+    }
     // local work index -> pixel and sample
     __device__ __forceinline__ void pixel(uint32_t lw, uint32_t s0, uint32_t* px, uint32_t* py, uint32_t* s) const {
         const uint32_t per_tile = 64u * ns;
         const uint32_t tile = lw / per_tile, rem = lw % per_tile;
-        *s = s0 + rem / 64u;
-        const uint32_t pt = rem % 64u;
-        *px = (tile % tiles_x) * tw + pt % tw;
-        *py = (tile / tiles_x) * th + pt / tw;
+        *s = s0 + rem % ns;
+        uint32_t x, y;
+        tile_xy(rem / ns, &x, &y);
+        *px = (tile % tiles_x) * tw + x;
+        *py = (tile / tiles_x) * th + y;
     }
     // pixel and sample -> local work index
     __device__ __forceinline__ uint32_t index(uint32_t px, uint32_t py, uint32_t s, uint32_t s0) const {
         const uint32_t tile = (py / th) * tiles_x + px / tw;
-        const uint32_t pt = (py % th) * tw + px % tw;
-        return tile * (64u * ns) + (s - s0) * 64u + pt;
+        return tile * (64u * ns) + tile_pt(px % tw, py % th) * ns + (s - s0);
     }
 };
 
