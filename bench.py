@@ -8,9 +8,9 @@ A step is one frame: LBVH traversal + brick DDA + shading for every pixel sample
 volumes, LBVH and instances already resident in HBM.  Rays counted = traversal queries
 (primary + shadow + bounce) from the library's own counters on an untimed counting frame.
 `value` = rays per frame / mean host time per frame over the K timed frames, rendered as
-render_tick renders them (two sample chunks on two streams, the library's default, each frame
-finished before the next).  The launch-stage table and the roofline come from the same frames
-on one stream with HIP events around every launch (`single_stream_ms_per_frame`).
+render_tick renders them (three chunks of tile rows on three streams, the library's default,
+each frame finished before the next).  The launch-stage table and the roofline come from the
+same frames on one stream with HIP events around every launch (`single_stream_ms_per_frame`).
 
 Roofline (HBM-bound gather work, no MFMA): algorithmic bytes = 32 B per LBVH node visited
 + 64 B per instance entered + (4 B + 64 B if non-empty) per 8^3 brick entered + 5 B per hit,
@@ -54,7 +54,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BF16_MFMA_PEAK_TFLOPS = 2500.0     # MI355X dense BF16
-FRAME_CHUNKS = 2                   # sample chunks (streams) per frame: the library's default
+FRAME_CHUNKS = 3                   # chunks of tile rows (streams) per frame: the library's default
 KERNEL_NAMES = {"render": "k_render0", "primary": "k_primary", "secondary": "k_secondary",
                 "accumulate": "k_accumulate"}
 

@@ -255,13 +255,16 @@ def test_bgra8_encode_matches_oracle():
     assert len(np.unique(bgra)) > 50
 
 
-@pytest.mark.parametrize("spp,chunks,gbuffer", [(2, 2, False), (3, 3, False), (4, 4, False), (5, 4, False),
-                                                (4, 2, False), (4, 4, True), (5, 3, True)])
+@pytest.mark.parametrize("spp,chunks,gbuffer", [(1, 3, False), (2, 2, False), (3, 3, False), (4, 4, False),
+                                                (5, 4, False), (4, 2, False), (4, 4, True), (5, 3, True),
+                                                (1, 4, True)])
 def test_split_frames_equal_one_stream(spp, chunks, gbuffer):
-    """Sample chunks on several streams give the one-stream frame bit for bit (hits, radiance,
-    G-buffer, counters): every chunk accumulates after the previous one, in sample order."""
+    """Chunks of tile rows on several streams give the one-stream frame bit for bit (hits,
+    radiance, G-buffer, counters): a chunk traces every sample of its pixels, summed in sample
+    order; the 100-row frame has 13 rows of tiles (the last one partial), so the chunks are
+    uneven."""
     from vtrace_amd import renderer as R
-    sc = scenes.multi_volume_scene(count=5, n=64, world=512, seed=21, width=160, height=96, spp=spp, max_bounces=3)
+    sc = scenes.multi_volume_scene(count=5, n=64, world=512, seed=21, width=160, height=100, spp=spp, max_bounces=3)
     _say(f"case: split spp={spp} chunks={chunks} gbuffer={gbuffer}")
     r = R.reset_renderer()
     R.load_scene(r, sc, stats=True)
@@ -357,9 +360,9 @@ def test_deep_tree_traversal():
 
 def test_full_size_benchmark_frame():
     """The benchmark workload at its full size (16 x 256^3 instances, 1920x1080, 4 spp, <= 4
-    bounces) on the benchmark's path: the two-chunk (default) and four-chunk frames equal the
-    one-stream frame bit for bit, and bands of rows of it match the oracle (hits bit-exact,
-    radiance within 1e-5)."""
+    bounces) on the benchmark's path: frames of two, three (the default) and four chunks of tile
+    rows equal the one-stream frame bit for bit, and bands of rows of it match the oracle (hits
+    bit-exact, radiance within 1e-5)."""
     from vtrace_amd import renderer as R
     sc = scenes.config2()
     _say("case: full-size benchmark frame 1920x1080 4 spp")
@@ -368,7 +371,7 @@ def test_full_size_benchmark_frame():
     r.lib.vt_set_frame_split(1)
     r.render(sc.P, sc.C)
     h1, rad1 = r.hits().copy(), r.radiance().copy()
-    for chunks in (2, 4):
+    for chunks in (2, 3, 4):
         r.lib.vt_set_frame_split(chunks)
         r.render(sc.P, sc.C)
         assert np.array_equal(r.hits(), h1)

@@ -32,7 +32,7 @@ static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 namespace {
 
 constexpr uint32_t kMaxTextures = 65536;  // lib/common.h:35
-constexpr uint32_t kMaxChunks = 4;         // sample chunks (streams) per frame
+constexpr uint32_t kMaxChunks = 4;         // chunks (streams) per frame
 
 // host staging of vt_render_into's request table (instead of the pinned ring the other
 // entry points allocate on first use); the call waits for its frame before returning
@@ -81,22 +81,18 @@ struct State {
     void* wave_mem = nullptr;   // wavefront path state + queues
     size_t wave_cap = 0;        // pixel samples it holds
     WaveBuffers wave;
-    // frames in sample chunks on several streams: chunk k > 0 on xstream[k] with its own
+    // frames in chunks of tile rows on several streams: chunk k > 0 on xstream[k] with its own
     // wavefront block; ev_chunk[k] marks chunk k accumulated
-    // chunks per path-traced frame (1 = one stream).  Two: on the 1080p 4-spp frame, 2 chunks of
-    // 3 workgroups per CU took 2.88 ms, 2 of 4 per CU 2.89, 4 of 2 per CU 2.91, 3 of 3 2.98, one
-    // stream 3.04 (means over 4-6 runs)
-    uint32_t split = 2;
+    // chunks per path-traced frame (1 = one stream).  Three: on the 1080p 4-spp frame, 3 chunks
+    // of tile rows at 2 workgroups per CU took 2.58 ms, 2 at 3 per CU 2.68, 4 at 2 per CU 2.69,
+    // 3 at 3 per CU 2.70, 2 at 4 per CU 2.72 (chunks of samples, the best split before: 2.79)
+    uint32_t split = 3;
     uint32_t cus = 0;             // compute units (persistent grids are sized per CU)
     hipStream_t xstream[kMaxChunks] = {};
     hipEvent_t ev_chunk[kMaxChunks] = {};
     void* xwave_mem[kMaxChunks] = {};
     size_t xwave_cap[kMaxChunks] = {};
     WaveBuffers xwave[kMaxChunks];
-    float4* acc = nullptr;      // per pixel running sum between the sample chunks
-    size_t acc_cap = 0;
-    float4* gacc = nullptr;     // with the G-buffer: per pixel running G-buffer sum (2 float4)
-    size_t gacc_cap = 0;
     float4* xgsamp[kMaxChunks] = {};  // with the G-buffer: per-sample state of chunks 1..n-1
     size_t xgsamp_cap[kMaxChunks] = {};
     unsigned long long* stats = nullptr;  // VT_MAX_STAGES x VT_STAT_COUNT
@@ -181,11 +177,13 @@ int ensure_wave_buffers() {
     return 0;
 }
 
-// wavefront blocks of chunks 1..n-1 and the per-pixel running sum for split frames
+// wavefront blocks of chunks 1..n-1 of a split frame (chunk k: tile rows [k T / n, (k + 1) T / n)
+// of the T rows of 8 x 8 tiles, every sample)
 int ensure_split_buffers(uint32_t spp, uint32_t n) {
+    const uint32_t tiles_y = (g.params.height + 7) / 8;
     for (uint32_t k = 1; k < n; ++k) {
-        const uint32_t s0 = k * spp / n, s1 = (k + 1) * spp / n;
-        const size_t paths = padded_paths(g.params.width, g.params.height, s1 - s0);
+        const uint32_t rows = 8 * ((k + 1) * tiles_y / n - k * tiles_y / n);
+        const size_t paths = padded_paths(g.params.width, rows, spp);
         if (paths > g.xwave_cap[k] || !g.xwave_mem[k]) {
             hipFree(g.xwave_mem[k]);
             g.xwave_mem[k] = nullptr;
@@ -213,29 +211,6 @@ int ensure_split_buffers(uint32_t spp, uint32_t n) {
             }
             g.xwave[k].gsamp = g.xgsamp[k];
         }
-    }
-    const size_t np = (size_t)g.params.width * g.params.height;
-    if (np > g.acc_cap || !g.acc) {
-        hipFree(g.acc);
-        g.acc = nullptr;
-        g.acc_cap = 0;
-        hipError_t e = hipMalloc(&g.acc, np * sizeof(float4));
-        if (e != hipSuccess) {
-            set_err("accumulation buffer allocation", e);
-            return -1;
-        }
-        g.acc_cap = np;
-    }
-    if (g.gbuf_on && (np > g.gacc_cap || !g.gacc)) {
-        hipFree(g.gacc);
-        g.gacc = nullptr;
-        g.gacc_cap = 0;
-        hipError_t e = hipMalloc(&g.gacc, np * 2 * sizeof(float4));
-        if (e != hipSuccess) {
-            set_err("G-buffer accumulation allocation", e);
-            return -1;
-        }
-        g.gacc_cap = np;
     }
     return 0;
 }
@@ -461,32 +436,38 @@ SceneGPU scene_gpu() {
     return S;
 }
 
-// A path-traced frame of n >= 2 samples runs as up to g.split sample chunks on as many streams
-// (chunk 0 on the library stream, chunk k on xstream[k], each with its own wavefront block),
-// so each chunk's launches fill the GPU while another's drain.  Chunk k accumulates after
-// chunk k-1, keeping the per-pixel sum in sample order; the library stream waits for the last.
+// A path-traced frame runs as n = min(g.split, rows of tiles) chunks of rows of 8 x 8 pixel tiles
+// on as many streams (chunk 0 on the library stream, chunk k on xstream[k], each with its own
+// wavefront block), so each chunk's launches fill the GPU while another's drain.  A chunk traces
+// every sample of its pixels and sums them in sample order, so the frame is the one-stream
+// frame bit for bit; chunk k accumulates after chunk k-1 and the library stream waits for the
+// last.  (Chunks of samples over every pixel were slower: a wave then holds one or two samples
+// of 32-64 pixels instead of all samples of 16, and its rays spread further.)
 int render_split(const FrameGPU& F, const SceneGPU& S, uint32_t n) {
-    const uint32_t spp = F.spp;
+    const uint32_t spp = F.spp ? F.spp : 1u;
+    const uint32_t tiles_y = (F.height + 7) / 8;
     if (ensure_split_buffers(spp, n) != 0) return -1;
     ReqGPU R;
     std::memset(&R, 0, sizeof(R));
     R.F = F;
     R.S = S;
     R.hits = g.hits;
-    R.acc = g.acc;
+    R.acc = nullptr;   // every chunk holds all samples of its pixels: no running sums
     R.rad = reinterpret_cast<float4*>(g.rad);
     R.gbuf = g.gbuf_on ? g.gbuf : nullptr;
-    R.gacc = g.gbuf_on ? g.gacc : nullptr;
+    R.gacc = nullptr;
     R.tile_w = 8;
+    R.s0 = 0;
+    R.s1 = spp;
+    R.final_chunk = 1;
     // persistent workgroups per launch: 4 per CU fill the GPU alone; two chunks in flight take 3
-    // per CU each, 3-4 chunks 2 per CU, so the launches of two chunks share the GPU (measured on
+    // per CU each, 3-4 chunks 2 per CU, so the launches of the chunks share the GPU (measured on
     // the 1080p 4-spp frame)
     const uint32_t blocks = g.cus ? g.cus * (n == 2 ? 3u : 2u) : 0u;
     int rc = 0;
     for (uint32_t k = 0; k < n && rc == 0; ++k) {
-        R.s0 = k * spp / n;
-        R.s1 = (k + 1) * spp / n;
-        R.final_chunk = k + 1 == n ? 1 : 0;
+        R.ty0 = k * tiles_y / n;
+        R.ty1 = (k + 1) * tiles_y / n;
         hipStream_t s = k == 0 ? g.stream : g.xstream[k];
         if (k > 0) hipStreamWaitEvent(s, g.ev0, 0);
         uint8_t kinds[VT_MAX_STAGES];
@@ -504,7 +485,7 @@ int render_frame(const FrameGPU& F) {
     if (F.stats) hipMemsetAsync(g.stats, 0, (size_t)VT_MAX_STAGES * VT_STAT_COUNT * 8, g.stream);
     hipEventRecord(g.ev0, g.stream);
     int rc;
-    const uint32_t n = std::min(g.split, std::min(F.spp, kMaxChunks));
+    const uint32_t n = std::min(g.split, std::min((F.height + 7) / 8, kMaxChunks));
     if (n >= 2 && F.mode >= 1)
         rc = render_split(F, scene_gpu(), n);
     else
@@ -756,8 +737,6 @@ void cleanup(void) {
         if (g.ev_chunk[k]) hipEventDestroy(g.ev_chunk[k]);
         if (g.xstream[k]) hipStreamDestroy(g.xstream[k]);
     }
-    hipFree(g.acc);
-    hipFree(g.gacc);
     for (uint32_t k = 0; k < kMaxChunks; ++k) hipFree(g.xgsamp[k]);
     hipEventDestroy(g.ev0); hipEventDestroy(g.ev1); hipEventDestroy(g.inst_copied); hipEventDestroy(g.depth_ready);
     hipHostFree(g.depth_host);

@@ -1107,11 +1107,13 @@ __device__ __forceinline__ uint32_t find_req(const uint32_t* base, uint32_t n, u
 // tile in Morton order, so a wave's 64 items cover a compact block of 64 / ns pixels (8 x 4 at
 // 2 samples, 4 x 4 at 4) whose rays stay close together.
 struct TileMap {
-    uint32_t tw, th, tiles_x, ns;
+    uint32_t tw, th, tiles_x, ns, t0;   // t0: first tile of the launch (row ty0)
     __device__ __forceinline__ TileMap(const ReqGPU& R) {
         tw = R.tile_w;
         th = 64u / tw;
-        tiles_x = (R.F.width + tw - 1u) / tw;
+        uint32_t ty0, ty1;
+        vt_req_tiles(R, &ty0, &ty1, &tiles_x);
+        t0 = ty0 * tiles_x;
         ns = R.s1 - R.s0;
     }
     // Morton order of an 8 x 8 tile: x in bits 0, 2, 4 and y in bits 1, 3, 5
@@ -1132,7 +1134,7 @@ struct TileMap {
     // local work index -> pixel and sample
     __device__ __forceinline__ void pixel(uint32_t lw, uint32_t s0, uint32_t* px, uint32_t* py, uint32_t* s) const {
         const uint32_t per_tile = 64u * ns;
-        const uint32_t tile = lw / per_tile, rem = lw % per_tile;
+        const uint32_t tile = t0 + lw / per_tile, rem = lw % per_tile;
         *s = s0 + rem % ns;
         uint32_t x, y;
         tile_xy(rem / ns, &x, &y);
@@ -1141,7 +1143,7 @@ struct TileMap {
     }
     // pixel and sample -> local work index
     __device__ __forceinline__ uint32_t index(uint32_t px, uint32_t py, uint32_t s, uint32_t s0) const {
-        const uint32_t tile = (py / th) * tiles_x + px / tw;
+        const uint32_t tile = (py / th) * tiles_x + px / tw - t0;
         return tile * (64u * ns) + tile_pt(px % tw, py % th) * ns + (s - s0);
     }
 };
@@ -1552,10 +1554,10 @@ __device__ void accumulate_pixel(const ReqGPU* __restrict__ reqs, const BatchGPU
     const uint32_t nreq = B->nreq;
     const uint32_t r = find_req(B->pix_base, nreq, t);
     const ReqGPU& Rq = reqs[r];
-    const uint32_t p = t - B->pix_base[r];
     const uint32_t W = Rq.F.width;
-    const uint32_t px = p % W, py = p / W;
     const TileMap tm(Rq);
+    const uint32_t p = t - B->pix_base[r] + tm.t0 / tm.tiles_x * tm.th * W;   // the launch's rows of tiles
+    const uint32_t px = p % W, py = p / W;
     const uint32_t wb = B->work_base[r];
     float acc[3] = {0.0f, 0.0f, 0.0f};
     if (Rq.s0 > 0) {
@@ -1786,12 +1788,15 @@ bool vt_batch_layout(const ReqGPU* reqs, uint32_t n, uint32_t npaths, BatchGPU& 
     uint64_t w = 0, px = 0;
     for (uint32_t k = 0; k < n; ++k) {
         const ReqGPU& R = reqs[k];
-        const uint32_t tw = R.tile_w, th = 64u / R.tile_w;
-        const uint64_t tiles = (uint64_t)((R.F.width + tw - 1) / tw) * ((R.F.height + th - 1) / th);
+        const uint32_t th = 64u / R.tile_w;
+        uint32_t ty0, ty1, tiles_x;
+        vt_req_tiles(R, &ty0, &ty1, &tiles_x);
+        const uint64_t tiles = (uint64_t)tiles_x * (ty1 - ty0);
+        const uint64_t r0 = (uint64_t)ty0 * th, r1 = (uint64_t)ty1 * th < R.F.height ? (uint64_t)ty1 * th : R.F.height;
         B.work_base[k] = (uint32_t)w;
         B.pix_base[k] = (uint32_t)px;
         w += tiles * 64u * (R.s1 - R.s0);
-        px += (uint64_t)R.F.width * R.F.height;
+        px += (uint64_t)R.F.width * (r1 - r0);
         const uint32_t mb = R.F.mode == 2 ? R.F.max_bounces : 0u;
         if (mb > B.max_bounces) B.max_bounces = mb;
     }

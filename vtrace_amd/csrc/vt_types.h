@@ -132,7 +132,20 @@ struct ReqGPU {
     uint32_t s0, s1;        // samples [s0, s1) are traced in this launch
     uint32_t final_chunk;   // 1: s1 == spp
     uint32_t tile_w;        // pixel tile of 64 work items: tile_w x (64 / tile_w)
+    uint32_t ty0, ty1;      // rows of tiles [ty0, ty1) are traced in this launch (ty1 = 0: all)
 };
+
+// The request's rows of tiles [*ty0, *ty1) and its tile columns.
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline void vt_req_tiles(const ReqGPU& R, uint32_t* ty0, uint32_t* ty1, uint32_t* tiles_x) {
+    const uint32_t tw = R.tile_w, th = 64u / R.tile_w;
+    const uint32_t tiles_y = (R.F.height + th - 1u) / th;
+    *tiles_x = (R.F.width + tw - 1u) / tw;
+    *ty1 = R.ty1 ? (R.ty1 < tiles_y ? R.ty1 : tiles_y) : tiles_y;
+    *ty0 = R.ty0 < *ty1 ? R.ty0 : *ty1;
+}
 
 #define VT_MAX_BATCH 64
 // Work and pixel spaces of a batch: request k owns work items [work_base[k], work_base[k+1])
