@@ -102,8 +102,9 @@ int32_t vt_add_textures(const uint8_t* rgba8, uint32_t count, uint32_t width, ui
  * get_input_data_pointer directly, as the reference's key callback does (lib/entry.c:31-52). */
 void vt_set_cursor(double x, double y);
 /* Path-traced frames run as min(chunks, rows of 8 x 8 tiles) chunks of tile rows on as many
- * streams (chunks <= 4, default 3; 0 or 1: one stream).  Results are identical: a chunk traces
- * every sample of its pixels and sums them in sample order (radiance and G-buffer). */
+ * streams (chunks <= 4; 0 or 1: one stream).  Until this is called, frames of 2^21 or more
+ * pixel samples run as 3 chunks and smaller ones on one stream.  Results are identical: a
+ * chunk traces every sample of its pixels and sums them in sample order (radiance, G-buffer). */
 int32_t vt_set_frame_split(uint32_t chunks);
 /* Workgroups per persistent launch of a one-stream frame (0: 4 per CU, the default).  Results
  * are identical for any value: work is fetched from counters and queued per path, and an append

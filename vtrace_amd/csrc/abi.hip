@@ -33,6 +33,7 @@ namespace {
 
 constexpr uint32_t kMaxTextures = 65536;  // lib/common.h:35
 constexpr uint32_t kMaxChunks = 4;         // chunks (streams) per frame
+constexpr uint64_t kSplitMinSamples = 1ull << 21;   // smaller frames: one stream by default
 
 // host staging of vt_render_into's request table (instead of the pinned ring the other
 // entry points allocate on first use); the call waits for its frame before returning
@@ -85,8 +86,11 @@ struct State {
     // wavefront block; ev_chunk[k] marks chunk k accumulated
     // chunks per path-traced frame (1 = one stream).  Three: on the 1080p 4-spp frame, 3 chunks
     // of tile rows at 2 workgroups per CU took 2.58 ms, 2 at 3 per CU 2.68, 4 at 2 per CU 2.69,
-    // 3 at 3 per CU 2.70, 2 at 4 per CU 2.72 (chunks of samples, the best split before: 2.79)
+    // 3 at 3 per CU 2.70, 2 at 4 per CU 2.72 (chunks of samples, the best split before: 2.79).
+    // Until vt_set_frame_split sets a count, frames under kSplitMinSamples pixel samples run on
+    // one stream (640x480 at 1 spp: 0.22 ms on one stream, 0.59 ms in three chunks).
     uint32_t split = 3;
+    bool split_auto = true;
     uint32_t cus = 0;             // compute units (persistent grids are sized per CU)
     hipStream_t xstream[kMaxChunks] = {};
     hipEvent_t ev_chunk[kMaxChunks] = {};
@@ -485,7 +489,9 @@ int render_frame(const FrameGPU& F) {
     if (F.stats) hipMemsetAsync(g.stats, 0, (size_t)VT_MAX_STAGES * VT_STAT_COUNT * 8, g.stream);
     hipEventRecord(g.ev0, g.stream);
     int rc;
-    const uint32_t n = std::min(g.split, std::min((F.height + 7) / 8, kMaxChunks));
+    const uint64_t samples = (uint64_t)F.width * F.height * (F.spp ? F.spp : 1u);
+    const uint32_t want = g.split_auto && samples < kSplitMinSamples ? 1u : g.split;
+    const uint32_t n = std::min(want, std::min((F.height + 7) / 8, kMaxChunks));
     if (n >= 2 && F.mode >= 1)
         rc = render_split(F, scene_gpu(), n);
     else
@@ -975,6 +981,7 @@ int32_t vt_set_frame_split(uint32_t chunks) {
     if (!g.inited || chunks > kMaxChunks) return -1;
     hipStreamSynchronize(g.stream);
     g.split = chunks ? chunks : 1;
+    g.split_auto = false;
     return 0;
 }
 
