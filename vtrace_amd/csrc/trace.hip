@@ -1102,10 +1102,11 @@ __device__ __forceinline__ uint32_t find_req(const uint32_t* base, uint32_t n, u
     return lo;
 }
 
-// Work items of a request: tiles of 64 pixels (tw x th) in row-major order, each tile's
-// 64 * ns items its pixels' samples, a pixel's ns samples side by side; the pixels of an 8 x 8
-// tile in Morton order, so a wave's 64 items cover a compact block of 64 / ns pixels (8 x 4 at
-// 2 samples, 4 x 4 at 4) whose rays stay close together.
+// Work items of a request's launch: the tiles of 64 pixels (tw x th) in its rows of tiles
+// [ty0, ty1), row-major, each tile's 64 * ns items its pixels' samples [s0, s1), a pixel's ns
+// samples side by side; the pixels of an 8 x 8 tile in Morton order, so a wave's 64 items cover
+// a compact block of 64 / ns pixels (8 x 4 at 2 samples, 4 x 4 at 4) whose rays stay close
+// together.
 struct TileMap {
     uint32_t tw, th, tiles_x, ns, t0;   // t0: first tile of the launch (row ty0)
     __device__ __forceinline__ TileMap(const ReqGPU& R) {
@@ -1556,7 +1557,8 @@ __device__ void accumulate_pixel(const ReqGPU* __restrict__ reqs, const BatchGPU
     const ReqGPU& Rq = reqs[r];
     const uint32_t W = Rq.F.width;
     const TileMap tm(Rq);
-    const uint32_t p = t - B->pix_base[r] + tm.t0 / tm.tiles_x * tm.th * W;   // the launch's rows of tiles
+    // p: the pixel's index in the frame (the launch's pixels start at its first row of tiles)
+    const uint32_t p = t - B->pix_base[r] + tm.t0 / tm.tiles_x * tm.th * W;
     const uint32_t px = p % W, py = p / W;
     const uint32_t wb = B->work_base[r];
     float acc[3] = {0.0f, 0.0f, 0.0f};
